@@ -1,0 +1,335 @@
+"""fipi_amd: MI355X-native FIPI time step (Cahn-Hilliard + Stokes +
+interface-trapped particles) behind the C ABI of include/fipi.h.
+
+This module is the Python mirror of that ABI (ctypes), used by the tests
+and the benchmark.  The compute path is libfipi.so (HIP kernels for
+gfx950); there is no CPU fallback: every call fails loudly when the library
+or a GPU is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .params import FP32_MIXED, FP64, Params, config_params, reference_params
+
+__all__ = [
+    "FP64", "FP32_MIXED", "Params", "config_params", "reference_params",
+    "Simulation", "FipiError", "lib", "LIB_PATH", "init_phi", "init_rsa", "init_uniform",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfipi.so")
+
+ERRORS = {
+    -1: "FIPI_ERR_ARG", -2: "FIPI_ERR_HIP", -3: "FIPI_ERR_UNSUPPORTED",
+    -4: "FIPI_ERR_RANGE", -5: "FIPI_ERR_STATE", -6: "FIPI_ERR_OVERFLOW",
+}
+
+# names exported by libfipi.so (declared in include/fipi.h)
+ABI_SYMBOLS = [
+    "fipi_create", "fipi_destroy", "fipi_last_error", "fipi_stream",
+    "fipi_set_state", "fipi_set_velocity", "fipi_get_phi", "fipi_get_velocity",
+    "fipi_get_grad_phi", "fipi_get_particles", "fipi_get_time", "fipi_diagnostics",
+    "fipi_step", "fipi_step_async", "fipi_sync",
+    "fipi_fft_r2c", "fipi_fft_c2r", "fipi_cells_build", "fipi_contact_pairs",
+    "fipi_contact_forces", "fipi_interp", "fipi_spread",
+    "fipi_init_phi", "fipi_init_rsa", "fipi_init_uniform",
+]
+
+
+class FipiError(RuntimeError):
+    pass
+
+
+class CParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("n_particles", C.c_int32),
+        ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+        ("pi", C.c_double), ("dt", C.c_double), ("ramp_t", C.c_double),
+        ("epsilon", C.c_double), ("lambda_", C.c_double), ("mobility", C.c_double),
+        ("mu", C.c_double), ("sigma", C.c_double), ("a_pi", C.c_double),
+        ("k_n", C.c_double), ("r_p", C.c_double), ("r_eq", C.c_double),
+        ("r_c", C.c_double), ("margin", C.c_double),
+        ("rho1", C.c_double), ("rho2", C.c_double), ("gravity", C.c_double),
+        ("tanh2", C.c_double),
+        ("switch_particle", C.c_int32), ("switch_ch", C.c_int32), ("switch_ns", C.c_int32),
+        ("precision", C.c_int32),
+    ]
+
+
+class CDiag(C.Structure):
+    _fields_ = [
+        ("t", C.c_int32), ("it_mark", C.c_int32), ("mean_phi", C.c_double),
+        ("free_energy", C.c_double), ("interface_area", C.c_double), ("tau", C.c_double),
+    ]
+
+
+def _to_cparams(p: Params) -> CParams:
+    s = CParams()
+    for name, _ in CParams._fields_:
+        setattr(s, name, getattr(p, "lam" if name == "lambda_" else name))
+    return s
+
+
+_lib = None
+
+
+def lib():
+    """Load libfipi.so (built in-tree by fipi_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FipiError(f"{LIB_PATH} is missing: run `python -m fipi_amd.build` first")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    i64 = C.c_int64
+    L.fipi_create.argtypes = [C.POINTER(CParams), C.POINTER(vp)]
+    L.fipi_destroy.argtypes = [vp]
+    L.fipi_last_error.argtypes = [vp]
+    L.fipi_last_error.restype = C.c_char_p
+    L.fipi_stream.argtypes = [vp]
+    L.fipi_stream.restype = vp
+    L.fipi_set_state.argtypes = [vp, dp, dp, dp, dp]
+    L.fipi_set_velocity.argtypes = [vp, dp]
+    for nm in ("fipi_get_phi", "fipi_get_velocity", "fipi_get_grad_phi"):
+        getattr(L, nm).argtypes = [vp, dp]
+    L.fipi_get_particles.argtypes = [vp, dp, dp, dp, dp, dp, dp]
+    L.fipi_get_time.argtypes = [vp, ip]
+    L.fipi_diagnostics.argtypes = [vp, C.POINTER(CDiag)]
+    L.fipi_step.argtypes = [vp, C.c_int32]
+    L.fipi_step_async.argtypes = [vp, C.c_int32]
+    L.fipi_sync.argtypes = [vp]
+    L.fipi_fft_r2c.argtypes = [vp, vp, vp]
+    L.fipi_fft_c2r.argtypes = [vp, vp, vp]
+    L.fipi_cells_build.argtypes = [vp, vp, C.c_int32, vp, vp, ip]
+    L.fipi_contact_pairs.argtypes = [vp, vp, C.c_int32, vp, i64, C.POINTER(i64)]
+    L.fipi_contact_forces.argtypes = [vp, vp, C.c_int32, C.c_int32, vp]
+    L.fipi_interp.argtypes = [vp, vp, vp, C.c_int32, vp]
+    L.fipi_spread.argtypes = [vp, vp, vp, C.c_int32, vp]
+    L.fipi_init_phi.argtypes = [i64, C.c_double, C.c_double, C.c_uint64, dp]
+    L.fipi_init_rsa.argtypes = [i64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_uint64, i64, dp]
+    L.fipi_init_rsa.restype = i64
+    L.fipi_init_uniform.argtypes = [i64, C.c_double, C.c_double, C.c_uint64, dp]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(rc, h=None):
+    if rc != 0:
+        msg = lib().fipi_last_error(h)
+        raise FipiError(f"{ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+
+# ---- seeded synthetic inputs (host only) ---------------------------------
+def init_phi(shape, mean: float, amp: float, seed: int) -> np.ndarray:
+    out = np.empty(int(np.prod(shape)), np.float64)
+    _check(lib().fipi_init_phi(out.size, mean, amp, seed, _dp(out)))
+    return out.reshape(shape)
+
+
+def init_rsa(n: int, r: float, gap: float, box, seed: int, max_attempts: int = 100000) -> np.ndarray:
+    out = np.zeros((max(n, 1), 3), np.float64)
+    placed = lib().fipi_init_rsa(n, r, gap, box[0], box[1], box[2], seed, max_attempts, _dp(out))
+    if placed < 0:
+        _check(int(placed))
+    return out[:placed].copy()
+
+
+def init_uniform(n: int, lo: float, hi: float, seed: int) -> np.ndarray:
+    out = np.empty(n, np.float64)
+    _check(lib().fipi_init_uniform(n, lo, hi, seed, _dp(out)))
+    return out
+
+
+class Simulation:
+    """One periodic box on the current GPU (a fipi_ctx)."""
+
+    def __init__(self, params: Params):
+        self.params = params
+        self._h = C.c_void_p()
+        self._cp = _to_cparams(params)
+        _check(lib().fipi_create(C.byref(self._cp), C.byref(self._h)))
+        self.n = params.n_particles if params.switch_particle else 0
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().fipi_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        _check(rc, self._h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_state(self, phi, pos=None, radius=None, cos_theta=None):
+        p = self.params
+        phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+        assert phi.size == p.G
+        if self.n:
+            pos = np.ascontiguousarray(pos, dtype=np.float64)
+            assert pos.shape == (self.n, 3)
+        r = None if radius is None else np.ascontiguousarray(radius, dtype=np.float64)
+        ct = None if cos_theta is None else np.ascontiguousarray(cos_theta, dtype=np.float64)
+        self._c(lib().fipi_set_state(self._h, _dp(phi), _dp(pos) if self.n else None, _dp(r), _dp(ct)))
+
+    def set_velocity(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        assert u.size == 3 * self.params.G
+        self._c(lib().fipi_set_velocity(self._h, _dp(u)))
+
+    def step(self, n: int = 1):
+        self._c(lib().fipi_step(self._h, n))
+
+    def step_async(self, n: int = 1):
+        self._c(lib().fipi_step_async(self._h, n))
+
+    def sync(self):
+        self._c(lib().fipi_sync(self._h))
+
+    def phi(self) -> np.ndarray:
+        p = self.params
+        out = np.empty((p.nz, p.ny, p.nx))
+        self._c(lib().fipi_get_phi(self._h, _dp(out)))
+        return out
+
+    def velocity(self) -> np.ndarray:
+        p = self.params
+        out = np.empty((3, p.nz, p.ny, p.nx))
+        self._c(lib().fipi_get_velocity(self._h, _dp(out)))
+        return out
+
+    def grad_phi(self) -> np.ndarray:
+        p = self.params
+        out = np.empty((3, p.nz, p.ny, p.nx))
+        self._c(lib().fipi_get_grad_phi(self._h, _dp(out)))
+        return out
+
+    def particles(self) -> dict:
+        n = max(self.n, 1)
+        arrs = [np.zeros((n, 3)) for _ in range(6)]
+        self._c(lib().fipi_get_particles(self._h, *[_dp(a) for a in arrs]))
+        keys = ["p", "p_old", "u", "u_old", "f_pp", "f_pi"]
+        return {k: a[: self.n] for k, a in zip(keys, arrs)}
+
+    @property
+    def t(self) -> int:
+        v = C.c_int32()
+        self._c(lib().fipi_get_time(self._h, C.byref(v)))
+        return v.value
+
+    def diagnostics(self) -> dict:
+        d = CDiag()
+        self._c(lib().fipi_diagnostics(self._h, C.byref(d)))
+        return {k: getattr(d, k) for k, _ in CDiag._fields_}
+
+    # ---- unit entry points on device memory (torch tensors) --------------
+    # torch only provides device memory here; the work runs in libfipi.
+    def _sync_torch(self):
+        import torch
+        torch.cuda.synchronize()
+
+    def fft_r2c(self, real):
+        import torch
+        p = self.params
+        dt = torch.float64 if p.precision == FP64 else torch.float32
+        assert real.dtype == dt and real.is_contiguous() and real.numel() == p.G
+        out = torch.empty((p.nz, p.ny, p.nx // 2 + 1, 2), dtype=dt, device=real.device)
+        self._sync_torch()
+        self._c(lib().fipi_fft_r2c(self._h, C.c_void_p(real.data_ptr()), C.c_void_p(out.data_ptr())))
+        return out
+
+    def fft_c2r(self, half):
+        import torch
+        p = self.params
+        half = half.clone()
+        out = torch.empty((p.nz, p.ny, p.nx), dtype=half.dtype, device=half.device)
+        self._sync_torch()
+        self._c(lib().fipi_fft_c2r(self._h, C.c_void_p(half.data_ptr()), C.c_void_p(out.data_ptr())))
+        return out
+
+    def cells_build(self, pos):
+        import torch
+        pos = torch.as_tensor(np.ascontiguousarray(pos, dtype=np.float64)).cuda()
+        n = pos.shape[0]
+        nc, _ = self.params.cell_grid()
+        ncell = nc[0] * nc[1] * nc[2]
+        start = torch.zeros(ncell + 1, dtype=torch.int32, device=pos.device)
+        ids = torch.zeros(max(n, 1), dtype=torch.int32, device=pos.device)
+        ncell_out = C.c_int32()
+        self._sync_torch()
+        self._c(lib().fipi_cells_build(self._h, C.c_void_p(pos.data_ptr()), n,
+                                       C.c_void_p(start.data_ptr()), C.c_void_p(ids.data_ptr()),
+                                       C.byref(ncell_out)))
+        assert ncell_out.value == ncell
+        return start.cpu().numpy(), ids[:n].cpu().numpy()
+
+    def contact_pairs(self, pos):
+        import torch
+        pos = torch.as_tensor(np.ascontiguousarray(pos, dtype=np.float64)).cuda()
+        n = pos.shape[0]
+        cnt = C.c_int64()
+        self._sync_torch()
+        self._c(lib().fipi_contact_pairs(self._h, C.c_void_p(pos.data_ptr()), n, None, 0, C.byref(cnt)))
+        m = cnt.value
+        pairs = torch.zeros((max(m, 1), 2), dtype=torch.int32, device=pos.device)
+        self._c(lib().fipi_contact_pairs(self._h, C.c_void_p(pos.data_ptr()), n,
+                                         C.c_void_p(pairs.data_ptr()), m, C.byref(cnt)))
+        assert cnt.value == m
+        return pairs[:m].cpu().numpy()
+
+    def contact_forces(self, pos, t: int):
+        import torch
+        pos = torch.as_tensor(np.ascontiguousarray(pos, dtype=np.float64)).cuda()
+        n = pos.shape[0]
+        out = torch.zeros((max(n, 1), 3), dtype=torch.float64, device=pos.device)
+        self._sync_torch()
+        self._c(lib().fipi_contact_forces(self._h, C.c_void_p(pos.data_ptr()), n, t,
+                                          C.c_void_p(out.data_ptr())))
+        return out[:n].cpu().numpy()
+
+    def interp(self, grid, pos):
+        import torch
+        p = self.params
+        dt = torch.float64 if p.precision == FP64 else torch.float32
+        g = torch.as_tensor(np.ascontiguousarray(grid)).to(dt).cuda().contiguous()
+        pos = torch.as_tensor(np.ascontiguousarray(pos, dtype=np.float64)).cuda()
+        n = pos.shape[0]
+        out = torch.zeros(max(n, 1), dtype=torch.float64, device=pos.device)
+        self._sync_torch()
+        self._c(lib().fipi_interp(self._h, C.c_void_p(g.data_ptr()), C.c_void_p(pos.data_ptr()), n,
+                                  C.c_void_p(out.data_ptr())))
+        return out[:n].cpu().numpy()
+
+    def spread(self, pos, force):
+        import torch
+        p = self.params
+        dt = torch.float64 if p.precision == FP64 else torch.float32
+        pos = torch.as_tensor(np.ascontiguousarray(pos, dtype=np.float64)).cuda()
+        f = torch.as_tensor(np.ascontiguousarray(force, dtype=np.float64)).cuda()
+        n = pos.shape[0]
+        grid = torch.empty((3, p.nz, p.ny, p.nx), dtype=dt, device=pos.device)
+        self._sync_torch()
+        self._c(lib().fipi_spread(self._h, C.c_void_p(pos.data_ptr()), C.c_void_p(f.data_ptr()), n,
+                                  C.c_void_p(grid.data_ptr())))
+        return grid.cpu().numpy()

@@ -1,0 +1,188 @@
+// Device FFT building blocks for gfx950 (wave64, 160 KB LDS per CU).
+//
+// A length-N line transform is split N = N1 * N2 ("four-step" with one LDS
+// transpose).  Index n = n1 + N1*n2 on input, k = k2 + N2*k1 on output:
+//
+//   X[k2 + N2 k1] = sum_n1 W_N^{n1 k2} W_N1^{n1 k1} sum_n2 x[n1 + N1 n2] W_N2^{n2 k2}
+//
+// Stage A: thread n1 (< N1) holds x[n1 + N1*n2] for n2 < N2 in registers,
+//          runs a DFT-N2, multiplies by W_N^{n1 k2}, writes to LDS.
+// Stage B: thread k2 (< N2) reads the N1 values of its column, runs a
+//          DFT-N1 and holds X[k2 + N2*k1] for k1 < N1.
+// The output "distribution" (thread t holds elements t + N2*m) is the input
+// distribution of a transform split the other way round (N1' = N2), so a
+// forward transform, a pointwise k-space operation and an inverse transform
+// chain in registers with exactly one LDS transpose per transform.  Loads
+// and stores in this distribution are coalesced when consecutive threads
+// take consecutive t (contiguous lines) or when 16 neighbouring lines are
+// processed side by side (strided lines, 16-wide tiles).
+//
+// Twiddles W_N^t come from a host-computed table (long-double accurate).
+// The small DFTs use the exact constants cos/sin(2 pi j/16).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fipi {
+
+template <typename T> struct vec2;
+template <> struct vec2<double> { using type = double2; };
+template <> struct vec2<float> { using type = float2; };
+template <typename T> using cplx = typename vec2<T>::type;
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+{
+    return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// multiply by SIGN * i
+template <int SIGN, typename C> __device__ __forceinline__ C mul_si(C z)
+{
+    if constexpr (SIGN < 0) return C{z.y, -z.x};
+    else return C{-z.y, z.x};
+}
+// multiply by (c, SIGN*s)
+template <int SIGN, typename C, typename T> __device__ __forceinline__ C mul_cs(C z, T c, T s)
+{
+    const T ss = SIGN < 0 ? -s : s;
+    return C{z.x * c - z.y * ss, z.x * ss + z.y * c};
+}
+
+// cos/sin(2 pi j / 16), j = 0..15
+struct W16 {
+    static constexpr double C1 = 0.92387953251128675612818318939678828682;  // cos(pi/8)
+    static constexpr double S1 = 0.38268343236508977172845998403039886676;  // sin(pi/8)
+    static constexpr double R2 = 0.70710678118654752440084436210484903928;  // sqrt(1/2)
+};
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft2(C &a, C &b)
+{
+    C t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft4(C &a, C &b, C &c, C &d)
+{
+    C t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = mul_si<SIGN>(csub(b, d));
+    a = cadd(t0, t2);
+    c = csub(t0, t2);
+    b = cadd(t1, t3);
+    d = csub(t1, t3);
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft8(C *v)
+{
+    using T = decltype(v[0].x);
+    C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    C o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<SIGN>(e0, e1, e2, e3);
+    dft4<SIGN>(o0, o1, o2, o3);
+    const T r = (T)W16::R2;
+    o1 = mul_cs<SIGN>(o1, r, r);
+    o2 = mul_si<SIGN>(o2);
+    o3 = mul_cs<SIGN>(o3, -r, r);
+    v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+}
+
+template <int SIGN, typename C> __device__ __forceinline__ void dft16(C *v)
+{
+    using T = decltype(v[0].x);
+    // n = 4 n1 + n2; stage 1 over n1 for each n2
+    C a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; n2++) {
+        C p = v[n2], q = v[4 + n2], r = v[8 + n2], s = v[12 + n2];
+        dft4<SIGN>(p, q, r, s);
+        a[n2][0] = p; a[n2][1] = q; a[n2][2] = r; a[n2][3] = s;
+    }
+    const T c1 = (T)W16::C1, s1 = (T)W16::S1, r2 = (T)W16::R2;
+    // twiddles W16^{n2 k1}
+    a[1][1] = mul_cs<SIGN>(a[1][1], c1, s1);
+    a[1][2] = mul_cs<SIGN>(a[1][2], r2, r2);
+    a[1][3] = mul_cs<SIGN>(a[1][3], s1, c1);
+    a[2][1] = mul_cs<SIGN>(a[2][1], r2, r2);
+    a[2][2] = mul_si<SIGN>(a[2][2]);
+    a[2][3] = mul_cs<SIGN>(a[2][3], -r2, r2);
+    a[3][1] = mul_cs<SIGN>(a[3][1], s1, c1);
+    a[3][2] = mul_cs<SIGN>(a[3][2], -r2, r2);
+    a[3][3] = mul_cs<SIGN>(a[3][3], -c1, -s1);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        C p = a[0][k1], q = a[1][k1], r = a[2][k1], s = a[3][k1];
+        dft4<SIGN>(p, q, r, s);
+        v[k1] = p; v[k1 + 4] = q; v[k1 + 8] = r; v[k1 + 12] = s;
+    }
+}
+
+template <int R, int SIGN, typename C> __device__ __forceinline__ void dft(C *v)
+{
+    if constexpr (R == 1) {
+    } else if constexpr (R == 2) {
+        dft2<SIGN>(v[0], v[1]);
+    } else if constexpr (R == 4) {
+        dft4<SIGN>(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 8) {
+        dft8<SIGN>(v);
+    } else if constexpr (R == 16) {
+        dft16<SIGN>(v);
+    } else {
+        static_assert(R == 16, "unsupported radix");
+    }
+}
+
+// Factorisation of each supported line length: N = N1 * N2.
+template <int N> struct Split;
+template <> struct Split<16> { static constexpr int N1 = 4, N2 = 4; };
+template <> struct Split<32> { static constexpr int N1 = 8, N2 = 4; };
+template <> struct Split<64> { static constexpr int N1 = 8, N2 = 8; };
+template <> struct Split<128> { static constexpr int N1 = 16, N2 = 8; };
+template <> struct Split<256> { static constexpr int N1 = 16, N2 = 16; };
+
+template <int A, int B> struct cmax { static constexpr int v = A > B ? A : B; };
+
+// Threads per line and register slots per thread for a length-N line.
+template <int N> struct LineShape {
+    static constexpr int N1 = Split<N>::N1;
+    static constexpr int N2 = Split<N>::N2;
+    static constexpr int P = cmax<N1, N2>::v;      // threads per line
+    static constexpr int M = cmax<N1, N2>::v;      // register slots per thread
+    // complex LDS slots per line: padded transpose buffer for either split
+    static constexpr int LDS = cmax<N1 * (N2 + 1), N2 * (N1 + 1)>::v;
+    static_assert(LDS >= N, "line buffer must hold a natural-order line");
+};
+
+// Two-factor transform with split (A, B) = (first-stage thread count,
+// second-stage thread count), A*B = N.  Input: thread t < A holds
+// v[m] = x[t + A*m], m < B.  Output: thread t < B holds v[m] = X[t + B*m],
+// m < A.  tw[j] = exp(-2 pi i j / N) (forward table; conjugated for SIGN>0).
+// Must be called by every thread of the block (contains barriers).
+template <typename T, int N, int A, int SIGN>
+__device__ __forceinline__ void fft_line(cplx<T> *v, cplx<T> *lds, int t, const cplx<T> *__restrict__ tw)
+{
+    constexpr int B = N / A;
+    using C = cplx<T>;
+    if (t < A) {
+        dft<B, SIGN>(v);
+#pragma unroll
+        for (int k2 = 1; k2 < B; k2++) {
+            C w = tw[t * k2];
+            if constexpr (SIGN > 0) w.y = -w.y;
+            v[k2] = cmul(v[k2], w);
+        }
+#pragma unroll
+        for (int k2 = 0; k2 < B; k2++) lds[t * (B + 1) + k2] = v[k2];
+    }
+    __syncthreads();
+    if (t < B) {
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++) v[n1] = lds[n1 * (B + 1) + t];
+        dft<A, SIGN>(v);
+    }
+    __syncthreads();
+}
+
+}  // namespace fipi

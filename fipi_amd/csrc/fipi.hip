@@ -1,0 +1,1079 @@
+// libfipi: host side of the MI355X FIPI time step behind the C ABI of
+// include/fipi.h.  One context = one periodic box on the current HIP device
+// with one stream; all state lives in device memory.
+//
+// Per step (reference src/main.cpp:75-222), with R = real grid, H = half
+// spectrum scratch, K = persistent spectra:
+//   x fwd   conv = u.grad c          R(6)  -> H0              (main.cpp:91-92)
+//   y fwd                            H0    -> H0
+//   z hub   CH SBDF2 update          H0,K  -> K(c,hist), H0,H1 (main.cpp:95-140)
+//   y inv   c, dy c, dz c            H0,H1 -> H0,H2,H1
+//   x inv   c, grad c; f(c) fwd      H     -> R(4), H0        (main.cpp:153,160-162)
+//   y fwd                            H0    -> H0
+//   z hub   f_hat, mu_hat            H0,K  -> K(f), H0         (main.cpp:154-158)
+//   y inv                            H0    -> H0
+//   x inv   mu; F = mu grad c + spread + buoyancy; fwd        (main.cpp:164-175)
+//   y fwd, z hub Stokes, y inv, x inv -> u                     (main.cpp:177-179)
+//   two Heun particle stages                                   (main.cpp:198-217)
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/fipi.h"
+#include "field_kernels.hpp"
+#include "particle_kernels.hpp"
+
+using namespace fipi;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F>
+bool with_n(int n, F &&f)
+{
+    switch (n) {
+    case 16: f(std::integral_constant<int, 16>{}); return true;
+    case 32: f(std::integral_constant<int, 32>{}); return true;
+    case 64: f(std::integral_constant<int, 64>{}); return true;
+    case 128: f(std::integral_constant<int, 128>{}); return true;
+    case 256: f(std::integral_constant<int, 256>{}); return true;
+    default: return false;
+    }
+}
+
+bool supported_n(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256; }
+
+}  // namespace
+
+struct fipi_ctx {
+    fipi_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool fp64 = true;
+    bool has_state = false;
+    int t = 0;
+    Dims d{};
+    size_t G = 0, GH = 0;
+    size_t esz = 8;  // bytes per real element
+    // field state
+    void *c = nullptr, *g[3] = {}, *u[3] = {};
+    void *h[3] = {};
+    void *ck[2] = {}, *fk = nullptr, *hist = nullptr;
+    int cur = 0;
+    double *kx = nullptr, *ky = nullptr, *kz = nullptr;
+    void *twx = nullptr, *twy = nullptr, *twz = nullptr;
+    // particles
+    int n = 0;
+    PartParams pp{};
+    PartState ps{};
+    int ncell = 0;
+    int *cell_cnt = nullptr, *cell_start = nullptr, *cell_ids = nullptr, *cell_cursor = nullptr;
+    int nlines = 0;
+    int *line_cnt = nullptr, *line_start = nullptr, *line_cursor = nullptr;
+    void *spr = nullptr;
+    int *d_flags = nullptr;  // [0] error bits, [1] contact-order overflow
+    int *h_flags = nullptr;  // pinned
+    double *d_red = nullptr; // reduction scratch
+    std::vector<void *> allocs;
+};
+
+namespace {
+
+int set_err(fipi_ctx *ctx, int code, const std::string &msg)
+{
+    if (ctx) ctx->err = msg;
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                           \
+    do {                                                                                            \
+        hipError_t e_ = (call);                                                                     \
+        if (e_ != hipSuccess)                                                                       \
+            return set_err(ctx, FIPI_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+int dalloc(fipi_ctx *ctx, void **ptr, size_t bytes)
+{
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) return set_err(ctx, FIPI_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    ctx->allocs.push_back(*ptr);
+    return FIPI_OK;
+}
+
+// Wave numbers, src/field.cpp:108-115: i*2.0*pi/L with i <= N/2 kept positive.
+std::vector<double> wave_numbers(int n, double pi, double L)
+{
+    std::vector<double> k(n);
+    for (int i = 0; i < n; i++) k[i] = (i <= n / 2 ? i : i - n) * 2.0 * pi / L;
+    return k;
+}
+
+// Forward twiddles exp(-2 pi i j / N), j < N, long-double accurate.
+template <typename T>
+std::vector<T> twiddles(int n)
+{
+    std::vector<T> w(2 * n);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int j = 0; j < n; j++) {
+        const long double a = two_pi * (long double)j / (long double)n;
+        w[2 * j] = (T)cosl(a);
+        w[2 * j + 1] = (T)(-sinl(a));
+    }
+    return w;
+}
+
+int threads_per_line(int n)
+{
+    int r = 0;
+    with_n(n, [&](auto NC) { r = LineShape<decltype(NC)::value>::P; });
+    return r;
+}
+
+// at least one block: every kernel bounds-checks its index
+inline dim3 grid1(size_t n, int b) { return dim3((unsigned)(n == 0 ? 1 : (n + b - 1) / b)); }
+
+// ---------------------------------------------------------------------------
+// Templated launch sequences
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Impl {
+    using C = cplx<T>;
+
+    static T *R(void *p) { return static_cast<T *>(p); }
+    static C *H(void *p) { return static_cast<C *>(p); }
+
+    static KParams kparams(const fipi_ctx *ctx)
+    {
+        KParams kp;
+        kp.dt = ctx->p.dt;
+        kp.mobility = ctx->p.mobility;
+        kp.lambda = ctx->p.lambda;
+        kp.mu = ctx->p.mu;
+        kp.inv_g = 1.0 / (double)((int64_t)ctx->p.nx * ctx->p.ny * ctx->p.nz);
+        kp.first = ctx->t == 0;
+        kp.use_conv = ctx->p.switch_ns != 0;
+        return kp;
+    }
+
+    static T fscale(const fipi_ctx *ctx) { return (T)(ctx->p.lambda / (ctx->p.epsilon * ctx->p.epsilon)); }
+
+    template <int NF, class Prod>
+    static void x_forward(fipi_ctx *ctx, Prod prod, C *o0, C *o1)
+    {
+        const Dims d = ctx->d;
+        with_n(d.nx, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            hipLaunchKernelGGL((k_x_forward<T, N, NF, Prod>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
+                               dim3(TILE * LineShape<N>::P), 0, ctx->stream, prod, o0, o1, d,
+                               (const C *)ctx->twx);
+        });
+    }
+
+    template <int NF>
+    static void x_inverse(fipi_ctx *ctx, const C *i0, const C *i1, const C *i2, T *o0, T *o1, T *o2)
+    {
+        const Dims d = ctx->d;
+        with_n(d.nx, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            hipLaunchKernelGGL((k_x_inverse<T, N, NF>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
+                               dim3(TILE * LineShape<N>::P), 0, ctx->stream, i0, i1, i2, o0, o1, o2, d,
+                               (const C *)ctx->twx);
+        });
+    }
+
+    template <int SIGN, int NF, bool GRAD>
+    static void y_pass(fipi_ctx *ctx, C *a0, C *a1, C *a2, C *extra)
+    {
+        const Dims d = ctx->d;
+        with_n(d.ny, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD>), dim3((unsigned)((d.nxh + TILE - 1) / TILE), d.nz),
+                               dim3(TILE * LineShape<N>::P), 0, ctx->stream, a0, a1, a2, extra, d, ctx->ky,
+                               (const C *)ctx->twy);
+        });
+    }
+
+    static dim3 zgrid(const fipi_ctx *ctx)
+    {
+        return dim3((unsigned)((ctx->d.nxh + TILE - 1) / TILE), ctx->d.ny);
+    }
+
+    template <int SIGN>
+    static void z_pass(fipi_ctx *ctx, C *a)
+    {
+        const Dims d = ctx->d;
+        with_n(d.nz, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            hipLaunchKernelGGL((k_z<T, N, SIGN>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, ctx->stream, a, d,
+                               (const C *)ctx->twz);
+        });
+    }
+
+    // c^0 -> c_hat^0, f_hat^0, grad c^0 (see set_state)
+    static void init_fields(fipi_ctx *ctx)
+    {
+        const Dims d = ctx->d;
+        ProdCF<T> pr{R(ctx->c), fscale(ctx)};
+        x_forward<2>(ctx, pr, H(ctx->h[0]), H(ctx->h[1]));
+        y_pass<-1, 2, false>(ctx, H(ctx->h[0]), H(ctx->h[1]), nullptr, nullptr);
+        const KParams kp = kparams(ctx);
+        with_n(d.nz, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            hipLaunchKernelGGL((k_hub_init<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, ctx->stream,
+                               H(ctx->h[0]), H(ctx->h[1]), H(ctx->ck[ctx->cur]), H(ctx->fk), kp, d, ctx->kz,
+                               (const C *)ctx->twz);
+        });
+        y_pass<+1, 2, true>(ctx, H(ctx->h[0]), H(ctx->h[1]), nullptr, H(ctx->h[2]));
+        cgrad_f(ctx);
+    }
+
+    static void cgrad_f(fipi_ctx *ctx)
+    {
+        const Dims d = ctx->d;
+        with_n(d.nx, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            hipLaunchKernelGGL((k_x_cgrad_f<T, N>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
+                               dim3(TILE * LineShape<N>::P), 0, ctx->stream, H(ctx->h[0]), H(ctx->h[2]),
+                               H(ctx->h[1]), R(ctx->c), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), H(ctx->h[0]),
+                               fscale(ctx), d, ctx->kx, (const C *)ctx->twx);
+        });
+    }
+
+    static void spread_build(fipi_ctx *ctx, const PartParams &pp, const double *px, const double *py,
+                             const double *pz, const double *f0, const double *f1, const double *f2)
+    {
+        hipStream_t st = ctx->stream;
+        const int nl = ctx->nlines;
+        hipLaunchKernelGGL(k_fill_int, grid1(nl, 256), dim3(256), 0, st, ctx->line_cnt, nl, 0);
+        hipLaunchKernelGGL(k_spread_count, grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, ctx->line_cnt,
+                           ctx->d_flags);
+        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, ctx->line_cnt, ctx->line_start, nl,
+                           (const double *)nullptr, 0, (int *)nullptr);
+        hipLaunchKernelGGL(k_fill_int, grid1(nl, 256), dim3(256), 0, st, ctx->line_cursor, nl, 0);
+        hipLaunchKernelGGL((k_spread_scatter<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, f0, f1, f2,
+                           ctx->line_start, ctx->line_cursor, (SpreadEntry<T> *)ctx->spr);
+        hipLaunchKernelGGL((k_spread_sort<T>), grid1(nl, 256), dim3(256), 0, st, nl, ctx->line_start,
+                           (SpreadEntry<T> *)ctx->spr);
+    }
+
+    static void cells_build(fipi_ctx *ctx, const PartParams &pp, const PartState &ps, int *ids, int *start,
+                            bool interp)
+    {
+        hipStream_t st = ctx->stream;
+        hipLaunchKernelGGL(k_fill_int, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->cell_cnt, ctx->ncell, 0);
+        if (interp) {
+            hipLaunchKernelGGL((k_interp_fpi_count<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, ps, R(ctx->c),
+                               R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), ctx->cell_cnt);
+        } else {
+            hipLaunchKernelGGL(k_cell_count, grid1(pp.n, 256), dim3(256), 0, st, pp, ps.p[0], ps.p[1], ps.p[2],
+                               ps.cell, ctx->cell_cnt, ps.err);
+        }
+        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, ctx->cell_cnt, start, ctx->ncell,
+                           (const double *)ps.p[1], pp.n, ps.mark);
+        hipLaunchKernelGGL(k_fill_int, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->cell_cursor, ctx->ncell, 0);
+        hipLaunchKernelGGL(k_cell_scatter, grid1(pp.n, 256), dim3(256), 0, st, pp.n, ps.cell, start,
+                           ctx->cell_cursor, ids);
+        hipLaunchKernelGGL(k_cell_sort, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->ncell, start, ids);
+    }
+
+    static double ramp(double t, double thr)
+    {
+        // primitive.cpp:122-129
+        if (t < 0) return 0.0;
+        if (t < thr) return t / thr;
+        return 1.0;
+    }
+
+    static void particle_stage(fipi_ctx *ctx, int stage)
+    {
+        hipStream_t st = ctx->stream;
+        const PartParams &pp = ctx->pp;
+        cells_build(ctx, pp, ctx->ps, ctx->cell_ids, ctx->cell_start, true);
+        const double rmp = ramp((double)(ctx->t + 1), ctx->p.ramp_t);
+        hipLaunchKernelGGL(k_contact, grid1(pp.n, 128), dim3(128), 0, st, pp, ctx->ps, ctx->cell_start,
+                           ctx->cell_ids, rmp, ctx->d_flags + 1);
+        hipLaunchKernelGGL(k_advect, grid1(pp.n, 256), dim3(256), 0, st, pp, ctx->ps, stage);
+    }
+
+    static void step(fipi_ctx *ctx)
+    {
+        const Dims d = ctx->d;
+        hipStream_t st = ctx->stream;
+        const bool ns = ctx->p.switch_ns != 0;
+        const bool parts = ctx->n > 0;
+        C *h0 = H(ctx->h[0]), *h1 = H(ctx->h[1]), *h2 = H(ctx->h[2]);
+        if (ns) {
+            ProdConv<T> pr{R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2])};
+            x_forward<1>(ctx, pr, h0, nullptr);
+            y_pass<-1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
+        }
+        const KParams kp = kparams(ctx);
+        const int nxt = ctx->cur ^ 1;
+        with_n(d.nz, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            hipLaunchKernelGGL((k_hub_ch<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0, h1,
+                               (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]), (const C *)ctx->fk, H(ctx->hist), kp,
+                               d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
+        });
+        ctx->cur = nxt;
+        y_pass<+1, 2, true>(ctx, h0, h1, nullptr, h2);
+        cgrad_f(ctx);
+        y_pass<-1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
+        with_n(d.nz, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            hipLaunchKernelGGL((k_hub_mu<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0,
+                               H(ctx->fk), (const C *)ctx->ck[ctx->cur], kp, d, ctx->kx, ctx->ky, ctx->kz,
+                               (const C *)ctx->twz);
+        });
+        if (ns) {
+            y_pass<+1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
+            if (parts) {
+                const PartState &ps = ctx->ps;
+                spread_build(ctx, ctx->pp, ps.p[0], ps.p[1], ps.p[2], ps.fpp[0], ps.fpp[1], ps.fpp[2]);
+            }
+            with_n(d.nx, [&](auto NC) {
+                constexpr int N = decltype(NC)::value;
+                hipLaunchKernelGGL((k_x_force<T, N>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
+                                   dim3(TILE * LineShape<N>::P), 0, st, h0, h1, h2, (const T *)R(ctx->c),
+                                   (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
+                                   (const int *)ctx->line_start, (const SpreadEntry<T> *)ctx->spr, parts ? 1 : 0,
+                                   (T)ctx->p.rho1, (T)ctx->p.rho2, (T)ctx->p.gravity, d, (const C *)ctx->twx);
+            });
+            y_pass<-1, 3, false>(ctx, h0, h1, h2, nullptr);
+            with_n(d.nz, [&](auto NC) {
+                constexpr int N = decltype(NC)::value;
+                hipLaunchKernelGGL((k_hub_stokes<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0, h1,
+                                   h2, kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
+            });
+            y_pass<+1, 3, false>(ctx, h0, h1, h2, nullptr);
+            x_inverse<3>(ctx, h0, h1, h2, R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]));
+        }
+        if (parts) {
+            particle_stage(ctx, 1);
+            particle_stage(ctx, 2);
+        }
+        ctx->t++;
+    }
+
+    // unit entry: R2C of one field
+    static void r2c(fipi_ctx *ctx, const T *in, C *out)
+    {
+        x_forward<1>(ctx, ProdCopy<T>{in}, out, nullptr);
+        y_pass<-1, 1, false>(ctx, out, nullptr, nullptr, nullptr);
+        z_pass<-1>(ctx, out);
+    }
+
+    static void c2r(fipi_ctx *ctx, C *in, T *out)
+    {
+        z_pass<+1>(ctx, in);
+        y_pass<+1, 1, false>(ctx, in, nullptr, nullptr, nullptr);
+        x_inverse<1>(ctx, in, nullptr, nullptr, out, nullptr, nullptr);
+    }
+};
+
+// scale a real array (c2r normalisation in the unit entry)
+template <typename T>
+__global__ void k_scale(T *a, size_t n, double s)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = (T)((double)a[i] * s);
+}
+
+template <typename T>
+__global__ void k_to_double(const T *a, double *b, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (double)a[i];
+}
+
+template <typename T>
+__global__ void k_from_double(const double *a, T *b, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (T)a[i];
+}
+
+__global__ void k_aos_to_soa(const double *a, double *x, double *y, double *z, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { x[i] = a[3 * i]; y[i] = a[3 * i + 1]; z[i] = a[3 * i + 2]; }
+}
+
+__global__ void k_soa_to_aos(const double *x, const double *y, const double *z, double *a, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) { a[3 * i] = x[i]; a[3 * i + 1] = y[i]; a[3 * i + 2] = z[i]; }
+}
+
+template <typename T>
+__global__ void k_interp_field(PartParams P, const T *grid, const double *x, const double *y, const double *z,
+                               double *out)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    out[a] = grid_cell_ok(P, x[a], y[a], z[a]) ? interp1(P, grid, x[a], y[a], z[a]) : 0.0;
+}
+
+// Diagnostics partial sums: mean phi, free energy density, |grad phi|.
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_partial(const T *c, const T *g0, const T *g1, const T *g2, size_t G,
+                                                      double lambda, double eps, double *part)
+{
+    __shared__ double s[3][256];
+    double a = 0, b = 0, e = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < G; i += (size_t)gridDim.x * 256) {
+        const double cv = c[i], x = g0[i], y = g1[i], z = g2[i];
+        const double gg = (x * x + y * y) + z * z;
+        const double w = cv * cv - 1.0;
+        a += cv;
+        b += lambda * (0.5 * gg + 0.25 / (eps * eps) * (w * w));
+        e += sqrt(gg);
+    }
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    s[2][threadIdx.x] = e;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int q = 0; q < 3; q++) s[q][threadIdx.x] += s[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; q++) part[3 * blockIdx.x + q] = s[q][0];
+}
+
+__global__ __launch_bounds__(256) void k_sum_strided(const double *part, int nb, int stride, int nq, double *out)
+{
+    __shared__ double s[256];
+    for (int q = 0; q < nq; q++) {
+        double a = 0;
+        for (int i = threadIdx.x; i < nb; i += 256) a += part[(size_t)i * stride + q];
+        s[threadIdx.x] = a;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[q] = s[0];
+        __syncthreads();
+    }
+}
+
+constexpr int DIAG_BLOCKS = 1024;
+
+int check_params(const fipi_params *p)
+{
+    if (!p) return set_err(nullptr, FIPI_ERR_ARG, "params is NULL");
+    if (!supported_n(p->nx) || !supported_n(p->ny) || !supported_n(p->nz))
+        return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "grid sizes must each be one of 16, 32, 64, 128, 256");
+    if (p->switch_ch != 1) return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "switch_ch must be 1");
+    if (p->precision != FIPI_FP64 && p->precision != FIPI_FP32_MIXED)
+        return set_err(nullptr, FIPI_ERR_ARG, "precision must be FIPI_FP64 or FIPI_FP32_MIXED");
+    if (p->n_particles < 0) return set_err(nullptr, FIPI_ERR_ARG, "n_particles < 0");
+    if (!(p->lx > 0 && p->ly > 0 && p->lz > 0 && p->dt > 0))
+        return set_err(nullptr, FIPI_ERR_ARG, "box lengths and dt must be positive");
+    if (p->switch_particle && p->n_particles > 0) {
+        if (!(p->r_c > 0)) return set_err(nullptr, FIPI_ERR_ARG, "r_c must be positive");
+        const double L[3] = {p->lx, p->ly, p->lz};
+        for (int a = 0; a < 3; a++) {
+            const int nc = (int)floor(L[a] / p->r_c + 2 * p->margin / p->r_c);
+            if (nc < 3) return set_err(nullptr, FIPI_ERR_ARG, "fewer than 3 contact cells per axis");
+        }
+    }
+    return FIPI_OK;
+}
+
+template <typename T>
+int setup_buffers(fipi_ctx *ctx)
+{
+    const fipi_params &p = ctx->p;
+    ctx->esz = sizeof(T);
+    const size_t G = ctx->G, GH = ctx->GH;
+    int rc;
+#define ALLOC(ptr, bytes) \
+    if ((rc = dalloc(ctx, (void **)&(ptr), (bytes))) != FIPI_OK) return rc
+    ALLOC(ctx->c, G * sizeof(T));
+    for (int a = 0; a < 3; a++) {
+        ALLOC(ctx->g[a], G * sizeof(T));
+        ALLOC(ctx->u[a], G * sizeof(T));
+        ALLOC(ctx->h[a], GH * 2 * sizeof(T));
+    }
+    ALLOC(ctx->ck[0], GH * 2 * sizeof(T));
+    ALLOC(ctx->ck[1], GH * 2 * sizeof(T));
+    ALLOC(ctx->fk, GH * 2 * sizeof(T));
+    ALLOC(ctx->hist, GH * 2 * sizeof(T));
+    ALLOC(ctx->kx, sizeof(double) * p.nx);
+    ALLOC(ctx->ky, sizeof(double) * p.ny);
+    ALLOC(ctx->kz, sizeof(double) * p.nz);
+    ALLOC(ctx->twx, sizeof(T) * 2 * p.nx);
+    ALLOC(ctx->twy, sizeof(T) * 2 * p.ny);
+    ALLOC(ctx->twz, sizeof(T) * 2 * p.nz);
+    ALLOC(ctx->d_flags, sizeof(int) * 4);
+    ALLOC(ctx->d_red, sizeof(double) * (3 * DIAG_BLOCKS + 16));
+    auto kx = wave_numbers(p.nx, p.pi, p.lx), ky = wave_numbers(p.ny, p.pi, p.ly), kz = wave_numbers(p.nz, p.pi, p.lz);
+    HIPCHK(ctx, hipMemcpy(ctx->kx, kx.data(), sizeof(double) * p.nx, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->ky, ky.data(), sizeof(double) * p.ny, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->kz, kz.data(), sizeof(double) * p.nz, hipMemcpyHostToDevice));
+    auto tx = twiddles<T>(p.nx), ty = twiddles<T>(p.ny), tz = twiddles<T>(p.nz);
+    HIPCHK(ctx, hipMemcpy(ctx->twx, tx.data(), sizeof(T) * 2 * p.nx, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->twy, ty.data(), sizeof(T) * 2 * p.ny, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemcpy(ctx->twz, tz.data(), sizeof(T) * 2 * p.nz, hipMemcpyHostToDevice));
+    HIPCHK(ctx, hipMemset(ctx->d_flags, 0, sizeof(int) * 4));
+    HIPCHK(ctx, hipHostMalloc((void **)&ctx->h_flags, sizeof(int) * 4, hipHostMallocDefault));
+    // contact grid (cluster.cpp:26-32)
+    const double L[3] = {p.lx, p.ly, p.lz};
+    int nc[3];
+    double dc[3];
+    for (int a = 0; a < 3; a++) {
+        nc[a] = (int)floor(L[a] / p.r_c + 2 * p.margin / p.r_c);
+        dc[a] = fmod(L[a] + 2 * p.margin, p.r_c) / nc[a] + p.r_c;
+    }
+    PartParams &pp = ctx->pp;
+    pp.n = ctx->n;
+    pp.nx = p.nx; pp.ny = p.ny; pp.nz = p.nz;
+    pp.dx = p.lx / p.nx; pp.dy = p.ly / p.ny; pp.dz = p.lz / p.nz;
+    pp.lx = p.lx; pp.ly = p.ly; pp.lz = p.lz;
+    pp.ncx = nc[0]; pp.ncy = nc[1]; pp.ncz = nc[2];
+    pp.dcx = dc[0]; pp.dcy = dc[1]; pp.dcz = dc[2];
+    pp.margin = p.margin;
+    pp.r_p = p.r_p; pp.r_eq = p.r_eq; pp.k_n = p.k_n;
+    pp.pi = p.pi; pp.mu = p.mu; pp.dt = p.dt; pp.epsilon = p.epsilon; pp.sigma = p.sigma; pp.a_pi = p.a_pi;
+    pp.tanh2 = p.tanh2;
+    pp.sqrt2 = std::pow(2.0, 0.5);
+    pp.has_ctheta = 0;
+    ctx->ncell = (nc[0] > 0 && nc[1] > 0 && nc[2] > 0) ? nc[0] * nc[1] * nc[2] : 1;
+    ctx->nlines = p.ny * p.nz;
+    ALLOC(ctx->cell_cnt, sizeof(int) * (ctx->ncell + 1));
+    ALLOC(ctx->cell_start, sizeof(int) * (ctx->ncell + 1));
+    ALLOC(ctx->cell_cursor, sizeof(int) * (ctx->ncell + 1));
+    ALLOC(ctx->line_cnt, sizeof(int) * (ctx->nlines + 1));
+    ALLOC(ctx->line_start, sizeof(int) * (ctx->nlines + 1));
+    ALLOC(ctx->line_cursor, sizeof(int) * (ctx->nlines + 1));
+    HIPCHK(ctx, hipMemset(ctx->line_start, 0, sizeof(int) * (ctx->nlines + 1)));
+    const size_t n = ctx->n > 0 ? ctx->n : 1;
+    ALLOC(ctx->cell_ids, sizeof(int) * n);
+    ALLOC(ctx->spr, sizeof(SpreadEntry<T>) * 8 * n);
+    PartState &ps = ctx->ps;
+    for (int a = 0; a < 3; a++) {
+        ALLOC(ps.p[a], sizeof(double) * n);
+        ALLOC(ps.po[a], sizeof(double) * n);
+        ALLOC(ps.u[a], sizeof(double) * n);
+        ALLOC(ps.uo[a], sizeof(double) * n);
+        ALLOC(ps.fpp[a], sizeof(double) * n);
+        ALLOC(ps.fpi[a], sizeof(double) * n);
+        ALLOC(ps.dcdx[a], sizeof(double) * n);
+    }
+    ALLOC(ps.rad, sizeof(double) * n);
+    ALLOC(ps.cth, sizeof(double) * n);
+    ALLOC(ps.cloc, sizeof(double) * n);
+    ALLOC(ps.tau_i, sizeof(double) * n);
+    ALLOC(ps.cell, sizeof(int) * n);
+    ps.err = ctx->d_flags;
+    ALLOC(ps.mark, sizeof(int));
+#undef ALLOC
+    return FIPI_OK;
+}
+
+int copy_in_real(fipi_ctx *ctx, void *dst, const double *src, size_t n)
+{
+    if (ctx->fp64) {
+        HIPCHK(ctx, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        return FIPI_OK;
+    }
+    double *tmp;
+    HIPCHK(ctx, hipMalloc((void **)&tmp, n * sizeof(double)));
+    HIPCHK(ctx, hipMemcpyAsync(tmp, src, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL((k_from_double<float>), grid1(n, 256), dim3(256), 0, ctx->stream, tmp, (float *)dst, n);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipFree(tmp));
+    return FIPI_OK;
+}
+
+int copy_out_real(fipi_ctx *ctx, double *dst, const void *src, size_t n)
+{
+    if (ctx->fp64) {
+        HIPCHK(ctx, hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return FIPI_OK;
+    }
+    double *tmp;
+    HIPCHK(ctx, hipMalloc((void **)&tmp, n * sizeof(double)));
+    hipLaunchKernelGGL((k_to_double<float>), grid1(n, 256), dim3(256), 0, ctx->stream, (const float *)src, tmp, n);
+    HIPCHK(ctx, hipMemcpyAsync(dst, tmp, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipFree(tmp));
+    return FIPI_OK;
+}
+
+int check_flags(fipi_ctx *ctx)
+{
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_flags, ctx->d_flags, sizeof(int) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipGetLastError());
+    if (ctx->h_flags[0] & 1) return set_err(ctx, FIPI_ERR_RANGE, "particle outside the contact-cell sorting range");
+    if (ctx->h_flags[0] & 2) return set_err(ctx, FIPI_ERR_RANGE, "particle outside the grid");
+    return FIPI_OK;
+}
+
+// Temporary SoA particle arrays for the unit entry points.
+struct TmpParts {
+    fipi_ctx *ctx;
+    PartState ps{};
+    std::vector<void *> mem;
+    int n;
+    int alloc(void **p, size_t b)
+    {
+        hipError_t e = hipMalloc(p, b ? b : 16);
+        if (e != hipSuccess) return set_err(ctx, FIPI_ERR_HIP, "hipMalloc failed");
+        mem.push_back(*p);
+        return FIPI_OK;
+    }
+    int init(fipi_ctx *c, const double *d_pos, int nn)
+    {
+        ctx = c;
+        n = nn;
+        const size_t b = sizeof(double) * (n > 0 ? n : 1);
+        int rc;
+        for (int a = 0; a < 3; a++) {
+            if ((rc = alloc((void **)&ps.p[a], b))) return rc;
+            if ((rc = alloc((void **)&ps.fpp[a], b))) return rc;
+            if ((rc = alloc((void **)&ps.fpi[a], b))) return rc;
+        }
+        if ((rc = alloc((void **)&ps.tau_i, b))) return rc;
+        if ((rc = alloc((void **)&ps.cell, sizeof(int) * (n > 0 ? n : 1)))) return rc;
+        if ((rc = alloc((void **)&ps.mark, sizeof(int)))) return rc;
+        ps.err = ctx->d_flags;
+        HIPCHK(ctx, hipMemsetAsync(ps.mark, 0xff, sizeof(int), ctx->stream));
+        for (int a = 0; a < 3; a++) HIPCHK(ctx, hipMemsetAsync(ps.fpi[a], 0, b, ctx->stream));
+        if (n > 0)
+            hipLaunchKernelGGL(k_aos_to_soa, grid1(n, 256), dim3(256), 0, ctx->stream, d_pos, ps.p[0], ps.p[1],
+                               ps.p[2], n);
+        return FIPI_OK;
+    }
+    ~TmpParts()
+    {
+        if (ctx) (void)hipStreamSynchronize(ctx->stream);
+        for (void *p : mem) (void)hipFree(p);
+    }
+};
+
+// cells build on caller arrays (unit entries)
+int unit_cells(fipi_ctx *ctx, TmpParts &tp, int *start, int *ids)
+{
+    PartParams pp = ctx->pp;
+    pp.n = tp.n;
+    if (ctx->fp64) Impl<double>::cells_build(ctx, pp, tp.ps, ids, start, false);
+    else Impl<float>::cells_build(ctx, pp, tp.ps, ids, start, false);
+    return FIPI_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int fipi_create(const fipi_params *params, fipi_ctx **out)
+{
+    if (!out) return set_err(nullptr, FIPI_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = check_params(params);
+    if (rc != FIPI_OK) return rc;
+    fipi_ctx *ctx = new fipi_ctx();
+    ctx->p = *params;
+    ctx->fp64 = params->precision == FIPI_FP64;
+    ctx->d = Dims{params->nx, params->ny, params->nz, params->nx / 2 + 1};
+    ctx->G = (size_t)params->nx * params->ny * params->nz;
+    ctx->GH = (size_t)ctx->d.nxh * params->ny * params->nz;
+    ctx->n = params->switch_particle ? params->n_particles : 0;
+    hipError_t e = hipGetDevice(&ctx->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        rc = set_err(nullptr, FIPI_ERR_HIP, std::string("HIP init: ") + hipGetErrorString(e));
+        delete ctx;
+        return rc;
+    }
+    rc = ctx->fp64 ? setup_buffers<double>(ctx) : setup_buffers<float>(ctx);
+    if (rc != FIPI_OK) {
+        fipi_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return FIPI_OK;
+}
+
+int fipi_destroy(fipi_ctx *ctx)
+{
+    if (!ctx) return FIPI_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (void *p : ctx->allocs) (void)hipFree(p);
+    if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FIPI_OK;
+}
+
+const char *fipi_last_error(const fipi_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : g_last_error.c_str();
+}
+
+void *fipi_stream(fipi_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int fipi_set_state(fipi_ctx *ctx, const double *phi, const double *pos, const double *radius,
+                   const double *cos_theta)
+{
+    if (!ctx || !phi) return set_err(ctx, FIPI_ERR_ARG, "ctx and phi are required");
+    if (ctx->n > 0 && !pos) return set_err(ctx, FIPI_ERR_ARG, "pos is required when particles are on");
+    int rc;
+    if ((rc = copy_in_real(ctx, ctx->c, phi, ctx->G))) return rc;
+    for (int a = 0; a < 3; a++) HIPCHK(ctx, hipMemsetAsync(ctx->u[a], 0, ctx->G * ctx->esz, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->hist, 0, ctx->GH * 2 * ctx->esz, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4, ctx->stream));
+    ctx->t = 0;
+    ctx->cur = 0;
+    if (ctx->fp64) Impl<double>::init_fields(ctx);
+    else Impl<float>::init_fields(ctx);
+    // keep phi exactly as given (the transform round trip above rewrote it)
+    if ((rc = copy_in_real(ctx, ctx->c, phi, ctx->G))) return rc;
+    const int n = ctx->n;
+    if (n > 0) {
+        PartState &ps = ctx->ps;
+        double *tmp;
+        HIPCHK(ctx, hipMalloc((void **)&tmp, sizeof(double) * 3 * n));
+        HIPCHK(ctx, hipMemcpyAsync(tmp, pos, sizeof(double) * 3 * n, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL(k_aos_to_soa, grid1(n, 256), dim3(256), 0, ctx->stream, tmp, ps.p[0], ps.p[1], ps.p[2],
+                           n);
+        hipLaunchKernelGGL(k_aos_to_soa, grid1(n, 256), dim3(256), 0, ctx->stream, tmp, ps.po[0], ps.po[1],
+                           ps.po[2], n);
+        for (int a = 0; a < 3; a++) {
+            HIPCHK(ctx, hipMemsetAsync(ps.u[a], 0, sizeof(double) * n, ctx->stream));
+            HIPCHK(ctx, hipMemsetAsync(ps.uo[a], 0, sizeof(double) * n, ctx->stream));
+            HIPCHK(ctx, hipMemsetAsync(ps.fpp[a], 0, sizeof(double) * n, ctx->stream));
+            HIPCHK(ctx, hipMemsetAsync(ps.fpi[a], 0, sizeof(double) * n, ctx->stream));
+            HIPCHK(ctx, hipMemsetAsync(ps.dcdx[a], 0, sizeof(double) * n, ctx->stream));
+        }
+        HIPCHK(ctx, hipMemsetAsync(ps.tau_i, 0, sizeof(double) * n, ctx->stream));
+        HIPCHK(ctx, hipMemsetAsync(ps.mark, 0xff, sizeof(int), ctx->stream));
+        std::vector<double> h(n);
+        for (int i = 0; i < n; i++) h[i] = radius ? radius[i] : ctx->p.r_p;
+        HIPCHK(ctx, hipMemcpyAsync(ps.rad, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < n; i++) h[i] = 1.0;  // element.cpp:34
+        HIPCHK(ctx, hipMemcpyAsync(ps.cloc, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        ctx->pp.has_ctheta = 0;
+        if (cos_theta) {
+            for (int i = 0; i < n; i++) {
+                h[i] = cos_theta[i];
+                if (h[i] != 0.0) ctx->pp.has_ctheta = 1;
+            }
+        } else {
+            for (int i = 0; i < n; i++) h[i] = 0.0;
+        }
+        HIPCHK(ctx, hipMemcpyAsync(ps.cth, h.data(), sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHK(ctx, hipFree(tmp));
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipGetLastError());
+    ctx->has_state = true;
+    return FIPI_OK;
+}
+
+int fipi_set_velocity(fipi_ctx *ctx, const double *u)
+{
+    if (!ctx || !u) return set_err(ctx, FIPI_ERR_ARG, "ctx and u are required");
+    for (int a = 0; a < 3; a++) {
+        int rc = copy_in_real(ctx, ctx->u[a], u + (size_t)a * ctx->G, ctx->G);
+        if (rc) return rc;
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return FIPI_OK;
+}
+
+int fipi_get_phi(fipi_ctx *ctx, double *phi)
+{
+    if (!ctx || !phi) return set_err(ctx, FIPI_ERR_ARG, "ctx and phi are required");
+    return copy_out_real(ctx, phi, ctx->c, ctx->G);
+}
+
+int fipi_get_velocity(fipi_ctx *ctx, double *u)
+{
+    if (!ctx || !u) return set_err(ctx, FIPI_ERR_ARG, "ctx and u are required");
+    for (int a = 0; a < 3; a++) {
+        int rc = copy_out_real(ctx, u + (size_t)a * ctx->G, ctx->u[a], ctx->G);
+        if (rc) return rc;
+    }
+    return FIPI_OK;
+}
+
+int fipi_get_grad_phi(fipi_ctx *ctx, double *g)
+{
+    if (!ctx || !g) return set_err(ctx, FIPI_ERR_ARG, "ctx and grad are required");
+    for (int a = 0; a < 3; a++) {
+        int rc = copy_out_real(ctx, g + (size_t)a * ctx->G, ctx->g[a], ctx->G);
+        if (rc) return rc;
+    }
+    return FIPI_OK;
+}
+
+int fipi_get_particles(fipi_ctx *ctx, double *pos, double *pos_old, double *u, double *u_old, double *f_pp,
+                       double *f_pi)
+{
+    if (!ctx) return set_err(ctx, FIPI_ERR_ARG, "ctx is NULL");
+    const int n = ctx->n;
+    if (n == 0) return FIPI_OK;
+    double *tmp;
+    HIPCHK(ctx, hipMalloc((void **)&tmp, sizeof(double) * 3 * n));
+    double *const *src[6] = {ctx->ps.p, ctx->ps.po, ctx->ps.u, ctx->ps.uo, ctx->ps.fpp, ctx->ps.fpi};
+    double *dst[6] = {pos, pos_old, u, u_old, f_pp, f_pi};
+    for (int k = 0; k < 6; k++) {
+        if (!dst[k]) continue;
+        hipLaunchKernelGGL(k_soa_to_aos, grid1(n, 256), dim3(256), 0, ctx->stream, src[k][0], src[k][1], src[k][2],
+                           tmp, n);
+        HIPCHK(ctx, hipMemcpyAsync(dst[k], tmp, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    HIPCHK(ctx, hipFree(tmp));
+    return FIPI_OK;
+}
+
+int fipi_get_time(fipi_ctx *ctx, int32_t *t)
+{
+    if (!ctx || !t) return set_err(ctx, FIPI_ERR_ARG, "ctx and t are required");
+    *t = ctx->t;
+    return FIPI_OK;
+}
+
+int fipi_diagnostics(fipi_ctx *ctx, fipi_diag *out)
+{
+    if (!ctx || !out) return set_err(ctx, FIPI_ERR_ARG, "ctx and out are required");
+    hipStream_t st = ctx->stream;
+    const double dx = ctx->p.lx / ctx->p.nx, dy = ctx->p.ly / ctx->p.ny, dz = ctx->p.lz / ctx->p.nz;
+    (void)dy;
+    (void)dz;
+    if (ctx->fp64)
+        hipLaunchKernelGGL((k_diag_partial<double>), dim3(DIAG_BLOCKS), dim3(256), 0, st, (const double *)ctx->c,
+                           (const double *)ctx->g[0], (const double *)ctx->g[1], (const double *)ctx->g[2], ctx->G,
+                           ctx->p.lambda, ctx->p.epsilon, ctx->d_red);
+    else
+        hipLaunchKernelGGL((k_diag_partial<float>), dim3(DIAG_BLOCKS), dim3(256), 0, st, (const float *)ctx->c,
+                           (const float *)ctx->g[0], (const float *)ctx->g[1], (const float *)ctx->g[2], ctx->G,
+                           ctx->p.lambda, ctx->p.epsilon, ctx->d_red);
+    hipLaunchKernelGGL(k_sum_strided, dim3(1), dim3(256), 0, st, ctx->d_red, DIAG_BLOCKS, 3, 3,
+                       ctx->d_red + 3 * DIAG_BLOCKS);
+    if (ctx->n > 0)
+        hipLaunchKernelGGL(k_sum_strided, dim3(1), dim3(256), 0, st, ctx->ps.tau_i, ctx->n, 1, 1,
+                           ctx->d_red + 3 * DIAG_BLOCKS + 3);
+    double h[4] = {0, 0, 0, 0};
+    HIPCHK(ctx, hipMemcpyAsync(h, ctx->d_red + 3 * DIAG_BLOCKS, sizeof(double) * (ctx->n > 0 ? 4 : 3),
+                               hipMemcpyDeviceToHost, st));
+    int mark = -1;
+    if (ctx->n > 0) HIPCHK(ctx, hipMemcpyAsync(&mark, ctx->ps.mark, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    const double cell = dx * dx * dx;  // cubic(dx), field.cpp:512
+    out->t = ctx->t;
+    out->it_mark = mark;
+    out->mean_phi = h[0] / (double)ctx->G;
+    out->free_energy = h[1] * cell;
+    out->interface_area = h[2] * cell;
+    out->tau = h[3];
+    return FIPI_OK;
+}
+
+int fipi_step_async(fipi_ctx *ctx, int32_t nsteps)
+{
+    if (!ctx) return set_err(ctx, FIPI_ERR_ARG, "ctx is NULL");
+    if (!ctx->has_state) return set_err(ctx, FIPI_ERR_STATE, "fipi_set_state must be called first");
+    if (nsteps < 0) return set_err(ctx, FIPI_ERR_ARG, "nsteps < 0");
+    for (int s = 0; s < nsteps; s++) {
+        if (ctx->fp64) Impl<double>::step(ctx);
+        else Impl<float>::step(ctx);
+    }
+    HIPCHK(ctx, hipGetLastError());
+    return FIPI_OK;
+}
+
+int fipi_sync(fipi_ctx *ctx)
+{
+    if (!ctx) return set_err(ctx, FIPI_ERR_ARG, "ctx is NULL");
+    return check_flags(ctx);
+}
+
+int fipi_step(fipi_ctx *ctx, int32_t nsteps)
+{
+    int rc = fipi_step_async(ctx, nsteps);
+    if (rc) return rc;
+    return check_flags(ctx);
+}
+
+int fipi_fft_r2c(fipi_ctx *ctx, const void *d_real, void *d_half)
+{
+    if (!ctx || !d_real || !d_half) return set_err(ctx, FIPI_ERR_ARG, "null argument");
+    if (ctx->fp64) Impl<double>::r2c(ctx, (const double *)d_real, (double2 *)d_half);
+    else Impl<float>::r2c(ctx, (const float *)d_real, (float2 *)d_half);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipGetLastError());
+    return FIPI_OK;
+}
+
+int fipi_fft_c2r(fipi_ctx *ctx, void *d_half, void *d_real)
+{
+    if (!ctx || !d_real || !d_half) return set_err(ctx, FIPI_ERR_ARG, "null argument");
+    const double s = 1.0 / (double)ctx->G;
+    if (ctx->fp64) {
+        Impl<double>::c2r(ctx, (double2 *)d_half, (double *)d_real);
+        hipLaunchKernelGGL((k_scale<double>), grid1(ctx->G, 256), dim3(256), 0, ctx->stream, (double *)d_real,
+                           ctx->G, s);
+    } else {
+        Impl<float>::c2r(ctx, (float2 *)d_half, (float *)d_real);
+        hipLaunchKernelGGL((k_scale<float>), grid1(ctx->G, 256), dim3(256), 0, ctx->stream, (float *)d_real,
+                           ctx->G, s);
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipGetLastError());
+    return FIPI_OK;
+}
+
+int fipi_cells_build(fipi_ctx *ctx, const double *d_pos, int32_t n, int32_t *d_cell_start, int32_t *d_ids,
+                     int32_t *ncell_out)
+{
+    if (!ctx || (n > 0 && (!d_pos || !d_ids)) || !d_cell_start || n < 0)
+        return set_err(ctx, FIPI_ERR_ARG, "invalid argument");
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4, ctx->stream));
+    TmpParts tp;
+    int rc = tp.init(ctx, d_pos, n);
+    if (rc) return rc;
+    if ((rc = unit_cells(ctx, tp, d_cell_start, d_ids))) return rc;
+    if (ncell_out) *ncell_out = ctx->ncell;
+    return check_flags(ctx);
+}
+
+int fipi_contact_pairs(fipi_ctx *ctx, const double *d_pos, int32_t n, int32_t *d_pairs, int64_t max_pairs,
+                       int64_t *count)
+{
+    if (!ctx || n < 0 || (n > 0 && !d_pos) || !count || (max_pairs > 0 && !d_pairs))
+        return set_err(ctx, FIPI_ERR_ARG, "invalid argument");
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4, ctx->stream));
+    TmpParts tp;
+    int rc = tp.init(ctx, d_pos, n);
+    if (rc) return rc;
+    int *ids, *pcnt, *pstart;
+    if ((rc = tp.alloc((void **)&ids, sizeof(int) * (n > 0 ? n : 1)))) return rc;
+    if ((rc = tp.alloc((void **)&pcnt, sizeof(int) * (n + 1)))) return rc;
+    if ((rc = tp.alloc((void **)&pstart, sizeof(int) * (n + 1)))) return rc;
+    if ((rc = unit_cells(ctx, tp, ctx->cell_start, ids))) return rc;
+    PartParams pp = ctx->pp;
+    pp.n = n;
+    hipStream_t st = ctx->stream;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_pairs, grid1(n, 128), dim3(128), 0, st, pp, tp.ps, ctx->cell_start, ids,
+                           (const int *)nullptr, pcnt, (int *)nullptr, (int64_t)0);
+        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, pcnt, pstart, n, (const double *)nullptr, 0,
+                           (int *)nullptr);
+        if (max_pairs > 0) {
+            hipLaunchKernelGGL(k_pairs, grid1(n, 128), dim3(128), 0, st, pp, tp.ps, ctx->cell_start, ids,
+                               (const int *)pstart, pcnt, d_pairs, max_pairs);
+            hipLaunchKernelGGL(k_pairs_sort, grid1(n, 256), dim3(256), 0, st, n, pstart, d_pairs, max_pairs);
+        }
+        int tot = 0;
+        HIPCHK(ctx, hipMemcpyAsync(&tot, pstart + n, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(ctx, hipStreamSynchronize(st));
+        *count = tot;
+    } else {
+        *count = 0;
+    }
+    return check_flags(ctx);
+}
+
+int fipi_contact_forces(fipi_ctx *ctx, const double *d_pos, int32_t n, int32_t t, double *d_f_pp)
+{
+    if (!ctx || n < 0 || (n > 0 && (!d_pos || !d_f_pp))) return set_err(ctx, FIPI_ERR_ARG, "invalid argument");
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4, ctx->stream));
+    TmpParts tp;
+    int rc = tp.init(ctx, d_pos, n);
+    if (rc) return rc;
+    int *ids;
+    if ((rc = tp.alloc((void **)&ids, sizeof(int) * (n > 0 ? n : 1)))) return rc;
+    if ((rc = unit_cells(ctx, tp, ctx->cell_start, ids))) return rc;
+    PartParams pp = ctx->pp;
+    pp.n = n;
+    const double thr = ctx->p.ramp_t;
+    const double rmp = t < 0 ? 0.0 : (t < thr ? t / thr : 1.0);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_contact, grid1(n, 128), dim3(128), 0, ctx->stream, pp, tp.ps, ctx->cell_start, ids, rmp,
+                           ctx->d_flags + 1);
+        hipLaunchKernelGGL(k_soa_to_aos, grid1(n, 256), dim3(256), 0, ctx->stream, tp.ps.fpp[0], tp.ps.fpp[1],
+                           tp.ps.fpp[2], d_f_pp, n);
+    }
+    return check_flags(ctx);
+}
+
+int fipi_interp(fipi_ctx *ctx, const void *d_grid, const double *d_pos, int32_t n, double *d_out)
+{
+    if (!ctx || !d_grid || n < 0 || (n > 0 && (!d_pos || !d_out))) return set_err(ctx, FIPI_ERR_ARG, "invalid argument");
+    TmpParts tp;
+    int rc = tp.init(ctx, d_pos, n);
+    if (rc) return rc;
+    PartParams pp = ctx->pp;
+    pp.n = n;
+    if (n > 0) {
+        if (ctx->fp64)
+            hipLaunchKernelGGL((k_interp_field<double>), grid1(n, 256), dim3(256), 0, ctx->stream, pp,
+                               (const double *)d_grid, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], d_out);
+        else
+            hipLaunchKernelGGL((k_interp_field<float>), grid1(n, 256), dim3(256), 0, ctx->stream, pp,
+                               (const float *)d_grid, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], d_out);
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipGetLastError());
+    return FIPI_OK;
+}
+
+int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32_t n, void *d_grid3)
+{
+    if (!ctx || !d_grid3 || n < 0 || (n > 0 && (!d_pos || !d_force)))
+        return set_err(ctx, FIPI_ERR_ARG, "invalid argument");
+    if (n > ctx->n && n > 0) {
+        // entry buffer is sized for the context's particle count
+        if (ctx->n < n) return set_err(ctx, FIPI_ERR_ARG, "n exceeds the context's n_particles");
+    }
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4, ctx->stream));
+    TmpParts tp;
+    int rc = tp.init(ctx, d_pos, n);
+    if (rc) return rc;
+    double *f[3];
+    for (int a = 0; a < 3; a++)
+        if ((rc = tp.alloc((void **)&f[a], sizeof(double) * (n > 0 ? n : 1)))) return rc;
+    if (n > 0)
+        hipLaunchKernelGGL(k_aos_to_soa, grid1(n, 256), dim3(256), 0, ctx->stream, d_force, f[0], f[1], f[2], n);
+    PartParams pp = ctx->pp;
+    pp.n = n;
+    HIPCHK(ctx, hipMemsetAsync(d_grid3, 0, 3 * ctx->G * ctx->esz, ctx->stream));
+    if (ctx->fp64) {
+        Impl<double>::spread_build(ctx, pp, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], f[0], f[1], f[2]);
+        hipLaunchKernelGGL((k_spread_dense<double>), grid1(ctx->nlines, 256), dim3(256), 0, ctx->stream, ctx->nlines,
+                           ctx->p.nx, ctx->G, ctx->line_start, (const SpreadEntry<double> *)ctx->spr,
+                           (double *)d_grid3);
+    } else {
+        Impl<float>::spread_build(ctx, pp, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], f[0], f[1], f[2]);
+        hipLaunchKernelGGL((k_spread_dense<float>), grid1(ctx->nlines, 256), dim3(256), 0, ctx->stream, ctx->nlines,
+                           ctx->p.nx, ctx->G, ctx->line_start, (const SpreadEntry<float> *)ctx->spr,
+                           (float *)d_grid3);
+    }
+    return check_flags(ctx);
+}
+
+}  // extern "C"

@@ -1,0 +1,634 @@
+// Particle kernels: cell list, 27-cell contact forces, trilinear
+// interpolation, capillary attachment force, Heun advection and the
+// deterministic spreading of particle forces to the grid.
+//
+// All particle quantities are FP64 in both precision modes.  Every kernel
+// is deterministic: integer atomics only fix counts and slots, and every
+// floating-point sum runs in a fixed order (no float atomics).
+//
+// Floating-point expressions follow the reference's association order and
+// this file is compiled with -ffp-contract=off, so for identical inputs the
+// cell membership, contact-pair set and contact forces are bitwise those of
+// the reference restatement.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "field_kernels.hpp"
+
+namespace fipi {
+
+struct PartParams {
+    int n;
+    int nx, ny, nz;              // grid
+    double dx, dy, dz;           // grid spacing L/N (head.hpp:37-39)
+    double lx, ly, lz;
+    int ncx, ncy, ncz;           // contact cells (cluster.cpp:26-28)
+    double dcx, dcy, dcz;        // contact cell size (cluster.cpp:30-32)
+    double margin;
+    double r_p, r_eq, k_n;
+    double pi, mu, dt, epsilon, sigma, a_pi, tanh2;
+    double sqrt2;                // pow(2.0, 0.5) as the reference's sqrt override
+    int has_ctheta;
+};
+
+struct PartState {
+    double *p[3], *po[3], *u[3], *uo[3], *fpp[3], *fpi[3];
+    double *rad, *cth;
+    double *cloc, *dcdx[3];
+    double *tau_i;               // per-particle packing-energy contributions
+    int *cell;                   // contact cell of each particle
+    int *err;                    // device error flags (bit 0: sorting range)
+    int *mark;                   // it_mark (argmax y), persists across sorts
+};
+
+// ---------------------------------------------------------------------------
+// wave64 inclusive scan of one int per lane
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Exclusive scan of cnt[0..n) into start[0..n] (start[n] = total), one
+// workgroup of 1024 threads; each thread scans a contiguous chunk.
+// Optionally also finds it_mark = first index of the strictly largest
+// y > 0 (cluster.cpp:128-138), keeping the previous value if none.
+__global__ __launch_bounds__(1024) void k_scan_excl(const int *__restrict__ cnt, int *__restrict__ start, int n,
+                                                    const double *__restrict__ py, int np, int *__restrict__ mark)
+{
+    __shared__ int wsum[16];
+    __shared__ double wy[16];
+    __shared__ int wi[16];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int chunk = (n + 1023) / 1024;
+    const int b0 = tid * chunk;
+    const int b1 = min(b0 + chunk, n);
+    int s = 0;
+    for (int i = b0; i < b1; i++) s += cnt[i];
+    const int inc = wave_incl_scan(s);
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wid; w++) off += wsum[w];
+    int run = off + inc - s;
+    for (int i = b0; i < b1; i++) {
+        start[i] = run;
+        run += cnt[i];
+    }
+    if (tid == 1023) {
+        int tot = 0;
+        for (int w = 0; w < 16; w++) tot += wsum[w];
+        start[n] = tot;
+    }
+    if (py == nullptr) return;
+    // argmax y (strictly greater, first index), y_max starts at 0.0
+    double by = 0.0;
+    int bi = -1;
+    for (int i = tid; i < np; i += 1024) {
+        const double y = py[i];
+        if (y > by) { by = y; bi = i; }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double oy = __shfl_down(by, o, 64);
+        const int oi = __shfl_down(bi, o, 64);
+        if (oi >= 0 && (bi < 0 || oy > by || (oy == by && oi < bi))) { by = oy; bi = oi; }
+    }
+    if (lane == 0) { wy[wid] = by; wi[wid] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+        for (int w = 1; w < 16; w++) {
+            const double oy = wy[w];
+            const int oi = wi[w];
+            if (oi >= 0 && (bi < 0 || oy > by || (oy == by && oi < bi))) { by = oy; bi = oi; }
+        }
+        if (bi >= 0) *mark = bi;
+    }
+}
+
+__global__ void k_fill_int(int *__restrict__ a, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// cluster.cpp:140-142 / 197-199
+__device__ __forceinline__ int cell_coord(double x, double dc, double margin)
+{
+    return (int)floor(x / dc + margin / dc);
+}
+
+__device__ __forceinline__ int cell_of(const PartParams &P, double x, double y, double z, int *ci, int *cj,
+                                       int *ck)
+{
+    const int i = cell_coord(x, P.dcx, P.margin), j = cell_coord(y, P.dcy, P.margin),
+              k = cell_coord(z, P.dcz, P.margin);
+    *ci = i; *cj = j; *ck = k;
+    if (i < 0 || i >= P.ncx || j < 0 || j >= P.ncy || k < 0 || k >= P.ncz) return -1;
+    return i + j * P.ncx + k * P.ncx * P.ncy;
+}
+
+// Histogram of contact cells; flags particles outside the sorting range.
+__global__ void k_cell_count(PartParams P, const double *__restrict__ px, const double *__restrict__ py,
+                             const double *__restrict__ pz, int *__restrict__ cell, int *__restrict__ cnt,
+                             int *__restrict__ err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    int a, b, c;
+    const int cc = cell_of(P, px[i], py[i], pz[i], &a, &b, &c);
+    cell[i] = cc;
+    if (cc < 0) { atomicOr(err, 1); return; }
+    atomicAdd(&cnt[cc], 1);
+}
+
+__global__ void k_cell_scatter(int n, const int *__restrict__ cell, const int *__restrict__ start,
+                               int *__restrict__ cursor, int *__restrict__ ids)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int cc = cell[i];
+    if (cc < 0) return;
+    const int slot = atomicAdd(&cursor[cc], 1);
+    ids[start[cc] + slot] = i;
+}
+
+// Ascending ids within each cell (list::append order, linked_list.cpp:30-43).
+__global__ void k_cell_sort(int ncell, const int *__restrict__ start, int *__restrict__ ids)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    const int b = start[c], e = start[c + 1];
+    for (int i = b + 1; i < e; i++) {
+        const int v = ids[i];
+        int j = i - 1;
+        while (j >= b && ids[j] > v) { ids[j + 1] = ids[j]; j--; }
+        ids[j + 1] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Contact forces (cluster.cpp:174-275)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Deterministic stand-in for the rand() jitter of coincident particles
+// (cluster.cpp:234-242), for the ordered pair i < j.
+__device__ __forceinline__ double jitter(uint32_t i, uint32_t j, int a, double r_p)
+{
+    uint64_t h = splitmix64(((uint64_t)i << 32) | (uint64_t)j);
+    h = splitmix64(h + (uint64_t)(a + 1));
+    return (double)(h % 1000u) / 1e3 * 2.0 * r_p - r_p;
+}
+
+__device__ __forceinline__ double vnorm(double x, double y, double z)
+{
+    // primitive.hpp:58, pow(pow(x,2)+pow(y,2)+pow(z,2),0.5)
+    return sqrt((x * x + y * y) + z * z);
+}
+
+// d = p_a - p_b with the reference's minimum image (cluster.cpp:223-232);
+// coincident particles get the jitter of the ordered pair (lo, hi), negated
+// when a > b so that F(a,b) = -F(b,a) bitwise.
+__device__ __forceinline__ void pair_disp(const PartParams &P, const PartState &S, int a, int b, double *d)
+{
+    d[0] = S.p[0][a] - S.p[0][b];
+    d[1] = S.p[1][a] - S.p[1][b];
+    d[2] = S.p[2][a] - S.p[2][b];
+    const double L[3] = {P.lx, P.ly, P.lz};
+    const double th[3] = {P.dcx * (P.ncx - 2), P.dcy * (P.ncy - 2), P.dcz * (P.ncz - 2)};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        if (d[q] > th[q]) d[q] -= L[q];
+        else if (d[q] < -th[q]) d[q] += L[q];
+    }
+    if (vnorm(d[0], d[1], d[2]) == 0.0) {
+        const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double jv = jitter(lo, hi, q, P.r_p);
+            d[q] = a < b ? jv : -jv;
+        }
+    }
+}
+
+// Force on a from b if in contact; returns false otherwise.
+__device__ __forceinline__ bool pair_force(const PartParams &P, const PartState &S, int a, int b, double rmp,
+                                           double *f, double *d)
+{
+    pair_disp(P, S, a, b, d);
+    const double nrm = vnorm(d[0], d[1], d[2]);
+    if (!(nrm < P.r_eq)) return false;
+    const double s = (P.r_eq - nrm) * rmp * P.k_n;
+    f[0] = (d[0] / nrm) * s;
+    f[1] = (d[1] / nrm) * s;
+    f[2] = (d[2] / nrm) * s;
+    return true;
+}
+
+__device__ __forceinline__ int wrapc(int v, int n) { return (v + n) % n; }
+
+constexpr int MAX_LOWER = 48;  // contacts with lower id held for ordering
+
+// One thread per particle (taken in cell-sorted order for locality).
+// f_pp[a] is summed in the order the reference accumulates it: first the
+// contributions of partners b < a in ascending b (they were added when the
+// outer loop visited b), then partners b > a in 27-cell traversal order.
+// tau_i[a] = sum over b > a of 0.5 F.d when f_pi[a] and f_pi[mark] are
+// non-zero (cluster.cpp:256-259).
+__global__ void k_contact(PartParams P, PartState S, const int *__restrict__ start, const int *__restrict__ ids,
+                          double rmp, int *__restrict__ overflow)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    // only the slots the sort filled (particles outside the range are left out)
+    if (s >= P.n || s >= start[P.ncx * P.ncy * P.ncz]) return;
+    const int a = ids[s];
+    int o, pp, q;
+    cell_of(P, S.p[0][a], S.p[1][a], S.p[2][a], &o, &pp, &q);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    // lower partners
+    int low[MAX_LOWER];
+    int nl = 0;
+    bool ovf = false;
+    for (int k = q - 1; k <= q + 1; k++)
+        for (int j = pp - 1; j <= pp + 1; j++)
+            for (int i = o - 1; i <= o + 1; i++) {
+                const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                for (int e = start[co]; e < start[co + 1]; e++) {
+                    const int b = ids[e];
+                    if (b >= a) continue;
+                    double d[3];
+                    pair_disp(P, S, a, b, d);
+                    if (vnorm(d[0], d[1], d[2]) < P.r_eq) {
+                        if (nl < MAX_LOWER) low[nl++] = b;
+                        else ovf = true;
+                    }
+                }
+            }
+    if (!ovf) {
+        for (int x = 1; x < nl; x++) {
+            const int v = low[x];
+            int y = x - 1;
+            while (y >= 0 && low[y] > v) { low[y + 1] = low[y]; y--; }
+            low[y + 1] = v;
+        }
+        for (int x = 0; x < nl; x++) {
+            double f[3], d[3];
+            pair_force(P, S, a, low[x], rmp, f, d);
+            fx = fx + f[0];
+            fy = fy + f[1];
+            fz = fz + f[2];
+        }
+    } else {
+        // too many lower partners to order: take them by repeated minimum
+        // search (same order, slower)
+        atomicOr(overflow, 1);
+        int last = -1;
+        for (;;) {
+            int best = 0x7fffffff;
+            for (int k = q - 1; k <= q + 1; k++)
+                for (int j = pp - 1; j <= pp + 1; j++)
+                    for (int i = o - 1; i <= o + 1; i++) {
+                        const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                        for (int e = start[co]; e < start[co + 1]; e++) {
+                            const int b = ids[e];
+                            if (b >= a || b <= last || b >= best) continue;
+                            double d[3];
+                            pair_disp(P, S, a, b, d);
+                            if (vnorm(d[0], d[1], d[2]) < P.r_eq) best = b;
+                        }
+                    }
+            if (best == 0x7fffffff) break;
+            double f[3], d[3];
+            pair_force(P, S, a, best, rmp, f, d);
+            fx = fx + f[0];
+            fy = fy + f[1];
+            fz = fz + f[2];
+            last = best;
+        }
+    }
+    // upper partners in traversal order
+    const int mk = *S.mark;
+    bool tau_on = false;
+    if (mk >= 0) {
+        tau_on = vnorm(S.fpi[0][a], S.fpi[1][a], S.fpi[2][a]) != 0.0 &&
+                 vnorm(S.fpi[0][mk], S.fpi[1][mk], S.fpi[2][mk]) != 0.0;
+    }
+    double tau = 0.0;
+    for (int k = q - 1; k <= q + 1; k++)
+        for (int j = pp - 1; j <= pp + 1; j++)
+            for (int i = o - 1; i <= o + 1; i++) {
+                const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                for (int e = start[co]; e < start[co + 1]; e++) {
+                    const int b = ids[e];
+                    if (b <= a) continue;
+                    double f[3], d[3];
+                    if (pair_force(P, S, a, b, rmp, f, d)) {
+                        fx = fx + f[0];
+                        fy = fy + f[1];
+                        fz = fz + f[2];
+                        if (tau_on) tau += 0.5 * ((f[0] * d[0] + f[1] * d[1]) + f[2] * d[2]);
+                    }
+                }
+            }
+    S.fpp[0][a] = fx;
+    S.fpp[1][a] = fy;
+    S.fpp[2][a] = fz;
+    S.tau_i[a] = tau;
+}
+
+// Contact pairs (a < b) per particle: count pass then write pass.
+__global__ void k_pairs(PartParams P, PartState S, const int *__restrict__ start, const int *__restrict__ ids,
+                        const int *__restrict__ pstart, int *__restrict__ pcount, int *__restrict__ pairs,
+                        int64_t max_pairs)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    if (S.cell[a] < 0) {  // outside the sorting range: flagged, no pairs
+        if (!pstart) pcount[a] = 0;
+        return;
+    }
+    int o, pp, q;
+    cell_of(P, S.p[0][a], S.p[1][a], S.p[2][a], &o, &pp, &q);
+    int cnt = 0;
+    const int base = pstart ? pstart[a] : 0;
+    for (int k = q - 1; k <= q + 1; k++)
+        for (int j = pp - 1; j <= pp + 1; j++)
+            for (int i = o - 1; i <= o + 1; i++) {
+                const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                for (int e = start[co]; e < start[co + 1]; e++) {
+                    const int b = ids[e];
+                    if (b <= a) continue;
+                    double d[3];
+                    pair_disp(P, S, a, b, d);
+                    if (vnorm(d[0], d[1], d[2]) < P.r_eq) {
+                        if (pstart && (int64_t)(base + cnt) < max_pairs) {
+                            pairs[2 * (base + cnt)] = a;
+                            pairs[2 * (base + cnt) + 1] = b;
+                        }
+                        cnt++;
+                    }
+                }
+            }
+    if (!pstart) pcount[a] = cnt;
+}
+
+// Sort each particle's pair segment by partner id.
+__global__ void k_pairs_sort(int n, const int *__restrict__ pstart, int *__restrict__ pairs, int64_t max_pairs)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const int b = pstart[a];
+    const int e = (int)min((int64_t)pstart[a + 1], max_pairs);
+    for (int i = b + 1; i < e; i++) {
+        const int v = pairs[2 * i + 1];
+        int j = i - 1;
+        while (j >= b && pairs[2 * j + 1] > v) { pairs[2 * (j + 1) + 1] = pairs[2 * j + 1]; j--; }
+        pairs[2 * (j + 1) + 1] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Trilinear interpolation, field::interpolate (field.cpp:223-248)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ double interp1(const PartParams &P, const T *__restrict__ v, double px, double py,
+                                          double pz)
+{
+    const int Nx = P.nx, Ny = P.ny, Nz = P.nz;
+    const int i = (int)floor(px / P.dx), j = (int)floor(py / P.dy), k = (int)floor(pz / P.dz);
+    const double m = px / P.dx - floor(px / P.dx);
+    const double n = py / P.dy - floor(py / P.dy);
+    const double o = pz / P.dz - floor(pz / P.dz);
+    const int i1 = (i + 1) % Nx, j1 = (j + 1) % Ny, k1 = (k + 1) % Nz;
+    const size_t sy = (size_t)Nx, sz = (size_t)Nx * Ny;
+    double r;
+    r = (1.0 - m) * (1.0 - n) * (1.0 - o) * (double)v[i + j * sy + k * sz];
+    r += (1.0 - m) * n * (1.0 - o) * (double)v[i + j1 * sy + k * sz];
+    r += m * (1.0 - n) * (1.0 - o) * (double)v[i1 + j * sy + k * sz];
+    r += m * n * (1.0 - o) * (double)v[i1 + j1 * sy + k * sz];
+    r += (1.0 - m) * (1.0 - n) * o * (double)v[i + j * sy + k1 * sz];
+    r += (1.0 - m) * n * o * (double)v[i + j1 * sy + k1 * sz];
+    r += m * (1.0 - n) * o * (double)v[i1 + j * sy + k1 * sz];
+    r += m * n * o * (double)v[i1 + j1 * sy + k1 * sz];
+    return r;
+}
+
+__device__ __forceinline__ bool grid_cell_ok(const PartParams &P, double px, double py, double pz)
+{
+    const int i = (int)floor(px / P.dx), j = (int)floor(py / P.dy), k = (int)floor(pz / P.dz);
+    return i >= 0 && i < P.nx && j >= 0 && j < P.ny && k >= 0 && k < P.nz;
+}
+
+// Interpolate c and grad c (element.cpp:55-71; u is not interpolated,
+// element.cpp:38-52), then f_pi (element.cpp:75-97), and the contact cell
+// histogram of the following sort (cluster.cpp:116-169).
+template <typename T>
+__global__ void k_interp_fpi_count(PartParams P, PartState S, const T *__restrict__ c, const T *__restrict__ g0,
+                                   const T *__restrict__ g1, const T *__restrict__ g2, int *__restrict__ cnt)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    const double px = S.p[0][a], py = S.p[1][a], pz = S.p[2][a];
+    if (!grid_cell_ok(P, px, py, pz)) {
+        atomicOr(S.err, 2);
+        S.cell[a] = -1;
+        return;
+    }
+    const double phi = interp1(P, c, px, py, pz);
+    const double gx = interp1(P, g0, px, py, pz);
+    const double gy = interp1(P, g1, px, py, pz);
+    const double gz = interp1(P, g2, px, py, pz);
+    S.cloc[a] = phi;
+    S.dcdx[0][a] = gx;
+    S.dcdx[1][a] = gy;
+    S.dcdx[2][a] = gz;
+    double f[3] = {0.0, 0.0, 0.0};
+    if (fabs(phi) <= P.tanh2) {
+        const double gn = vnorm(gx, gy, gz);
+        if (gn != 0.0) {
+            const double r = S.rad[a];
+            double d = 0.5 * log((1.0 + phi) / (1.0 - phi)) * P.sqrt2 * P.epsilon;
+            if (P.has_ctheta) {
+                const double ct = S.cth[a];
+                if (ct != 0.0) d = d - r * ct;
+            }
+            double fm = 0.0;
+            if (fabs(d) < r) fm = -P.a_pi * P.pi * d * P.sigma;
+            f[0] = gx / gn * fm;
+            f[1] = gy / gn * fm;
+            f[2] = gz / gn * fm;
+        }
+    }
+    S.fpi[0][a] = f[0];
+    S.fpi[1][a] = f[1];
+    S.fpi[2][a] = f[2];
+    int ci, cj, ck;
+    const int cc = cell_of(P, px, py, pz, &ci, &cj, &ck);
+    S.cell[a] = cc;
+    if (cc < 0) { atomicOr(S.err, 1); return; }
+    atomicAdd(&cnt[cc], 1);
+}
+
+// Heun stage (element.cpp:156-173; uf = 0, f_ext = 0) and periodic wrap
+// (element.cpp:177-186).
+__global__ void k_advect(PartParams P, PartState S, int stage)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    const double r = S.rad[a];
+    const double L[3] = {P.lx, P.ly, P.lz};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double fs = S.fpi[q][a] + 0.0 + S.fpp[q][a];
+        const double v = 0.0 + fs / 6.0 / P.pi / P.mu / r;
+        double x;
+        if (stage == 1) {
+            S.uo[q][a] = v;
+            S.po[q][a] = S.p[q][a];
+            x = S.p[q][a] + v * P.dt;
+        } else {
+            S.u[q][a] = v;
+            x = S.po[q][a] + (S.uo[q][a] + v) * (0.5 * P.dt);
+        }
+        S.p[q][a] = fmod(x + L[q], L[q]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spreading (cluster.cpp:344-410) as a line-sorted segmented reduction.
+// Each particle yields 8 entries (node x, particle id, force * weight); they
+// are bucketed by real grid line (y + ny*z), sorted by (x, id) inside each
+// line, and the x pass of the force transform sums them per node in that
+// order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void spread_corner(const PartParams &P, double px, double py, double pz, int *x, int *y,
+                                              int *z, double *o, double *p, double *q)
+{
+    *x = (int)floor(px / P.dx);
+    *y = (int)floor(py / P.dy);
+    *z = (int)floor(pz / P.dz);
+    *o = px / P.dx - floor(px / P.dx);
+    *p = py / P.dy - floor(py / P.dy);
+    *q = pz / P.dz - floor(pz / P.dz);
+}
+
+__global__ void k_spread_count(PartParams P, const double *__restrict__ px, const double *__restrict__ py,
+                               const double *__restrict__ pz, int *__restrict__ line_cnt, int *__restrict__ err)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    int x, y, z;
+    double o, p, q;
+    spread_corner(P, px[a], py[a], pz[a], &x, &y, &z, &o, &p, &q);
+    if (x < 0 || x >= P.nx || y < 0 || y >= P.ny || z < 0 || z >= P.nz) { atomicOr(err, 2); return; }
+    const int y1 = (y + 1) % P.ny, z1 = (z + 1) % P.nz;
+    atomicAdd(&line_cnt[y + P.ny * z], 2);
+    atomicAdd(&line_cnt[y1 + P.ny * z], 2);
+    atomicAdd(&line_cnt[y + P.ny * z1], 2);
+    atomicAdd(&line_cnt[y1 + P.ny * z1], 2);
+}
+
+template <typename T>
+__global__ void k_spread_scatter(PartParams P, const double *__restrict__ px, const double *__restrict__ py,
+                                 const double *__restrict__ pz, const double *__restrict__ f0,
+                                 const double *__restrict__ f1, const double *__restrict__ f2,
+                                 const int *__restrict__ line_start, int *__restrict__ cursor,
+                                 SpreadEntry<T> *__restrict__ out)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    int x, y, z;
+    double o, p, q;
+    spread_corner(P, px[a], py[a], pz[a], &x, &y, &z, &o, &p, &q);
+    if (x < 0 || x >= P.nx || y < 0 || y >= P.ny || z < 0 || z >= P.nz) return;
+    // f = (f_pp + f_ext)/dx/dy/dz, f_ext = 0
+    const double f[3] = {(f0[a] + 0.0) / P.dx / P.dy / P.dz, (f1[a] + 0.0) / P.dx / P.dy / P.dz,
+                         (f2[a] + 0.0) / P.dx / P.dy / P.dz};
+    const int x1 = (x + 1) % P.nx, y1 = (y + 1) % P.ny, z1 = (z + 1) % P.nz;
+    // corners in the reference order (cluster.cpp:368-406)
+    const int cx[8] = {x, x1, x1, x1, x1, x, x, x};
+    const int cy[8] = {y, y, y1, y, y1, y, y1, y1};
+    const int cz[8] = {z, z, z, z1, z1, z1, z, z1};
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        SpreadEntry<T> e;
+        e.x = cx[c];
+        e.pid = a;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            double w;
+            switch (c) {
+            case 0: w = f[d] * (1 - o) * (1 - p) * (1 - q); break;
+            case 1: w = f[d] * o * (1 - p) * (1 - q); break;
+            case 2: w = f[d] * o * p * (1 - q); break;
+            case 3: w = f[d] * o * (1 - p) * q; break;
+            case 4: w = f[d] * o * p * q; break;
+            case 5: w = f[d] * (1 - o) * (1 - p) * q; break;
+            case 6: w = f[d] * (1 - o) * p * (1 - q); break;
+            default: w = f[d] * (1 - o) * p * q; break;
+            }
+            e.v[d] = (T)w;
+        }
+        const int line = cy[c] + P.ny * cz[c];
+        const int slot = atomicAdd(&cursor[line], 1);
+        out[line_start[line] + slot] = e;
+    }
+}
+
+template <typename T>
+__global__ void k_spread_sort(int nlines, const int *__restrict__ line_start, SpreadEntry<T> *__restrict__ e)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlines) return;
+    const int b = line_start[l], en = line_start[l + 1];
+    for (int i = b + 1; i < en; i++) {
+        const SpreadEntry<T> v = e[i];
+        int j = i - 1;
+        while (j >= b && (e[j].x > v.x || (e[j].x == v.x && e[j].pid > v.pid))) {
+            e[j + 1] = e[j];
+            j--;
+        }
+        e[j + 1] = v;
+    }
+}
+
+// Dense spread (unit entry point): one thread per grid line sums its sorted
+// entries per node into a zeroed 3-component grid.
+template <typename T>
+__global__ void k_spread_dense(int nlines, int nx, size_t G, const int *__restrict__ line_start,
+                               const SpreadEntry<T> *__restrict__ e, T *__restrict__ grid3)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlines) return;
+    int i = line_start[l];
+    const int en = line_start[l + 1];
+    while (i < en) {
+        const int x = e[i].x;
+        T s0 = T(0), s1 = T(0), s2 = T(0);
+        for (; i < en && e[i].x == x; i++) {
+            s0 += e[i].v[0];
+            s1 += e[i].v[1];
+            s2 += e[i].v[2];
+        }
+        const size_t node = (size_t)l * nx + x;
+        grid3[node] += s0;
+        grid3[G + node] += s1;
+        grid3[2 * G + node] += s2;
+    }
+}
+
+}  // namespace fipi

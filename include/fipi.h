@@ -1,0 +1,173 @@
+/*
+ * fipi.h -- C ABI of the MI355X-native FIPI time step (libfipi.so).
+ *
+ * The reference (FIPI, a serial C++ code) has no plugin or FFI layer: its
+ * boundary is the C++ class API that src/main.cpp drives (`field`,
+ * `cluster`, `element`, `list`) plus FFTW plan calls.  This header is the
+ * thin C layer those classes call instead (the C++ facade in
+ * fipi_amd/facade/ keeps the reference class names).  Plain pointers and
+ * sizes only; every call returns a status code (0 = success) and never
+ * exits, where the reference prints and calls exit(0) on misuse.
+ *
+ * Layouts
+ *   real grid field   double[nz][ny][nx], x fastest (it = i + nx*(j + ny*k),
+ *                     src/field.cpp:88-92)
+ *   vector field      3 such arrays back to back (component-major)
+ *   particle vectors  double[n][3] (x, y, z), as `vector` in primitive.hpp:37-43
+ * Host pointers are copied in and out; "device" pointers are HIP device
+ * memory used on the context's stream.
+ */
+#ifndef FIPI_H
+#define FIPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FIPI_OK 0
+#define FIPI_ERR_ARG (-1)         /* invalid argument or size */
+#define FIPI_ERR_HIP (-2)         /* HIP runtime error */
+#define FIPI_ERR_UNSUPPORTED (-3) /* grid size or option not built */
+#define FIPI_ERR_RANGE (-4)       /* particle outside the sorting range (cluster.cpp:144-163) */
+#define FIPI_ERR_STATE (-5)       /* call out of order (e.g. step before set_state) */
+#define FIPI_ERR_OVERFLOW (-6)    /* a fixed-capacity buffer was too small */
+
+#define FIPI_FP64 0        /* every field and particle quantity in FP64 */
+#define FIPI_FP32_MIXED 1  /* FFTs, k-space update, interp/spread in FP32; k=0 mode,
+                              mean phi, contact overlaps, positions in FP64 */
+
+/* Runtime parameters.  The reference holds these as compile-time constants
+ * (src/head.hpp:27-74, src/parameter.hpp:18-20); field names follow it. */
+typedef struct fipi_params {
+    int32_t nx, ny, nz;         /* grid, head.hpp:29-31 (each 16, 32, 64, 128 or 256) */
+    int32_t n_particles;        /* head.hpp:63 */
+    double lx, ly, lz;          /* box, head.hpp:33-35 */
+    double pi;                  /* head.hpp:27: the reference uses 3.1415926 */
+    double dt;                  /* head.hpp:41 */
+    double ramp_t;              /* contact ramp length 0.2*t_end, cluster.cpp:249 */
+    double epsilon;             /* interface width, head.hpp:46 */
+    double lambda;              /* gradient energy coefficient, head.hpp:47 */
+    double mobility;            /* head.hpp:58 */
+    double mu;                  /* fluid viscosity, head.hpp:54 */
+    double sigma;               /* surface tension, head.hpp:45 */
+    double a_pi;                /* attachment constant A, head.hpp:57 */
+    double k_n;                 /* contact stiffness, parameter.hpp:18 */
+    double r_p;                 /* particle radius, head.hpp:59 */
+    double r_eq;                /* contact cutoff, head.hpp:64 */
+    double r_c;                 /* contact cell size target, head.hpp:65 */
+    double margin;              /* parameter.hpp:20 */
+    double rho1, rho2;          /* head.hpp:52-53 */
+    double gravity;             /* head.hpp:51 */
+    double tanh2;               /* tanh(2.0), element.cpp:84 */
+    int32_t switch_particle;    /* head.hpp:72 */
+    int32_t switch_ch;          /* head.hpp:73 (must be 1) */
+    int32_t switch_ns;          /* head.hpp:74 */
+    int32_t precision;          /* FIPI_FP64 or FIPI_FP32_MIXED */
+} fipi_params;
+
+/* Diagnostics (printed by the reference at output steps). */
+typedef struct fipi_diag {
+    int32_t t;                  /* steps taken */
+    int32_t it_mark;            /* argmax-y particle of the last sort, cluster.cpp:134-138 */
+    double mean_phi;            /* FP64 mean of phi */
+    double free_energy;         /* sum lambda(1/2|grad phi|^2 + 1/4 eps^-2 (phi^2-1)^2) dx^3,
+                                   main.cpp:232 + field.cpp:506-516 */
+    double interface_area;      /* sum |grad phi| dx^3 */
+    double tau;                 /* packing energy, cluster.cpp:256-259,455 */
+} fipi_diag;
+
+typedef struct fipi_ctx fipi_ctx;
+
+/* ---- context --------------------------------------------------------- */
+/* Replaces the setup of main.cpp:29-73 (field/cluster construction and
+ * FFTW plans).  Allocates all device state for one box on the current
+ * HIP device and creates one stream. */
+int fipi_create(const fipi_params *params, fipi_ctx **out);
+int fipi_destroy(fipi_ctx *ctx);
+/* Last error message of ctx (or of the calling thread when ctx is NULL). */
+const char *fipi_last_error(const fipi_ctx *ctx);
+/* Context stream (hipStream_t) as an opaque pointer. */
+void *fipi_stream(fipi_ctx *ctx);
+
+/* ---- state ----------------------------------------------------------- */
+/* Initial state: phi (G reals, replaces field("SCALARS","phase_field"),
+ * field.cpp:85-99), particle positions (3n, replaces cluster::distribute,
+ * cluster.cpp:53-92), optional per-particle radius and cos(contact angle)
+ * (NULL: r_p and neutral wetting, the reference).  Velocity is set to zero,
+ * the step counter to 0. */
+int fipi_set_state(fipi_ctx *ctx, const double *phi, const double *pos, const double *radius,
+                   const double *cos_theta);
+/* Replace the fluid velocity u (3G reals). */
+int fipi_set_velocity(fipi_ctx *ctx, const double *u);
+/* Copy out phi (G), u (3G), grad phi (3G). */
+int fipi_get_phi(fipi_ctx *ctx, double *phi);
+int fipi_get_velocity(fipi_ctx *ctx, double *u);
+int fipi_get_grad_phi(fipi_ctx *ctx, double *grad);
+/* Particle state, each array 3n (NULL to skip): position, previous
+ * position, velocity, previous velocity, f_pp, f_pi (element.hpp:28-58). */
+int fipi_get_particles(fipi_ctx *ctx, double *pos, double *pos_old, double *u, double *u_old, double *f_pp,
+                       double *f_pi);
+int fipi_get_time(fipi_ctx *ctx, int32_t *t);
+int fipi_diagnostics(fipi_ctx *ctx, fipi_diag *out);
+
+/* ---- the time step ---------------------------------------------------- */
+/* Advance nsteps steps of the reference time loop body (main.cpp:75-222):
+ * Cahn-Hilliard SBDF2 update, Stokes solve, two Heun particle stages.
+ * Asynchronous on the context stream except for the error check at the
+ * end of the call. */
+int fipi_step(fipi_ctx *ctx, int32_t nsteps);
+/* Same as fipi_step without the final synchronisation and error check
+ * (for timing loops); errors are reported by the next fipi_sync. */
+int fipi_step_async(fipi_ctx *ctx, int32_t nsteps);
+int fipi_sync(fipi_ctx *ctx);
+
+/* ---- unit entry points (device pointers, context stream, synchronous) -- */
+/* 3-D real-to-complex forward transform, unnormalised (FFTW_FORWARD sign),
+ * of one real field into the half spectrum [nz][ny][nx/2+1] complex
+ * (interleaved re,im).  Replaces field::fft (field.cpp:438-443) for a real
+ * field.  Element type: double for FIPI_FP64 contexts, float otherwise. */
+int fipi_fft_r2c(fipi_ctx *ctx, const void *d_real, void *d_half);
+/* Inverse (complex-to-real) of the half spectrum, normalised by 1/G as
+ * field::ifft + normalize (field.cpp:447-455, primitive.cpp:54-62).  The
+ * input is not preserved. */
+int fipi_fft_c2r(fipi_ctx *ctx, void *d_half, void *d_real);
+
+/* Cell list of n particles (d_pos: double[n][3]) on the contact grid,
+ * replaces cluster::sorting (cluster.cpp:116-169): d_cell_start[ncell+1],
+ * d_ids[n] with ids ascending in each cell.  *ncell_out gets ncell. */
+int fipi_cells_build(fipi_ctx *ctx, const double *d_pos, int32_t n, int32_t *d_cell_start, int32_t *d_ids,
+                     int32_t *ncell_out);
+/* Contact pairs (i<j, |d| < r_eq) of the 27-cell search of
+ * cluster::compute_f_pp (cluster.cpp:193-262), sorted by (i, j).  Writes up
+ * to max_pairs pairs to d_pairs[2*max_pairs]; *count gets the total. */
+int fipi_contact_pairs(fipi_ctx *ctx, const double *d_pos, int32_t n, int32_t *d_pairs, int64_t max_pairs,
+                       int64_t *count);
+/* Contact forces f_pp (double[n][3]) at ramp time t, cluster.cpp:174-275. */
+int fipi_contact_forces(fipi_ctx *ctx, const double *d_pos, int32_t n, int32_t t, double *d_f_pp);
+/* Trilinear interpolation of a real grid field (element type as
+ * fipi_fft_r2c) to n particles, field::interpolate (field.cpp:223-248). */
+int fipi_interp(fipi_ctx *ctx, const void *d_grid, const double *d_pos, int32_t n, double *d_out);
+/* Spread n particle forces (double[n][3]) divided by dx*dy*dz onto a
+ * 3-component real grid (element type as fipi_fft_r2c), overwriting it,
+ * cluster::compute_f_r (cluster.cpp:344-410).  Deterministic: per grid node
+ * the contributions are summed in ascending particle order. */
+int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32_t n, void *d_grid3);
+
+/* ---- seeded synthetic inputs (host only, no GPU needed) ---------------- */
+/* phi[i] = mean + U(-amp, amp) from a counter-based generator. */
+int fipi_init_phi(int64_t g, double mean, double amp, uint64_t seed, double *phi);
+/* Random sequential addition of up to n spheres of radius r with minimum
+ * surface gap `gap` in the periodic box (lx, ly, lz); positions written to
+ * pos[3n].  Returns the number placed (< n when the box jams within
+ * max_attempts tries per sphere), or a negative status. */
+int64_t fipi_init_rsa(int64_t n, double r, double gap, double lx, double ly, double lz, uint64_t seed,
+                      int64_t max_attempts, double *pos);
+/* out[i] = U(lo, hi) from a counter-based generator (e.g. contact angles). */
+int fipi_init_uniform(int64_t n, double lo, double hi, uint64_t seed, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

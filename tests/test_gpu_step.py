@@ -1,0 +1,90 @@
+"""FP64 GPU time step against the CPU oracle (restatement of
+src/main.cpp:75-222) on identical seeded inputs.
+
+Bar: relative L2 error of phi, u and particle positions <= 1e-10 (the
+north-star FP64 tolerance); discrete sets (cells, contact pairs) bit-exact
+in test_gpu_particles.py."""
+import numpy as np
+import pytest
+
+import fipi_amd
+import oracle
+from _cases import make_case, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _run_both(p, phi0, pos, steps, chunks=1):
+    sim = fipi_amd.Simulation(p)
+    sim.set_state(phi0, pos)
+    ref = oracle.Sim(p, phi0, pos)
+    per = steps // chunks
+    out = []
+    for _ in range(chunks):
+        sim.step(per)
+        ref.step(per)
+        out.append((sim.t, sim.phi(), ref.phi(), sim.velocity(), ref.u(),
+                    sim.particles() if sim.n else None, ref.particles() if sim.n else None))
+    sim.close()
+    return out
+
+
+@pytest.mark.parametrize("ns", [0, 1])
+def test_ch_only_32(gpu, ns):
+    print(f"ch-only ns={ns}", flush=True)
+    p, phi0, _ = make_case(32, 0, steps=20, ns=ns, seed=3)
+    for t, g_phi, r_phi, g_u, r_u, _, _ in _run_both(p, phi0, None, 20, chunks=4):
+        e = rel_l2(g_phi, r_phi)
+        assert e < TOL, (t, e)
+        if ns:
+            assert rel_l2(g_u, r_u) < TOL, (t, rel_l2(g_u, r_u))
+
+
+def test_step_with_particles_32(gpu):
+    print("particles 32^3", flush=True)
+    p, phi0, pos = make_case(32, 150, steps=40, seed=5)
+    for t, g_phi, r_phi, g_u, r_u, gp, rp in _run_both(p, phi0, pos, 40, chunks=4):
+        assert rel_l2(g_phi, r_phi) < TOL, (t, rel_l2(g_phi, r_phi))
+        assert rel_l2(g_u, r_u) < TOL, (t, rel_l2(g_u, r_u))
+        assert rel_l2(gp["p"], rp["p"]) < TOL, (t, rel_l2(gp["p"], rp["p"]))
+        assert rel_l2(gp["f_pp"], rp["f_pp"]) < 1e-8, (t, rel_l2(gp["f_pp"], rp["f_pp"]))
+        assert rel_l2(gp["f_pi"], rp["f_pi"]) < 1e-8, (t, rel_l2(gp["f_pi"], rp["f_pi"]))
+
+
+def test_step_config1_64(gpu):
+    """64^3 benchmark units with 2.5 dx spheres (config 1 physics), 10 steps."""
+    print("config1 64^3", flush=True)
+    p, phi0, pos = make_case(64, 800, r_p=2.5, steps=200, seed=7)
+    (t, g_phi, r_phi, g_u, r_u, gp, rp), = _run_both(p, phi0, pos, 10)
+    assert rel_l2(g_phi, r_phi) < TOL
+    assert rel_l2(g_u, r_u) < TOL
+    assert rel_l2(gp["p"], rp["p"]) < TOL
+
+
+def test_reference_defaults_64(gpu):
+    """The reference's own constants (head.hpp: L = 2 pi, dt = 1e-3,
+    r_eq = 5 r_p), 400 particles, 5 steps."""
+    print("reference defaults", flush=True)
+    p = fipi_amd.reference_params(n_particles=400)
+    phi0 = fipi_amd.init_phi((64, 64, 64), 0.0, 0.1, 11)
+    pos = fipi_amd.init_rsa(400, p.r_p, 0.0, (p.lx, p.ly, p.lz), 12)
+    assert pos.shape[0] == 400
+    (t, g_phi, r_phi, g_u, r_u, gp, rp), = _run_both(p, phi0, pos, 5)
+    assert rel_l2(g_phi, r_phi) < TOL
+    assert rel_l2(g_u, r_u) < TOL
+    assert rel_l2(gp["p"], rp["p"]) < TOL
+
+
+def test_determinism(gpu):
+    p, phi0, pos = make_case(32, 150, steps=20, seed=9)
+    outs = []
+    for _ in range(2):
+        sim = fipi_amd.Simulation(p)
+        sim.set_state(phi0, pos)
+        sim.step(10)
+        outs.append((sim.phi(), sim.velocity(), sim.particles()["p"]))
+        sim.close()
+    for a, b in zip(outs[0], outs[1]):
+        assert np.array_equal(a, b)
