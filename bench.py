@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Benchmark of the FIPI time step on one MI355X.
+
+Workload (the reference's real-scale configuration): 256^3 periodic box,
+dx = 1, Cahn-Hilliard (eps = 1, M = 1, kappa = 1) coupled to Stokes
+(eta = 1), semi-implicit spectral step dt = 0.01; phi0 = U(-0.05, 0.05)
+from a seed; 100,000 rigid spheres of radius 2 dx placed by seeded random
+sequential addition (gap 0.5 dx), contact angles U(60, 120) deg.
+
+One step = one pass of the reference time loop body (src/main.cpp:75-222):
+CH SBDF2 update, Stokes solve, two Heun particle stages.  Inputs are
+resident in HBM before the timed region.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "steps/s at 256^3+100k particles (FP64 vs FP32-mixed @ stated err), HBM GB/s/peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def workload(n, n_particles, precision, seed=1):
+    import fipi_amd
+    p = fipi_amd.config_params(n, n_particles, 2.0, 1000, precision=precision)
+    phi0 = fipi_amd.init_phi((n, n, n), 0.0, 0.05, seed)
+    pos = fipi_amd.init_rsa(n_particles, 2.0, 0.5, (p.lx, p.ly, p.lz), seed + 1)
+    if pos.shape[0] != n_particles:
+        raise SystemExit(f"RSA placed only {pos.shape[0]} of {n_particles} spheres")
+    theta = fipi_amd.init_uniform(n_particles, 60.0, 120.0, seed + 2)
+    cos_theta = np.cos(np.deg2rad(theta))
+    return p, phi0, pos, cos_theta
+
+
+def step_bytes(p, n_particles):
+    """Algorithmic HBM bytes of one step with the minimal dataflow
+    (every array read or written once per stage): 20 real fields + 18 half
+    spectra + 2 Heun stages of particle state, gathers and the spread."""
+    s = 8 if p.precision == 0 else 4
+    R = p.G * s
+    H = p.nz * p.ny * (p.nx // 2 + 1) * 2 * s
+    bp = 2 * (6 * 24 + 32 * s) + 24 * s
+    return 20 * R + 18 * H + n_particles * bp
+
+
+def cpu_baseline(p, phi0, pos, cos_theta, steps):
+    """The oracle (C restatement of the reference step, 25 complex 3-D FFTs
+    per step like the reference) timed on this host's cores."""
+    import oracle
+    threads = min(os.cpu_count() or 1, 16)
+    sim = oracle.Sim(p, phi0, pos, None, cos_theta, n_threads=threads)
+    sim.step(1)  # first-order start step, untimed
+    t0 = time.perf_counter()
+    sim.step(steps)
+    dt = time.perf_counter() - t0
+    del sim
+    return {
+        "value": steps / dt,
+        "unit": "steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} reference-structured FP64 steps (25 complex 3-D FFTs each, "
+                  f"OpenMP) of the same {p.nx}^3 + {pos.shape[0]} workload after 1 warm-up step, "
+                  f"{dt:.1f} s wall",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--particles", type=int, default=100000)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=3)
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build measures one GPU only (one box, one MI355X); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import fipi_amd
+
+    prec = fipi_amd.FP64 if args.precision == "fp64" else fipi_amd.FP32_MIXED
+    p, phi0, pos, cos_theta = workload(args.n, args.particles, prec)
+    sim = fipi_amd.Simulation(p)
+    sim.set_state(phi0, pos, None, cos_theta)
+
+    sim.step(args.warmup)
+    stream = torch.cuda.ExternalStream(sim.stream_ptr)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    sim.sync()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    sim.step_async(args.steps)
+    e1.record(stream)
+    sim.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1)
+    ms_per_step = wall * 1e3 / args.steps
+    value = args.steps / wall
+
+    # per-kernel HIP-event profile on the context stream (separate pass)
+    stats = sim.profile_steps(args.profile_steps)
+    for s in stats:
+        s["avg_ms"] = s["total_ms"] / max(s["launches"], 1)
+        s["bytes_per_launch"] = s["bytes"] / max(s["launches"], 1)
+        s["gbs"] = s["bytes_per_launch"] / (s["avg_ms"] * 1e-3) / 1e9 if s["avg_ms"] > 0 else 0.0
+    stats.sort(key=lambda s: -s["total_ms"])
+    dom = stats[0]
+    traffic = None
+    tr_path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tr_path):
+        with open(tr_path) as f:
+            tr = json.load(f)
+        key = f"{args.precision}:{dom['name']}"
+        if key in tr:
+            traffic = tr[key]
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom["name"],
+        "achieved": round(dom["gbs"], 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "bytes_per_launch": dom["bytes_per_launch"],
+        "avg_launch_ms": round(dom["avg_ms"], 5),
+    }
+    bstep = step_bytes(p, args.particles)
+    diag = sim.diagnostics()
+    sim.close()
+    del sim
+    torch.cuda.synchronize()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(p, phi0, pos, cos_theta, args.cpu_steps)
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == fipi_amd.FP64 else "f32-mixed",
+        "data": "synthetic: phi0 = U(-0.05,0.05) (seed 1), RSA spheres r=2dx gap 0.5dx (seed 2), "
+                "contact angles U(60,120) deg (seed 3)",
+        "config": {
+            "workload": f"{args.n}^3 periodic CH+Stokes + {args.particles} spheres r=2dx, dt=0.01, "
+                        f"{'FP64' if prec == fipi_amd.FP64 else 'FP32-mixed'}",
+            "grid": args.n,
+            "particles": args.particles,
+            "precision": args.precision,
+        },
+        "event_ms_per_step": round(ev_ms / args.steps, 4),
+        "grid_point_steps_per_s": value * p.G,
+        "step_roofline": {
+            "bytes_per_step": bstep,
+            "achieved_gbs": round(bstep * value / 1e9, 1),
+            "frac": round(bstep * value / 1e9 / HBM_PEAK_GBS, 4),
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "kernels": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()
+                     if k in ("name", "launches", "avg_ms", "gbs")} for s in stats[:12]],
+        "diagnostics": {k: diag[k] for k in ("t", "mean_phi", "free_energy", "interface_area")},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -33,7 +33,7 @@ ABI_SYMBOLS = [
     "fipi_create", "fipi_destroy", "fipi_last_error", "fipi_stream",
     "fipi_set_state", "fipi_set_velocity", "fipi_get_phi", "fipi_get_velocity",
     "fipi_get_grad_phi", "fipi_get_particles", "fipi_get_time", "fipi_diagnostics",
-    "fipi_step", "fipi_step_async", "fipi_sync",
+    "fipi_step", "fipi_step_async", "fipi_sync", "fipi_profile_steps",
     "fipi_fft_r2c", "fipi_fft_c2r", "fipi_cells_build", "fipi_contact_pairs",
     "fipi_contact_forces", "fipi_interp", "fipi_spread",
     "fipi_init_phi", "fipi_init_rsa", "fipi_init_uniform",
@@ -42,6 +42,11 @@ ABI_SYMBOLS = [
 
 class FipiError(RuntimeError):
     pass
+
+
+class CKernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 40), ("launches", C.c_int32), ("total_ms", C.c_double),
+                ("bytes", C.c_double)]
 
 
 class CParams(C.Structure):
@@ -104,6 +109,7 @@ def lib():
     L.fipi_step.argtypes = [vp, C.c_int32]
     L.fipi_step_async.argtypes = [vp, C.c_int32]
     L.fipi_sync.argtypes = [vp]
+    L.fipi_profile_steps.argtypes = [vp, C.c_int32, C.POINTER(CKernelStat), C.c_int32, ip]
     L.fipi_fft_r2c.argtypes = [vp, vp, vp]
     L.fipi_fft_c2r.argtypes = [vp, vp, vp]
     L.fipi_cells_build.argtypes = [vp, vp, C.c_int32, vp, vp, ip]
@@ -206,6 +212,19 @@ class Simulation:
 
     def sync(self):
         self._c(lib().fipi_sync(self._h))
+
+    def profile_steps(self, n: int = 1) -> list:
+        """Run n steps with per-launch HIP events; returns one dict per launch
+        site: name, launches, total_ms, bytes (algorithmic)."""
+        arr = (CKernelStat * 64)()
+        cnt = C.c_int32()
+        self._c(lib().fipi_profile_steps(self._h, n, arr, 64, C.byref(cnt)))
+        return [{"name": arr[i].name.decode(), "launches": arr[i].launches,
+                 "total_ms": arr[i].total_ms, "bytes": arr[i].bytes} for i in range(min(cnt.value, 64))]
+
+    @property
+    def stream_ptr(self) -> int:
+        return lib().fipi_stream(self._h) or 0
 
     def phi(self) -> np.ndarray:
         p = self.params

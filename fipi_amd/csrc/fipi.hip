@@ -82,6 +82,16 @@ struct fipi_ctx {
     int *h_flags = nullptr;  // pinned
     double *d_red = nullptr; // reduction scratch
     std::vector<void *> allocs;
+    // per-launch timing (fipi_profile_steps)
+    struct ProfRec {
+        const char *name;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    bool prof_on = false;
+    std::vector<ProfRec> prof;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
 };
 
 namespace {
@@ -138,6 +148,37 @@ int threads_per_line(int n)
     with_n(n, [&](auto NC) { r = LineShape<decltype(NC)::value>::P; });
     return r;
 }
+
+// Timing scope around one launch site when profiling is on: HIP events on
+// the context stream, plus the site's algorithmic bytes (each array the
+// kernel must touch, read or written once).
+hipEvent_t prof_event(fipi_ctx *ctx)
+{
+    if (ctx->ev_used == ctx->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->ev_pool.push_back(e);
+    }
+    return ctx->ev_pool[ctx->ev_used++];
+}
+
+struct ProfScope {
+    fipi_ctx *ctx;
+    size_t idx = (size_t)-1;
+    ProfScope(fipi_ctx *c, const char *name, double bytes) : ctx(c)
+    {
+        if (!ctx->prof_on) return;
+        fipi_ctx::ProfRec r{name, bytes, prof_event(ctx), prof_event(ctx)};
+        if (!r.a || !r.b) return;
+        (void)hipEventRecord(r.a, ctx->stream);
+        idx = ctx->prof.size();
+        ctx->prof.push_back(r);
+    }
+    ~ProfScope()
+    {
+        if (idx != (size_t)-1) (void)hipEventRecord(ctx->prof[idx].b, ctx->stream);
+    }
+};
 
 // at least one block: every kernel bounds-checks its index
 inline dim3 grid1(size_t n, int b) { return dim3((unsigned)(n == 0 ? 1 : (n + b - 1) / b)); }
@@ -254,36 +295,67 @@ struct Impl {
     {
         hipStream_t st = ctx->stream;
         const int nl = ctx->nlines;
-        hipLaunchKernelGGL(k_fill_int, grid1(nl, 256), dim3(256), 0, st, ctx->line_cnt, nl, 0);
-        hipLaunchKernelGGL(k_spread_count, grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, ctx->line_cnt,
-                           ctx->d_flags);
-        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, ctx->line_cnt, ctx->line_start, nl,
-                           (const double *)nullptr, 0, (int *)nullptr);
-        hipLaunchKernelGGL(k_fill_int, grid1(nl, 256), dim3(256), 0, st, ctx->line_cursor, nl, 0);
-        hipLaunchKernelGGL((k_spread_scatter<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, f0, f1, f2,
-                           ctx->line_start, ctx->line_cursor, (SpreadEntry<T> *)ctx->spr);
-        hipLaunchKernelGGL((k_spread_sort<T>), grid1(nl, 256), dim3(256), 0, st, nl, ctx->line_start,
-                           (SpreadEntry<T> *)ctx->spr);
+        const double n = pp.n, E = (double)sizeof(SpreadEntry<T>);
+        {
+            ProfScope ps_(ctx, "spread_fill", 8.0 * nl);
+            hipLaunchKernelGGL(k_fill_int, grid1(nl, 256), dim3(256), 0, st, ctx->line_cnt, nl, 0);
+            hipLaunchKernelGGL(k_fill_int, grid1(nl, 256), dim3(256), 0, st, ctx->line_cursor, nl, 0);
+        }
+        {
+            ProfScope ps_(ctx, "spread_count", 24.0 * n + 32.0 * n);
+            hipLaunchKernelGGL(k_spread_count, grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, ctx->line_cnt,
+                               ctx->d_flags);
+        }
+        {
+            ProfScope ps_(ctx, "spread_scan", 8.0 * nl);
+            hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, ctx->line_cnt, ctx->line_start, nl,
+                               (const double *)nullptr, 0, (int *)nullptr);
+        }
+        {
+            ProfScope ps_(ctx, "spread_scatter", 48.0 * n + 8.0 * n * E);
+            hipLaunchKernelGGL((k_spread_scatter<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, f0, f1,
+                               f2, ctx->line_start, ctx->line_cursor, (SpreadEntry<T> *)ctx->spr);
+        }
+        {
+            ProfScope ps_(ctx, "spread_sort", 16.0 * n * E + 4.0 * nl);
+            hipLaunchKernelGGL((k_spread_sort<T>), grid1(nl, 256), dim3(256), 0, st, nl, ctx->line_start,
+                               (SpreadEntry<T> *)ctx->spr);
+        }
     }
 
     static void cells_build(fipi_ctx *ctx, const PartParams &pp, const PartState &ps, int *ids, int *start,
                             bool interp)
     {
         hipStream_t st = ctx->stream;
-        hipLaunchKernelGGL(k_fill_int, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->cell_cnt, ctx->ncell, 0);
+        const double n = pp.n, nc = ctx->ncell;
+        {
+            ProfScope ps_(ctx, "cells_fill", 8.0 * nc);
+            hipLaunchKernelGGL(k_fill_int, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->cell_cnt, ctx->ncell, 0);
+            hipLaunchKernelGGL(k_fill_int, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->cell_cursor, ctx->ncell,
+                               0);
+        }
         if (interp) {
+            ProfScope ps_(ctx, "interp_fpi_count", n * (24.0 + 32.0 * sizeof(T) + 16.0 + 56.0 + 4.0));
             hipLaunchKernelGGL((k_interp_fpi_count<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, ps, R(ctx->c),
                                R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), ctx->cell_cnt);
         } else {
             hipLaunchKernelGGL(k_cell_count, grid1(pp.n, 256), dim3(256), 0, st, pp, ps.p[0], ps.p[1], ps.p[2],
                                ps.cell, ctx->cell_cnt, ps.err);
         }
-        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, ctx->cell_cnt, start, ctx->ncell,
-                           (const double *)ps.p[1], pp.n, ps.mark);
-        hipLaunchKernelGGL(k_fill_int, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->cell_cursor, ctx->ncell, 0);
-        hipLaunchKernelGGL(k_cell_scatter, grid1(pp.n, 256), dim3(256), 0, st, pp.n, ps.cell, start,
-                           ctx->cell_cursor, ids);
-        hipLaunchKernelGGL(k_cell_sort, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->ncell, start, ids);
+        {
+            ProfScope ps_(ctx, "cells_scan_mark", 8.0 * nc + 8.0 * n);
+            hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, ctx->cell_cnt, start, ctx->ncell,
+                               (const double *)ps.p[1], pp.n, ps.mark);
+        }
+        {
+            ProfScope ps_(ctx, "cells_scatter", 12.0 * n);
+            hipLaunchKernelGGL(k_cell_scatter, grid1(pp.n, 256), dim3(256), 0, st, pp.n, ps.cell, start,
+                               ctx->cell_cursor, ids);
+        }
+        {
+            ProfScope ps_(ctx, "cells_sort", 4.0 * nc + 8.0 * n);
+            hipLaunchKernelGGL(k_cell_sort, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->ncell, start, ids);
+        }
     }
 
     static double ramp(double t, double thr)
@@ -300,9 +372,15 @@ struct Impl {
         const PartParams &pp = ctx->pp;
         cells_build(ctx, pp, ctx->ps, ctx->cell_ids, ctx->cell_start, true);
         const double rmp = ramp((double)(ctx->t + 1), ctx->p.ramp_t);
-        hipLaunchKernelGGL(k_contact, grid1(pp.n, 128), dim3(128), 0, st, pp, ctx->ps, ctx->cell_start,
-                           ctx->cell_ids, rmp, ctx->d_flags + 1);
-        hipLaunchKernelGGL(k_advect, grid1(pp.n, 256), dim3(256), 0, st, pp, ctx->ps, stage);
+        {
+            ProfScope ps_(ctx, "contact", pp.n * (24.0 + 4.0 + 24.0 + 8.0 + 24.0));
+            hipLaunchKernelGGL(k_contact, grid1(pp.n, 128), dim3(128), 0, st, pp, ctx->ps, ctx->cell_start,
+                               ctx->cell_ids, rmp, ctx->d_flags + 1);
+        }
+        {
+            ProfScope ps_(ctx, "advect", pp.n * (5 * 24.0 + 8.0 + 2 * 24.0));
+            hipLaunchKernelGGL(k_advect, grid1(pp.n, 256), dim3(256), 0, st, pp, ctx->ps, stage);
+        }
     }
 
     static void step(fipi_ctx *ctx)
@@ -312,51 +390,90 @@ struct Impl {
         const bool ns = ctx->p.switch_ns != 0;
         const bool parts = ctx->n > 0;
         C *h0 = H(ctx->h[0]), *h1 = H(ctx->h[1]), *h2 = H(ctx->h[2]);
+        const double Rb = (double)ctx->G * sizeof(T), Hb = (double)ctx->GH * 2 * sizeof(T);
         if (ns) {
             ProdConv<T> pr{R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2])};
-            x_forward<1>(ctx, pr, h0, nullptr);
+            {
+                ProfScope ps_(ctx, "x_fwd_conv", 6 * Rb + Hb);
+                x_forward<1>(ctx, pr, h0, nullptr);
+            }
+            ProfScope ps_(ctx, "y_fwd_1", 2 * Hb);
             y_pass<-1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
         }
         const KParams kp = kparams(ctx);
         const int nxt = ctx->cur ^ 1;
-        with_n(d.nz, [&](auto NC) {
-            constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_hub_ch<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0, h1,
-                               (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]), (const C *)ctx->fk, H(ctx->hist), kp,
-                               d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
-        });
+        {
+            ProfScope ps_(ctx, "z_hub_ch", (ns ? 8 : 7) * Hb);
+            with_n(d.nz, [&](auto NC) {
+                constexpr int N = decltype(NC)::value;
+                hipLaunchKernelGGL((k_hub_ch<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0, h1,
+                                   (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]), (const C *)ctx->fk,
+                                   H(ctx->hist), kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
+            });
+        }
         ctx->cur = nxt;
-        y_pass<+1, 2, true>(ctx, h0, h1, nullptr, h2);
-        cgrad_f(ctx);
-        y_pass<-1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
-        with_n(d.nz, [&](auto NC) {
-            constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_hub_mu<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0,
-                               H(ctx->fk), (const C *)ctx->ck[ctx->cur], kp, d, ctx->kx, ctx->ky, ctx->kz,
-                               (const C *)ctx->twz);
-        });
+        {
+            ProfScope ps_(ctx, "y_inv_grad", 5 * Hb);
+            y_pass<+1, 2, true>(ctx, h0, h1, nullptr, h2);
+        }
+        {
+            ProfScope ps_(ctx, "x_inv_cgrad_f", 4 * Hb + 4 * Rb);
+            cgrad_f(ctx);
+        }
+        {
+            ProfScope ps_(ctx, "y_fwd_1", 2 * Hb);
+            y_pass<-1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
+        }
+        {
+            ProfScope ps_(ctx, "z_hub_mu", 4 * Hb);
+            with_n(d.nz, [&](auto NC) {
+                constexpr int N = decltype(NC)::value;
+                hipLaunchKernelGGL((k_hub_mu<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0,
+                                   H(ctx->fk), (const C *)ctx->ck[ctx->cur], kp, d, ctx->kx, ctx->ky, ctx->kz,
+                                   (const C *)ctx->twz);
+            });
+        }
         if (ns) {
-            y_pass<+1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
+            {
+                ProfScope ps_(ctx, "y_inv_1", 2 * Hb);
+                y_pass<+1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
+            }
             if (parts) {
                 const PartState &ps = ctx->ps;
                 spread_build(ctx, ctx->pp, ps.p[0], ps.p[1], ps.p[2], ps.fpp[0], ps.fpp[1], ps.fpp[2]);
             }
-            with_n(d.nx, [&](auto NC) {
-                constexpr int N = decltype(NC)::value;
-                hipLaunchKernelGGL((k_x_force<T, N>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
-                                   dim3(TILE * LineShape<N>::P), 0, st, h0, h1, h2, (const T *)R(ctx->c),
-                                   (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
-                                   (const int *)ctx->line_start, (const SpreadEntry<T> *)ctx->spr, parts ? 1 : 0,
-                                   (T)ctx->p.rho1, (T)ctx->p.rho2, (T)ctx->p.gravity, d, (const C *)ctx->twx);
-            });
-            y_pass<-1, 3, false>(ctx, h0, h1, h2, nullptr);
-            with_n(d.nz, [&](auto NC) {
-                constexpr int N = decltype(NC)::value;
-                hipLaunchKernelGGL((k_hub_stokes<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0, h1,
-                                   h2, kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
-            });
-            y_pass<+1, 3, false>(ctx, h0, h1, h2, nullptr);
-            x_inverse<3>(ctx, h0, h1, h2, R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]));
+            {
+                ProfScope ps_(ctx, "x_inv_force_fwd", 4 * Hb + 4 * Rb + (parts ? 8.0 * ctx->n * sizeof(SpreadEntry<T>) : 0.0));
+                with_n(d.nx, [&](auto NC) {
+                    constexpr int N = decltype(NC)::value;
+                    hipLaunchKernelGGL((k_x_force<T, N>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
+                                       dim3(TILE * LineShape<N>::P), 0, st, h0, h1, h2, (const T *)R(ctx->c),
+                                       (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
+                                       (const int *)ctx->line_start, (const SpreadEntry<T> *)ctx->spr,
+                                       parts ? 1 : 0, (T)ctx->p.rho1, (T)ctx->p.rho2, (T)ctx->p.gravity, d,
+                                       (const C *)ctx->twx);
+                });
+            }
+            {
+                ProfScope ps_(ctx, "y_fwd_3", 6 * Hb);
+                y_pass<-1, 3, false>(ctx, h0, h1, h2, nullptr);
+            }
+            {
+                ProfScope ps_(ctx, "z_hub_stokes", 6 * Hb);
+                with_n(d.nz, [&](auto NC) {
+                    constexpr int N = decltype(NC)::value;
+                    hipLaunchKernelGGL((k_hub_stokes<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0,
+                                       h1, h2, kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
+                });
+            }
+            {
+                ProfScope ps_(ctx, "y_inv_3", 6 * Hb);
+                y_pass<+1, 3, false>(ctx, h0, h1, h2, nullptr);
+            }
+            {
+                ProfScope ps_(ctx, "x_inv_u", 3 * Hb + 3 * Rb);
+                x_inverse<3>(ctx, h0, h1, h2, R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]));
+            }
         }
         if (parts) {
             particle_stage(ctx, 1);
@@ -718,6 +835,7 @@ int fipi_destroy(fipi_ctx *ctx)
     if (!ctx) return FIPI_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (void *p : ctx->allocs) (void)hipFree(p);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -917,6 +1035,38 @@ int fipi_step(fipi_ctx *ctx, int32_t nsteps)
     int rc = fipi_step_async(ctx, nsteps);
     if (rc) return rc;
     return check_flags(ctx);
+}
+
+int fipi_profile_steps(fipi_ctx *ctx, int32_t nsteps, fipi_kernel_stat *out, int32_t max_out, int32_t *count)
+{
+    if (!ctx || !count || max_out < 0 || (max_out > 0 && !out)) return set_err(ctx, FIPI_ERR_ARG, "invalid argument");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->prof.clear();
+    ctx->ev_used = 0;
+    ctx->prof_on = true;
+    int rc = fipi_step_async(ctx, nsteps);
+    ctx->prof_on = false;
+    if (rc) return rc;
+    if ((rc = check_flags(ctx))) return rc;
+    std::vector<fipi_kernel_stat> sites;
+    for (const auto &r : ctx->prof) {
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, r.a, r.b));
+        size_t k = 0;
+        while (k < sites.size() && std::strncmp(sites[k].name, r.name, sizeof(sites[k].name)) != 0) k++;
+        if (k == sites.size()) {
+            fipi_kernel_stat s{};
+            std::strncpy(s.name, r.name, sizeof(s.name) - 1);
+            sites.push_back(s);
+        }
+        sites[k].launches++;
+        sites[k].total_ms += ms;
+        sites[k].bytes += r.bytes;
+    }
+    *count = (int32_t)sites.size();
+    for (size_t k = 0; k < sites.size() && (int32_t)k < max_out; k++) out[k] = sites[k];
+    ctx->prof.clear();
+    return FIPI_OK;
 }
 
 int fipi_fft_r2c(fipi_ctx *ctx, const void *d_real, void *d_half)

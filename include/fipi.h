@@ -123,6 +123,19 @@ int fipi_step(fipi_ctx *ctx, int32_t nsteps);
 int fipi_step_async(fipi_ctx *ctx, int32_t nsteps);
 int fipi_sync(fipi_ctx *ctx);
 
+/* Per-kernel timing.  The reference's only instrumentation is clock()
+ * around the whole run (main.cpp:25-26,248-249). */
+typedef struct fipi_kernel_stat {
+    char name[40];      /* launch site */
+    int32_t launches;   /* launches timed */
+    double total_ms;    /* summed HIP-event time on the context stream */
+    double bytes;       /* algorithmic bytes (each array touched once), summed */
+} fipi_kernel_stat;
+/* Run nsteps steps (advancing the state like fipi_step) with HIP events
+ * around every launch; fills up to max_out launch-site records in first-
+ * launch order and sets *count to the number of sites. */
+int fipi_profile_steps(fipi_ctx *ctx, int32_t nsteps, fipi_kernel_stat *out, int32_t max_out, int32_t *count);
+
 /* ---- unit entry points (device pointers, context stream, synchronous) -- */
 /* 3-D real-to-complex forward transform, unnormalised (FFTW_FORWARD sign),
  * of one real field into the half spectrum [nz][ny][nx/2+1] complex
