@@ -77,7 +77,7 @@ def cpu_baseline(p, phi0, pos, cos_theta, steps):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=980)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--n", type=int, default=256)
@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--single", action="store_true", help="skip the other precision")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -97,59 +98,94 @@ def main():
     import fipi_amd
 
     prec = fipi_amd.FP64 if args.precision == "fp64" else fipi_amd.FP32_MIXED
+    other = fipi_amd.FP32_MIXED if prec == fipi_amd.FP64 else fipi_amd.FP64
     p, phi0, pos, cos_theta = workload(args.n, args.particles, prec)
-    sim = fipi_amd.Simulation(p)
-    sim.set_state(phi0, pos, None, cos_theta)
 
-    sim.step(args.warmup)
-    stream = torch.cuda.ExternalStream(sim.stream_ptr)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    sim.sync()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    sim.step_async(args.steps)
-    e1.record(stream)
-    sim.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ev_ms = e0.elapsed_time(e1)
+    def timed_run(precision):
+        """W untimed + K timed steps from the seeded state; returns the rate,
+        the final state and (for the headline precision) the kernel profile."""
+        q = p.replace(precision=precision)
+        sim = fipi_amd.Simulation(q)
+        sim.set_state(phi0, pos, None, cos_theta)
+        sim.step(args.warmup)
+        stream = torch.cuda.ExternalStream(sim.stream_ptr)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        sim.sync()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        sim.step_async(args.steps)
+        e1.record(stream)
+        sim.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        res = {
+            "wall": wall,
+            "ev_ms": e0.elapsed_time(e1),
+            "phi": sim.phi(),
+            "pos": sim.particles()["p"],
+            "diag": sim.diagnostics(),
+        }
+        if precision == prec and args.profile_steps > 0:
+            res["stats"] = sim.profile_steps(args.profile_steps)
+        sim.close()
+        del sim
+        torch.cuda.synchronize()
+        return res
+
+    main_run = timed_run(prec)
+    other_run = timed_run(other) if not args.single else None
+
+    wall = main_run["wall"]
     ms_per_step = wall * 1e3 / args.steps
     value = args.steps / wall
-
-    # per-kernel HIP-event profile on the context stream (separate pass)
-    stats = sim.profile_steps(args.profile_steps)
+    stats = main_run.get("stats", [])
     for s in stats:
         s["avg_ms"] = s["total_ms"] / max(s["launches"], 1)
         s["bytes_per_launch"] = s["bytes"] / max(s["launches"], 1)
         s["gbs"] = s["bytes_per_launch"] / (s["avg_ms"] * 1e-3) / 1e9 if s["avg_ms"] > 0 else 0.0
     stats.sort(key=lambda s: -s["total_ms"])
-    dom = stats[0]
-    traffic = None
-    tr_path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-    if os.path.exists(tr_path):
-        with open(tr_path) as f:
-            tr = json.load(f)
-        key = f"{args.precision}:{dom['name']}"
-        if key in tr:
-            traffic = tr[key]
-    roofline = {
-        "bound": "hbm",
-        "kernel": dom["name"],
-        "achieved": round(dom["gbs"], 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
-        "traffic": traffic,
-        "bytes_per_launch": dom["bytes_per_launch"],
-        "avg_launch_ms": round(dom["avg_ms"], 5),
-    }
+    roofline = None
+    if stats:
+        dom = stats[0]
+        traffic = None
+        tr_path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+        if os.path.exists(tr_path):
+            with open(tr_path) as f:
+                tr = json.load(f)
+            traffic = tr.get(f"{args.precision}:{dom['name']}")
+        roofline = {
+            "bound": "hbm",
+            "kernel": dom["name"],
+            "achieved": round(dom["gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch": dom["bytes_per_launch"],
+            "avg_launch_ms": round(dom["avg_ms"], 5),
+        }
     bstep = step_bytes(p, args.particles)
-    diag = sim.diagnostics()
-    sim.close()
-    del sim
-    torch.cuda.synchronize()
+    diag = main_run["diag"]
+
+    precision_cmp = None
+    if other_run is not None:
+        f64 = main_run if prec == fipi_amd.FP64 else other_run
+        mix = other_run if prec == fipi_amd.FP64 else main_run
+        a64, amx = f64["diag"]["interface_area"], mix["diag"]["interface_area"]
+        dp = mix["pos"] - f64["pos"]
+        dp -= np.round(dp / p.lx) * p.lx
+        precision_cmp = {
+            "fp64_steps_per_s": round(args.steps / f64["wall"], 3),
+            "fp32_mixed_steps_per_s": round(args.steps / mix["wall"], 3),
+            "mixed_speedup": round(f64["wall"] / mix["wall"], 3),
+            "after_steps": args.warmup + args.steps,
+            "phi_rel_l2": float(np.linalg.norm(mix["phi"] - f64["phi"]) / np.linalg.norm(f64["phi"])),
+            "interface_area_rel_err": float(abs(amx / a64 - 1.0)),
+            "max_particle_disp_err": float(np.abs(dp).max()),
+            "mean_phi_abs_err": float(abs(mix["diag"]["mean_phi"] - f64["diag"]["mean_phi"])),
+        }
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -176,7 +212,8 @@ def main():
             "particles": args.particles,
             "precision": args.precision,
         },
-        "event_ms_per_step": round(ev_ms / args.steps, 4),
+        "event_ms_per_step": round(main_run["ev_ms"] / args.steps, 4),
+        "precision_comparison": precision_cmp,
         "grid_point_steps_per_s": value * p.G,
         "step_roofline": {
             "bytes_per_step": bstep,

@@ -129,10 +129,44 @@ struct ProdCF {
     }
 };
 
+// Fixed-order FP64 sum of one value per thread over the block; the result
+// is valid in thread 0.  `red` holds blockDim.x doubles.
+__device__ __forceinline__ double block_sum_f64(double v, double *red)
+{
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+// Fixed-order FP64 sums: out[q] = sum_i part[i*stride + q], q < nq (one block).
+__global__ __launch_bounds__(256) void k_sum_strided(const double *part, int nb, int stride, int nq, double *out)
+{
+    __shared__ double s[256];
+    for (int q = 0; q < nq; q++) {
+        double a = 0;
+        for (int i = threadIdx.x; i < nb; i += 256) a += part[(size_t)i * stride + q];
+        s[threadIdx.x] = a;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[q] = s[0];
+        __syncthreads();
+    }
+}
+
+// part (optional): FP64 sum of field 0's real values per workgroup, for
+// the k = 0 mode of the FP32-mixed path.
 template <typename T, int N, int NF, class Prod>
 __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_x_forward(Prod prod, cplx<T> *__restrict__ out0,
                                                                       cplx<T> *__restrict__ out1, Dims d,
-                                                                      const cplx<T> *__restrict__ tw)
+                                                                      const cplx<T> *__restrict__ tw,
+                                                                      double *__restrict__ part)
 {
     using S = LineShape<N>;
     using C = cplx<T>;
@@ -143,12 +177,19 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_x_forward(Prod prod, 
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         C v[S::M];
+        double acc = 0.0;
         if (t < S::N1) {
 #pragma unroll
             for (int m = 0; m < S::N2; m++) {
                 const int x = t + S::N1 * m;
                 v[m] = C{prod(f, (size_t)rl * d.nx + x), prod(f, (size_t)(rl + 1) * d.nx + x)};
+                acc += (double)v[m].x + (double)v[m].y;
             }
+        }
+        if (f == 0 && part != nullptr) {
+            const double s = block_sum_f64(acc, reinterpret_cast<double *>(lds));
+            if (threadIdx.x == 0) part[blockIdx.x] = s;
+            __syncthreads();
         }
         fft_line<T, N, S::N1, -1>(v, L, t, tw);
         store_r2c<T, N, S::N1>(v, L, f == 0 ? out0 : out1, (size_t)rl * d.nxh, (size_t)(rl + 1) * d.nxh, t);
@@ -442,12 +483,21 @@ __device__ __forceinline__ void hub_store(const cplx<T> *v, cplx<T> *__restrict_
 //   c^1 = (c^0 - dtM k^2 f^0 - dt conv^0) / (1 + dtM lambda k^4)  (:106-113).
 // Outputs c^{n+1} (K), the new history (K), and c^{n+1}/G, i kz c^{n+1}/G
 // inverse-z transformed (XY) for the gradient and real-space c.
+// FP64 k = 0 mode (FP32-mixed path): k0[cur] = c_hat^n(0), k0[cur^1] gets
+// c_hat^{n+1}(0), k0[2] = history(0); conv0 = FP64 sum of conv (its k = 0
+// transform).  k0 == nullptr: the k = 0 mode is carried like every other.
+struct K0 {
+    double *k0;
+    const double *conv0;
+    int cur;
+};
+
 template <typename T, int N>
 __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_hub_ch(
     cplx<T> *__restrict__ conv_io /* in conv XY, out c XY */, cplx<T> *__restrict__ dzc_out,
     const cplx<T> *__restrict__ c_now, cplx<T> *__restrict__ c_next, const cplx<T> *__restrict__ f_now,
     cplx<T> *__restrict__ hist, KParams kp, Dims d, const double *__restrict__ kxv, const double *__restrict__ kyv,
-    const double *__restrict__ kzv, const cplx<T> *__restrict__ tw)
+    const double *__restrict__ kzv, const cplx<T> *__restrict__ tw, K0 k0)
 {
     using S = LineShape<N>;
     using C = cplx<T>;
@@ -503,10 +553,24 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_hub_ch(
             }
             const T m2 = (T)(k2 * (2.0 * kp.dt * kp.mobility));
             const T b2 = (T)(2.0 * kp.dt);
-            const C nh = C{-cn.x + m2 * fn.x + cv.x * b2, -cn.y + m2 * fn.y + cv.y * b2};
-            if (ok) { c_next[i] = nc; hist[i] = nh; }
+            C nh = C{-cn.x + m2 * fn.x + cv.x * b2, -cn.y + m2 * fn.y + cv.y * b2};
             const T ig = (T)kp.inv_g;
-            v[m] = C{nc.x * ig, nc.y * ig};
+            if (k0.k0 != nullptr && ok && kx == 0 && y == 0 && z == 0) {
+                // mass mode in FP64: k^2 = 0, so only c, conv and history enter
+                const double c0 = k0.k0[k0.cur], h0 = k0.k0[2];
+                const double cv0 = kp.use_conv ? *k0.conv0 : 0.0;
+                double n0;
+                if (kp.first) n0 = c0 - cv0 * kp.dt;
+                else n0 = (c0 * 4.0 - cv0 * (4.0 * kp.dt) + h0) / 3.0;
+                k0.k0[k0.cur ^ 1] = n0;
+                k0.k0[2] = -c0 + cv0 * (2.0 * kp.dt);
+                nc = C{(T)n0, T(0)};
+                nh = C{(T)k0.k0[2], T(0)};
+                v[m] = C{(T)(n0 * kp.inv_g), T(0)};
+            } else {
+                v[m] = C{nc.x * ig, nc.y * ig};
+            }
+            if (ok) { c_next[i] = nc; hist[i] = nh; }
             const T kz = z == N / 2 ? T(0) : (T)kzz;
             w[m] = C{-kz * v[m].y, kz * v[m].x};
         }

@@ -81,6 +81,8 @@ struct fipi_ctx {
     int *d_flags = nullptr;  // [0] error bits, [1] contact-order overflow
     int *h_flags = nullptr;  // pinned
     double *d_red = nullptr; // reduction scratch
+    // FP32-mixed: FP64 k = 0 mode [c(0) ring of 2, history(0)], conv(0)
+    double *k0 = nullptr, *conv_part = nullptr, *conv0 = nullptr;
     std::vector<void *> allocs;
     // per-launch timing (fipi_profile_steps)
     struct ProfRec {
@@ -209,14 +211,14 @@ struct Impl {
     static T fscale(const fipi_ctx *ctx) { return (T)(ctx->p.lambda / (ctx->p.epsilon * ctx->p.epsilon)); }
 
     template <int NF, class Prod>
-    static void x_forward(fipi_ctx *ctx, Prod prod, C *o0, C *o1)
+    static void x_forward(fipi_ctx *ctx, Prod prod, C *o0, C *o1, double *part = nullptr)
     {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
             hipLaunchKernelGGL((k_x_forward<T, N, NF, Prod>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
                                dim3(TILE * LineShape<N>::P), 0, ctx->stream, prod, o0, o1, d,
-                               (const C *)ctx->twx);
+                               (const C *)ctx->twx, part);
         });
     }
 
@@ -391,11 +393,18 @@ struct Impl {
         const bool parts = ctx->n > 0;
         C *h0 = H(ctx->h[0]), *h1 = H(ctx->h[1]), *h2 = H(ctx->h[2]);
         const double Rb = (double)ctx->G * sizeof(T), Hb = (double)ctx->GH * 2 * sizeof(T);
+        const bool mixed = !std::is_same<T, double>::value;
         if (ns) {
             ProdConv<T> pr{R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2])};
             {
                 ProfScope ps_(ctx, "x_fwd_conv", 6 * Rb + Hb);
-                x_forward<1>(ctx, pr, h0, nullptr);
+                x_forward<1>(ctx, pr, h0, nullptr, mixed ? ctx->conv_part : nullptr);
+            }
+            if (mixed) {
+                // FP64 conv(0) = sum of conv over the grid, fixed order
+                ProfScope ps_(ctx, "k0_sum", 8.0 * (d.ny * d.nz / (2 * TILE)));
+                hipLaunchKernelGGL(k_sum_strided, dim3(1), dim3(256), 0, st, (const double *)ctx->conv_part,
+                                   d.ny * d.nz / (2 * TILE), 1, 1, ctx->conv0);
             }
             ProfScope ps_(ctx, "y_fwd_1", 2 * Hb);
             y_pass<-1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
@@ -408,7 +417,8 @@ struct Impl {
                 constexpr int N = decltype(NC)::value;
                 hipLaunchKernelGGL((k_hub_ch<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0, h1,
                                    (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]), (const C *)ctx->fk,
-                                   H(ctx->hist), kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
+                                   H(ctx->hist), kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz,
+                                   K0{mixed ? ctx->k0 : nullptr, ctx->conv0, ctx->cur});
             });
         }
         ctx->cur = nxt;
@@ -569,23 +579,6 @@ __global__ __launch_bounds__(256) void k_diag_partial(const T *c, const T *g0, c
         for (int q = 0; q < 3; q++) part[3 * blockIdx.x + q] = s[q][0];
 }
 
-__global__ __launch_bounds__(256) void k_sum_strided(const double *part, int nb, int stride, int nq, double *out)
-{
-    __shared__ double s[256];
-    for (int q = 0; q < nq; q++) {
-        double a = 0;
-        for (int i = threadIdx.x; i < nb; i += 256) a += part[(size_t)i * stride + q];
-        s[threadIdx.x] = a;
-        __syncthreads();
-        for (int o = 128; o > 0; o >>= 1) {
-            if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[q] = s[0];
-        __syncthreads();
-    }
-}
-
 constexpr int DIAG_BLOCKS = 1024;
 
 int check_params(const fipi_params *p)
@@ -637,6 +630,9 @@ int setup_buffers(fipi_ctx *ctx)
     ALLOC(ctx->twz, sizeof(T) * 2 * p.nz);
     ALLOC(ctx->d_flags, sizeof(int) * 4);
     ALLOC(ctx->d_red, sizeof(double) * (3 * DIAG_BLOCKS + 16));
+    ALLOC(ctx->k0, sizeof(double) * 4);
+    ALLOC(ctx->conv0, sizeof(double) * 2);
+    ALLOC(ctx->conv_part, sizeof(double) * (p.ny * p.nz / (2 * TILE) + 1));
     auto kx = wave_numbers(p.nx, p.pi, p.lx), ky = wave_numbers(p.ny, p.pi, p.ly), kz = wave_numbers(p.nz, p.pi, p.lz);
     HIPCHK(ctx, hipMemcpy(ctx->kx, kx.data(), sizeof(double) * p.nx, hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->ky, ky.data(), sizeof(double) * p.ny, hipMemcpyHostToDevice));
@@ -861,6 +857,15 @@ int fipi_set_state(fipi_ctx *ctx, const double *phi, const double *pos, const do
     HIPCHK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4, ctx->stream));
     ctx->t = 0;
     ctx->cur = 0;
+    {
+        // FP64 k = 0 mode: c_hat(0) = sum of phi (used by the FP32-mixed path)
+        long double s = 0.0L;
+        for (size_t i = 0; i < ctx->G; i++) s += (long double)phi[i];
+        const double k0h[4] = {(double)s, 0.0, 0.0, 0.0};
+        HIPCHK(ctx, hipMemcpyAsync(ctx->k0, k0h, sizeof(k0h), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemsetAsync(ctx->conv0, 0, sizeof(double) * 2, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
     if (ctx->fp64) Impl<double>::init_fields(ctx);
     else Impl<float>::init_fields(ctx);
     // keep phi exactly as given (the transform round trip above rewrote it)
@@ -1005,6 +1010,12 @@ int fipi_diagnostics(fipi_ctx *ctx, fipi_diag *out)
     out->t = ctx->t;
     out->it_mark = mark;
     out->mean_phi = h[0] / (double)ctx->G;
+    if (!ctx->fp64) {
+        // FP32-mixed: the mass is carried in FP64 as the k = 0 mode
+        double k0h[3];
+        HIPCHK(ctx, hipMemcpy(k0h, ctx->k0, sizeof(k0h), hipMemcpyDeviceToHost));
+        out->mean_phi = k0h[ctx->cur] / (double)ctx->G;
+    }
     out->free_energy = h[1] * cell;
     out->interface_area = h[2] * cell;
     out->tau = h[3];
