@@ -77,7 +77,10 @@ struct fipi_ctx {
     int *cell_cnt = nullptr, *cell_start = nullptr, *cell_ids = nullptr, *cell_cursor = nullptr;
     int nlines = 0;
     int *line_cnt = nullptr, *line_start = nullptr, *line_cursor = nullptr;
-    void *spr = nullptr;
+    void *spr = nullptr, *spr_tmp = nullptr;
+    int *bsum = nullptr;       // scan block totals
+    double *ymax = nullptr;    // argmax-y block partials
+    int *yidx = nullptr;
     int *d_flags = nullptr;  // [0] error bits, [1] contact-order overflow
     int *h_flags = nullptr;  // pinned
     double *d_red = nullptr; // reduction scratch
@@ -184,6 +187,18 @@ struct ProfScope {
 
 // at least one block: every kernel bounds-checks its index
 inline dim3 grid1(size_t n, int b) { return dim3((unsigned)(n == 0 ? 1 : (n + b - 1) / b)); }
+
+// Exclusive scan of n counts into start[0..n] on the context stream;
+// bsum holds ceil(n/1024) block totals.  Optionally reduces argmax-y
+// partials into *mark.
+void scan(fipi_ctx *ctx, const int *cnt, int *start, int n, int *bsum, const double *ymax, const int *yidx,
+          int nparts, int *mark)
+{
+    const int nb = n > 0 ? (n + SCAN_ITEMS - 1) / SCAN_ITEMS : 1;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, ctx->stream, cnt, start, n, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, ctx->stream, bsum, nb, start, n, ymax, yidx, nparts, mark);
+    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, ctx->stream, start, n, (const int *)bsum);
+}
 
 // ---------------------------------------------------------------------------
 // Templated launch sequences
@@ -310,23 +325,22 @@ struct Impl {
         }
         {
             ProfScope ps_(ctx, "spread_scan", 8.0 * nl);
-            hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, ctx->line_cnt, ctx->line_start, nl,
-                               (const double *)nullptr, 0, (int *)nullptr);
+            scan(ctx, ctx->line_cnt, ctx->line_start, nl, ctx->bsum, nullptr, nullptr, 0, nullptr);
         }
         {
             ProfScope ps_(ctx, "spread_scatter", 48.0 * n + 8.0 * n * E);
             hipLaunchKernelGGL((k_spread_scatter<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, f0, f1,
-                               f2, ctx->line_start, ctx->line_cursor, (SpreadEntry<T> *)ctx->spr);
+                               f2, ctx->line_start, ctx->line_cursor, (SpreadEntry<T> *)ctx->spr_tmp);
         }
         {
-            ProfScope ps_(ctx, "spread_sort", 16.0 * n * E + 4.0 * nl);
-            hipLaunchKernelGGL((k_spread_sort<T>), grid1(nl, 256), dim3(256), 0, st, nl, ctx->line_start,
-                               (SpreadEntry<T> *)ctx->spr);
+            ProfScope ps_(ctx, "spread_rank", 16.0 * n * E + 4.0 * nl);
+            hipLaunchKernelGGL((k_spread_rank<T>), grid1(nl, 4), dim3(256), 0, st, nl, ctx->line_start,
+                               (const SpreadEntry<T> *)ctx->spr_tmp, (SpreadEntry<T> *)ctx->spr);
         }
     }
 
     static void cells_build(fipi_ctx *ctx, const PartParams &pp, const PartState &ps, int *ids, int *start,
-                            bool interp)
+                            bool interp, double *ymax, int *yidx)
     {
         hipStream_t st = ctx->stream;
         const double n = pp.n, nc = ctx->ncell;
@@ -339,15 +353,14 @@ struct Impl {
         if (interp) {
             ProfScope ps_(ctx, "interp_fpi_count", n * (24.0 + 32.0 * sizeof(T) + 16.0 + 56.0 + 4.0));
             hipLaunchKernelGGL((k_interp_fpi_count<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, ps, R(ctx->c),
-                               R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), ctx->cell_cnt);
+                               R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), ctx->cell_cnt, ymax, yidx);
         } else {
             hipLaunchKernelGGL(k_cell_count, grid1(pp.n, 256), dim3(256), 0, st, pp, ps.p[0], ps.p[1], ps.p[2],
-                               ps.cell, ctx->cell_cnt, ps.err);
+                               ps.cell, ctx->cell_cnt, ps.err, ymax, yidx);
         }
         {
-            ProfScope ps_(ctx, "cells_scan_mark", 8.0 * nc + 8.0 * n);
-            hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, ctx->cell_cnt, start, ctx->ncell,
-                               (const double *)ps.p[1], pp.n, ps.mark);
+            ProfScope ps_(ctx, "cells_scan_mark", 8.0 * nc + 12.0 * (pp.n / 256 + 1));
+            scan(ctx, ctx->cell_cnt, start, ctx->ncell, ctx->bsum, ymax, yidx, (int)grid1(pp.n, 256).x, ps.mark);
         }
         {
             ProfScope ps_(ctx, "cells_scatter", 12.0 * n);
@@ -372,7 +385,7 @@ struct Impl {
     {
         hipStream_t st = ctx->stream;
         const PartParams &pp = ctx->pp;
-        cells_build(ctx, pp, ctx->ps, ctx->cell_ids, ctx->cell_start, true);
+        cells_build(ctx, pp, ctx->ps, ctx->cell_ids, ctx->cell_start, true, ctx->ymax, ctx->yidx);
         const double rmp = ramp((double)(ctx->t + 1), ctx->p.ramp_t);
         {
             ProfScope ps_(ctx, "contact", pp.n * (24.0 + 4.0 + 24.0 + 8.0 + 24.0));
@@ -676,6 +689,13 @@ int setup_buffers(fipi_ctx *ctx)
     const size_t n = ctx->n > 0 ? ctx->n : 1;
     ALLOC(ctx->cell_ids, sizeof(int) * n);
     ALLOC(ctx->spr, sizeof(SpreadEntry<T>) * 8 * n);
+    ALLOC(ctx->spr_tmp, sizeof(SpreadEntry<T>) * 8 * n);
+    {
+        const size_t m = std::max<size_t>(std::max<size_t>(ctx->ncell, ctx->nlines), n);
+        ALLOC(ctx->bsum, sizeof(int) * (m / SCAN_ITEMS + 2));
+        ALLOC(ctx->ymax, sizeof(double) * (n / 256 + 2));
+        ALLOC(ctx->yidx, sizeof(int) * (n / 256 + 2));
+    }
     PartState &ps = ctx->ps;
     for (int a = 0; a < 3; a++) {
         ALLOC(ps.p[a], sizeof(double) * n);
@@ -785,8 +805,12 @@ int unit_cells(fipi_ctx *ctx, TmpParts &tp, int *start, int *ids)
 {
     PartParams pp = ctx->pp;
     pp.n = tp.n;
-    if (ctx->fp64) Impl<double>::cells_build(ctx, pp, tp.ps, ids, start, false);
-    else Impl<float>::cells_build(ctx, pp, tp.ps, ids, start, false);
+    double *ymax;
+    int *yidx, rc;
+    if ((rc = tp.alloc((void **)&ymax, sizeof(double) * (tp.n / 256 + 2)))) return rc;
+    if ((rc = tp.alloc((void **)&yidx, sizeof(int) * (tp.n / 256 + 2)))) return rc;
+    if (ctx->fp64) Impl<double>::cells_build(ctx, pp, tp.ps, ids, start, false, ymax, yidx);
+    else Impl<float>::cells_build(ctx, pp, tp.ps, ids, start, false, ymax, yidx);
     return FIPI_OK;
 }
 
@@ -1142,8 +1166,9 @@ int fipi_contact_pairs(fipi_ctx *ctx, const double *d_pos, int32_t n, int32_t *d
     if (n > 0) {
         hipLaunchKernelGGL(k_pairs, grid1(n, 128), dim3(128), 0, st, pp, tp.ps, ctx->cell_start, ids,
                            (const int *)nullptr, pcnt, (int *)nullptr, (int64_t)0);
-        hipLaunchKernelGGL(k_scan_excl, dim3(1), dim3(1024), 0, st, pcnt, pstart, n, (const double *)nullptr, 0,
-                           (int *)nullptr);
+        int *bs;
+        if ((rc = tp.alloc((void **)&bs, sizeof(int) * (n / SCAN_ITEMS + 2)))) return rc;
+        scan(ctx, pcnt, pstart, n, bs, nullptr, nullptr, 0, nullptr);
         if (max_pairs > 0) {
             hipLaunchKernelGGL(k_pairs, grid1(n, 128), dim3(128), 0, st, pp, tp.ps, ctx->cell_start, ids,
                                (const int *)pstart, pcnt, d_pairs, max_pairs);

@@ -56,60 +56,138 @@ __device__ __forceinline__ int wave_incl_scan(int v)
     return v;
 }
 
-// Exclusive scan of cnt[0..n) into start[0..n] (start[n] = total), one
-// workgroup of 1024 threads; each thread scans a contiguous chunk.
-// Optionally also finds it_mark = first index of the strictly largest
-// y > 0 (cluster.cpp:128-138), keeping the previous value if none.
-__global__ __launch_bounds__(1024) void k_scan_excl(const int *__restrict__ cnt, int *__restrict__ start, int n,
-                                                    const double *__restrict__ py, int np, int *__restrict__ mark)
+// ---------------------------------------------------------------------------
+// Multi-block exclusive scan: k_scan_blocks (each block of 256 threads scans
+// SCAN_ITEMS consecutive counts, 4 per thread, and writes its total),
+// k_scan_top (one block scans the block totals, writes start[n]),
+// k_scan_add (adds each block's offset).
+// ---------------------------------------------------------------------------
+constexpr int SCAN_ITEMS = 1024;
+
+__device__ __forceinline__ int block_excl_scan256(int v, int *wsum, int *total)
 {
-    __shared__ int wsum[16];
-    __shared__ double wy[16];
-    __shared__ int wi[16];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int chunk = (n + 1023) / 1024;
-    const int b0 = tid * chunk;
-    const int b1 = min(b0 + chunk, n);
-    int s = 0;
-    for (int i = b0; i < b1; i++) s += cnt[i];
-    const int inc = wave_incl_scan(s);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int inc = wave_incl_scan(v);
     if (lane == 63) wsum[wid] = inc;
     __syncthreads();
     int off = 0;
     for (int w = 0; w < wid; w++) off += wsum[w];
-    int run = off + inc - s;
-    for (int i = b0; i < b1; i++) {
-        start[i] = run;
-        run += cnt[i];
+    *total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_blocks(const int *__restrict__ cnt, int *__restrict__ start, int n,
+                                                     int *__restrict__ bsum)
+{
+    __shared__ int wsum[4];
+    const int b0 = blockIdx.x * SCAN_ITEMS + threadIdx.x * 4;
+    int v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = (b0 + q < n) ? cnt[b0 + q] : 0;
+    const int s = v[0] + v[1] + v[2] + v[3];
+    int tot;
+    int run = block_excl_scan256(s, wsum, &tot);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        if (b0 + q < n) start[b0 + q] = run;
+        run += v[q];
     }
-    if (tid == 1023) {
-        int tot = 0;
-        for (int w = 0; w < 16; w++) tot += wsum[w];
-        start[n] = tot;
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+// Scan of the nb block totals in place (chunks of 1024 with a carry);
+// start[n] = grand total.  Also reduces the argmax-y partials (y, index) of
+// nparts blocks into *mark: the strictly largest y > 0, first index on ties
+// (cluster.cpp:128-138); the previous value is kept if no y > 0.  The
+// (y descending, index ascending) order is total, so the tree is exact.
+__global__ __launch_bounds__(256) void k_scan_top(int *__restrict__ bsum, int nb, int *__restrict__ start, int n,
+                                                  const double *__restrict__ ymax, const int *__restrict__ yidx,
+                                                  int nparts, int *__restrict__ mark)
+{
+    __shared__ int wsum[4];
+    __shared__ double sy[256];
+    __shared__ int si[256];
+    int carry = 0;
+    for (int base = 0; base < nb; base += 1024) {
+        int v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = base + threadIdx.x * 4 + q;
+            v[q] = i < nb ? bsum[i] : 0;
+        }
+        const int s = v[0] + v[1] + v[2] + v[3];
+        int tot;
+        int run = carry + block_excl_scan256(s, wsum, &tot);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int i = base + threadIdx.x * 4 + q;
+            if (i < nb) bsum[i] = run;
+            run += v[q];
+        }
+        carry += tot;
     }
-    if (py == nullptr) return;
-    // argmax y (strictly greater, first index), y_max starts at 0.0
+    if (threadIdx.x == 0) start[n] = carry;
+    if (mark == nullptr) return;
     double by = 0.0;
     int bi = -1;
-    for (int i = tid; i < np; i += 1024) {
-        const double y = py[i];
-        if (y > by) { by = y; bi = i; }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double oy = __shfl_down(by, o, 64);
-        const int oi = __shfl_down(bi, o, 64);
+    for (int b = threadIdx.x; b < nparts; b += 256) {
+        const double oy = ymax[b];
+        const int oi = yidx[b];
         if (oi >= 0 && (bi < 0 || oy > by || (oy == by && oi < bi))) { by = oy; bi = oi; }
     }
-    if (lane == 0) { wy[wid] = by; wi[wid] = bi; }
+    sy[threadIdx.x] = by;
+    si[threadIdx.x] = bi;
     __syncthreads();
-    if (tid == 0) {
-        for (int w = 1; w < 16; w++) {
-            const double oy = wy[w];
-            const int oi = wi[w];
-            if (oi >= 0 && (bi < 0 || oy > by || (oy == by && oi < bi))) { by = oy; bi = oi; }
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            const double oy = sy[threadIdx.x + o];
+            const int oi = si[threadIdx.x + o];
+            if (oi >= 0 && (si[threadIdx.x] < 0 || oy > sy[threadIdx.x] ||
+                            (oy == sy[threadIdx.x] && oi < si[threadIdx.x]))) {
+                sy[threadIdx.x] = oy;
+                si[threadIdx.x] = oi;
+            }
         }
-        if (bi >= 0) *mark = bi;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && si[0] >= 0) *mark = si[0];
+}
+
+__global__ __launch_bounds__(256) void k_scan_add(int *__restrict__ start, int n, const int *__restrict__ bsum)
+{
+    const int i = blockIdx.x * SCAN_ITEMS + threadIdx.x * 4;
+    const int off = bsum[blockIdx.x];
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (i + q < n) start[i + q] += off;
+}
+
+// Block partial of argmax y (strictly greater, first index; y_max starts at
+// 0.0, cluster.cpp:128-138) over the particles of this block (256 threads).
+__device__ __forceinline__ void block_argmax_y(double y, int idx, double *ymax, int *yidx)
+{
+    __shared__ double sy[256];
+    __shared__ int si[256];
+    sy[threadIdx.x] = y > 0.0 ? y : 0.0;
+    si[threadIdx.x] = y > 0.0 ? idx : -1;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            const double oy = sy[threadIdx.x + o];
+            const int oi = si[threadIdx.x + o];
+            const int bi = si[threadIdx.x];
+            const double by = sy[threadIdx.x];
+            if (oi >= 0 && (bi < 0 || oy > by || (oy == by && oi < bi))) {
+                sy[threadIdx.x] = oy;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        ymax[blockIdx.x] = sy[0];
+        yidx[blockIdx.x] = si[0];
     }
 }
 
@@ -136,11 +214,14 @@ __device__ __forceinline__ int cell_of(const PartParams &P, double x, double y, 
 }
 
 // Histogram of contact cells; flags particles outside the sorting range.
-__global__ void k_cell_count(PartParams P, const double *__restrict__ px, const double *__restrict__ py,
-                             const double *__restrict__ pz, int *__restrict__ cell, int *__restrict__ cnt,
-                             int *__restrict__ err)
+__global__ __launch_bounds__(256) void k_cell_count(PartParams P, const double *__restrict__ px,
+                                                    const double *__restrict__ py, const double *__restrict__ pz,
+                                                    int *__restrict__ cell, int *__restrict__ cnt,
+                                                    int *__restrict__ err, double *__restrict__ ymax,
+                                                    int *__restrict__ yidx)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    block_argmax_y(i < P.n ? py[i] : -1.0, i, ymax, yidx);
     if (i >= P.n) return;
     int a, b, c;
     const int cc = cell_of(P, px[i], py[i], pz[i], &a, &b, &c);
@@ -241,91 +322,37 @@ __device__ __forceinline__ bool pair_force(const PartParams &P, const PartState 
 
 __device__ __forceinline__ int wrapc(int v, int n) { return (v + n) % n; }
 
-constexpr int MAX_LOWER = 48;  // contacts with lower id held for ordering
+constexpr int MAX_NB = 32;  // contact partners per side held for ordering
 
-// One thread per particle (taken in cell-sorted order for locality).
-// f_pp[a] is summed in the order the reference accumulates it: first the
-// contributions of partners b < a in ascending b (they were added when the
-// outer loop visited b), then partners b > a in 27-cell traversal order.
-// tau_i[a] = sum over b > a of 0.5 F.d when f_pi[a] and f_pi[mark] are
-// non-zero (cluster.cpp:256-259).
-__global__ void k_contact(PartParams P, PartState S, const int *__restrict__ start, const int *__restrict__ ids,
-                          double rmp, int *__restrict__ overflow)
+// Reference-order force sum by repeated scans, for particles with more than
+// MAX_NB contacts on one side (not expected for spheres; kept exact).
+__device__ void contact_slow(const PartParams &P, const PartState &S, const int *__restrict__ start,
+                             const int *__restrict__ ids, int a, int o, int pp, int q, double rmp, bool tau_on,
+                             double *fo, double *tau_out)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    // only the slots the sort filled (particles outside the range are left out)
-    if (s >= P.n || s >= start[P.ncx * P.ncy * P.ncz]) return;
-    const int a = ids[s];
-    int o, pp, q;
-    cell_of(P, S.p[0][a], S.p[1][a], S.p[2][a], &o, &pp, &q);
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    // lower partners
-    int low[MAX_LOWER];
-    int nl = 0;
-    bool ovf = false;
-    for (int k = q - 1; k <= q + 1; k++)
-        for (int j = pp - 1; j <= pp + 1; j++)
-            for (int i = o - 1; i <= o + 1; i++) {
-                const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
-                for (int e = start[co]; e < start[co + 1]; e++) {
-                    const int b = ids[e];
-                    if (b >= a) continue;
-                    double d[3];
-                    pair_disp(P, S, a, b, d);
-                    if (vnorm(d[0], d[1], d[2]) < P.r_eq) {
-                        if (nl < MAX_LOWER) low[nl++] = b;
-                        else ovf = true;
+    int last = -1;
+    for (;;) {
+        int best = 0x7fffffff;
+        for (int k = q - 1; k <= q + 1; k++)
+            for (int j = pp - 1; j <= pp + 1; j++)
+                for (int i = o - 1; i <= o + 1; i++) {
+                    const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                    for (int e = start[co]; e < start[co + 1]; e++) {
+                        const int b = ids[e];
+                        if (b >= a || b <= last || b >= best) continue;
+                        double d[3];
+                        pair_disp(P, S, a, b, d);
+                        if (vnorm(d[0], d[1], d[2]) < P.r_eq) best = b;
                     }
                 }
-            }
-    if (!ovf) {
-        for (int x = 1; x < nl; x++) {
-            const int v = low[x];
-            int y = x - 1;
-            while (y >= 0 && low[y] > v) { low[y + 1] = low[y]; y--; }
-            low[y + 1] = v;
-        }
-        for (int x = 0; x < nl; x++) {
-            double f[3], d[3];
-            pair_force(P, S, a, low[x], rmp, f, d);
-            fx = fx + f[0];
-            fy = fy + f[1];
-            fz = fz + f[2];
-        }
-    } else {
-        // too many lower partners to order: take them by repeated minimum
-        // search (same order, slower)
-        atomicOr(overflow, 1);
-        int last = -1;
-        for (;;) {
-            int best = 0x7fffffff;
-            for (int k = q - 1; k <= q + 1; k++)
-                for (int j = pp - 1; j <= pp + 1; j++)
-                    for (int i = o - 1; i <= o + 1; i++) {
-                        const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
-                        for (int e = start[co]; e < start[co + 1]; e++) {
-                            const int b = ids[e];
-                            if (b >= a || b <= last || b >= best) continue;
-                            double d[3];
-                            pair_disp(P, S, a, b, d);
-                            if (vnorm(d[0], d[1], d[2]) < P.r_eq) best = b;
-                        }
-                    }
-            if (best == 0x7fffffff) break;
-            double f[3], d[3];
-            pair_force(P, S, a, best, rmp, f, d);
-            fx = fx + f[0];
-            fy = fy + f[1];
-            fz = fz + f[2];
-            last = best;
-        }
-    }
-    // upper partners in traversal order
-    const int mk = *S.mark;
-    bool tau_on = false;
-    if (mk >= 0) {
-        tau_on = vnorm(S.fpi[0][a], S.fpi[1][a], S.fpi[2][a]) != 0.0 &&
-                 vnorm(S.fpi[0][mk], S.fpi[1][mk], S.fpi[2][mk]) != 0.0;
+        if (best == 0x7fffffff) break;
+        double f[3], d[3];
+        pair_force(P, S, a, best, rmp, f, d);
+        fx = fx + f[0];
+        fy = fy + f[1];
+        fz = fz + f[2];
+        last = best;
     }
     double tau = 0.0;
     for (int k = q - 1; k <= q + 1; k++)
@@ -344,9 +371,93 @@ __global__ void k_contact(PartParams P, PartState S, const int *__restrict__ sta
                     }
                 }
             }
-    S.fpp[0][a] = fx;
-    S.fpp[1][a] = fy;
-    S.fpp[2][a] = fz;
+    fo[0] = fx;
+    fo[1] = fy;
+    fo[2] = fz;
+    *tau_out = tau;
+}
+
+// One thread per particle (taken in cell-sorted order for locality).
+// f_pp[a] is summed in the order the reference accumulates it: first the
+// contributions of partners b < a in ascending b (they were added when the
+// outer loop visited b), then partners b > a in 27-cell traversal order.
+// One scan of the 27 cells collects both partner lists.
+// tau_i[a] = sum over b > a of 0.5 F.d when f_pi[a] and f_pi[mark] are
+// non-zero (cluster.cpp:256-259).
+__global__ __launch_bounds__(128) void k_contact(PartParams P, PartState S, const int *__restrict__ start,
+                                                 const int *__restrict__ ids, double rmp,
+                                                 int *__restrict__ overflow)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    // only the slots the sort filled (particles outside the range are left out)
+    if (s >= P.n || s >= start[P.ncx * P.ncy * P.ncz]) return;
+    const int a = ids[s];
+    int o, pp, q;
+    cell_of(P, S.p[0][a], S.p[1][a], S.p[2][a], &o, &pp, &q);
+    const int mk = *S.mark;
+    bool tau_on = false;
+    if (mk >= 0) {
+        tau_on = vnorm(S.fpi[0][a], S.fpi[1][a], S.fpi[2][a]) != 0.0 &&
+                 vnorm(S.fpi[0][mk], S.fpi[1][mk], S.fpi[2][mk]) != 0.0;
+    }
+    int low[MAX_NB], up[MAX_NB];
+    int nl = 0, nu = 0;
+    bool ovf = false;
+    for (int k = q - 1; k <= q + 1; k++)
+        for (int j = pp - 1; j <= pp + 1; j++)
+            for (int i = o - 1; i <= o + 1; i++) {
+                const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                const int e1 = start[co + 1];
+                for (int e = start[co]; e < e1; e++) {
+                    const int b = ids[e];
+                    if (b == a) continue;
+                    double d[3];
+                    pair_disp(P, S, a, b, d);
+                    if (vnorm(d[0], d[1], d[2]) < P.r_eq) {
+                        if (b < a) {
+                            if (nl < MAX_NB) low[nl++] = b;
+                            else ovf = true;
+                        } else {
+                            if (nu < MAX_NB) up[nu++] = b;
+                            else ovf = true;
+                        }
+                    }
+                }
+            }
+    double fo[3], tau = 0.0;
+    if (ovf) {
+        atomicOr(overflow, 1);
+        contact_slow(P, S, start, ids, a, o, pp, q, rmp, tau_on, fo, &tau);
+    } else {
+        for (int x = 1; x < nl; x++) {
+            const int v = low[x];
+            int y = x - 1;
+            while (y >= 0 && low[y] > v) { low[y + 1] = low[y]; y--; }
+            low[y + 1] = v;
+        }
+        double fx = 0.0, fy = 0.0, fz = 0.0;
+        for (int x = 0; x < nl; x++) {
+            double f[3], d[3];
+            pair_force(P, S, a, low[x], rmp, f, d);
+            fx = fx + f[0];
+            fy = fy + f[1];
+            fz = fz + f[2];
+        }
+        for (int x = 0; x < nu; x++) {
+            double f[3], d[3];
+            pair_force(P, S, a, up[x], rmp, f, d);
+            fx = fx + f[0];
+            fy = fy + f[1];
+            fz = fz + f[2];
+            if (tau_on) tau += 0.5 * ((f[0] * d[0] + f[1] * d[1]) + f[2] * d[2]);
+        }
+        fo[0] = fx;
+        fo[1] = fy;
+        fo[2] = fz;
+    }
+    S.fpp[0][a] = fo[0];
+    S.fpp[1][a] = fo[1];
+    S.fpp[2][a] = fo[2];
     S.tau_i[a] = tau;
 }
 
@@ -437,10 +548,13 @@ __device__ __forceinline__ bool grid_cell_ok(const PartParams &P, double px, dou
 // element.cpp:38-52), then f_pi (element.cpp:75-97), and the contact cell
 // histogram of the following sort (cluster.cpp:116-169).
 template <typename T>
-__global__ void k_interp_fpi_count(PartParams P, PartState S, const T *__restrict__ c, const T *__restrict__ g0,
-                                   const T *__restrict__ g1, const T *__restrict__ g2, int *__restrict__ cnt)
+__global__ __launch_bounds__(256) void k_interp_fpi_count(PartParams P, PartState S, const T *__restrict__ c,
+                                                          const T *__restrict__ g0, const T *__restrict__ g1,
+                                                          const T *__restrict__ g2, int *__restrict__ cnt,
+                                                          double *__restrict__ ymax, int *__restrict__ yidx)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    block_argmax_y(a < P.n ? S.p[1][a] : -1.0, a, ymax, yidx);
     if (a >= P.n) return;
     const double px = S.p[0][a], py = S.p[1][a], pz = S.p[2][a];
     if (!grid_cell_ok(P, px, py, pz)) {
@@ -586,6 +700,30 @@ __global__ void k_spread_scatter(PartParams P, const double *__restrict__ px, co
         const int line = cy[c] + P.ny * cz[c];
         const int slot = atomicAdd(&cursor[line], 1);
         out[line_start[line] + slot] = e;
+    }
+}
+
+// Sort each line's entries by (x, particle id) into `out`: one wave per
+// line, each lane ranks its entries by counting smaller keys.  Within a line
+// the keys are unique (a particle meets a line at two different x), so the
+// ranks are a permutation and the result is deterministic.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spread_rank(int nlines, const int *__restrict__ line_start,
+                                                     const SpreadEntry<T> *__restrict__ in,
+                                                     SpreadEntry<T> *__restrict__ out)
+{
+    const int line = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (line >= nlines) return;
+    const int b = line_start[line], e = line_start[line + 1];
+    for (int i = b + lane; i < e; i += 64) {
+        const SpreadEntry<T> v = in[i];
+        int r = 0;
+        for (int j = b; j < e; j++) {
+            const int xj = in[j].x, pj = in[j].pid;
+            r += (xj < v.x || (xj == v.x && pj < v.pid)) ? 1 : 0;
+        }
+        out[b + r] = v;
     }
 }
 
