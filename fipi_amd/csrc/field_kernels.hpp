@@ -532,28 +532,32 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_hub_ch(
             if (ok) { cn = c_now[i]; fn = f_now[i]; h = hist[i]; }
             const C cv = v[m];
             C nc;
+            // The update is evaluated in FP64 from the stored values (FP32 in
+            // the mixed path): coefficients rounded to FP32 would bias every
+            // mode's growth factor by ~1e-7 per step, and the bias compounds
+            // over the run; rounding only the result is unbiased.
+            const double cnx = cn.x, cny = cn.y, fnx = fn.x, fny = fn.y, cvx = cv.x, cvy = cv.y;
             if (kp.first) {
-                const T m1 = (T)(k2 * (kp.dt * kp.mobility));
-                const T div = (T)(k2 * k2 * (kp.dt * kp.mobility * kp.lambda) + 1.0);
-                const T dt = (T)kp.dt;
-                T r = cn.x - m1 * fn.x, im = cn.y - m1 * fn.y;
-                r -= cv.x * dt;
-                im -= cv.y * dt;
-                nc = C{r / div, im / div};
+                const double m1 = k2 * (kp.dt * kp.mobility);
+                const double div = k2 * k2 * (kp.dt * kp.mobility * kp.lambda) + 1.0;
+                double r = cnx - m1 * fnx, im = cny - m1 * fny;
+                r -= cvx * kp.dt;
+                im -= cvy * kp.dt;
+                nc = C{(T)(r / div), (T)(im / div)};
             } else {
-                const T m4 = (T)(k2 * (4.0 * kp.dt * kp.mobility));
-                const T b4 = (T)(4.0 * kp.dt);
-                const T div = (T)(k2 * k2 * (2.0 * kp.dt * kp.mobility * kp.lambda) + 3.0);
-                T r = cn.x * T(4) - m4 * fn.x, im = cn.y * T(4) - m4 * fn.y;
-                r -= cv.x * b4;
-                im -= cv.y * b4;
-                r += h.x;
-                im += h.y;
-                nc = C{r / div, im / div};
+                const double m4 = k2 * (4.0 * kp.dt * kp.mobility);
+                const double b4 = 4.0 * kp.dt;
+                const double div = k2 * k2 * (2.0 * kp.dt * kp.mobility * kp.lambda) + 3.0;
+                double r = cnx * 4.0 - m4 * fnx, im = cny * 4.0 - m4 * fny;
+                r -= cvx * b4;
+                im -= cvy * b4;
+                r += (double)h.x;
+                im += (double)h.y;
+                nc = C{(T)(r / div), (T)(im / div)};
             }
-            const T m2 = (T)(k2 * (2.0 * kp.dt * kp.mobility));
-            const T b2 = (T)(2.0 * kp.dt);
-            C nh = C{-cn.x + m2 * fn.x + cv.x * b2, -cn.y + m2 * fn.y + cv.y * b2};
+            const double m2 = k2 * (2.0 * kp.dt * kp.mobility);
+            const double b2 = 2.0 * kp.dt;
+            C nh = C{(T)(-cnx + m2 * fnx + cvx * b2), (T)(-cny + m2 * fny + cvy * b2)};
             const T ig = (T)kp.inv_g;
             if (k0.k0 != nullptr && ok && kx == 0 && y == 0 && z == 0) {
                 // mass mode in FP64: k^2 = 0, so only c, conv and history enter
@@ -651,7 +655,6 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_hub_mu(cplx<T> *__res
     fft_line<T, N, S::N1, -1>(v, L, t, tw);
     if (t < S::N2) {
         const double kxx = kxv[ok ? kx : 0], kyy = kyv[y];
-        const T ig = (T)kp.inv_g;
 #pragma unroll
         for (int m = 0; m < S::N1; m++) {
             const int z = t + S::N2 * m;
@@ -664,8 +667,10 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_hub_mu(cplx<T> *__res
             const C fh = v[m];
             C cc = C{0, 0};
             if (ok) { f_k[i] = fh; cc = c_k[i]; }
-            const T ml = (T)(k2 * kp.lambda);
-            v[m] = C{(ml * cc.x + fh.x) * ig, (ml * cc.y + fh.y) * ig};
+            // FP64 arithmetic on the stored values, one rounding at the end
+            const double ml = k2 * kp.lambda;
+            v[m] = C{(T)((ml * (double)cc.x + (double)fh.x) * kp.inv_g),
+                     (T)((ml * (double)cc.y + (double)fh.y) * kp.inv_g)};
         }
     }
     fft_line<T, N, S::N2, +1>(v, L, t, tw);
@@ -717,28 +722,26 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_hub_stokes(cplx<T> *_
             k2 += kxx * kxx;
             k2 += kyy * kyy;
             k2 += kzz * kzz;
-            const T kv[3] = {(T)kxx, (T)kyy, (T)kzz};
+            const double kv[3] = {kxx, kyy, kzz};
             const bool nq[3] = {nx_, ny_, nz_};
-            C F[3] = {v0[m], v1[m], v2[m]};
-            // k.F split by Nyquist class
-            C sa = C{0, 0}, sb = C{0, 0};
+            const double F[3][2] = {{v0[m].x, v0[m].y}, {v1[m].x, v1[m].y}, {v2[m].x, v2[m].y}};
+            // k.F split by Nyquist class (FP64 arithmetic on the stored values)
+            double sar = 0, sai = 0, sbr = 0, sbi = 0;
 #pragma unroll
             for (int e = 0; e < 3; e++) {
-                if (nq[e]) { sb.x += kv[e] * F[e].x; sb.y += kv[e] * F[e].y; }
-                else { sa.x += kv[e] * F[e].x; sa.y += kv[e] * F[e].y; }
+                if (nq[e]) { sbr += kv[e] * F[e][0]; sbi += kv[e] * F[e][1]; }
+                else { sar += kv[e] * F[e][0]; sai += kv[e] * F[e][1]; }
             }
-            const T kk = (T)k2;
-            const T vis = (T)(k2 * kp.mu);
-            const T ig = (T)kp.inv_g;
+            const double vis = k2 * kp.mu;
             C U[3];
 #pragma unroll
             for (int e = 0; e < 3; e++) {
-                const C s = nq[e] ? sb : sa;
-                T pr = kv[e] * s.x, pi_ = kv[e] * s.y;
-                if (kk != T(0)) { pr /= kk; pi_ /= kk; } else { pr = T(0); pi_ = T(0); }
-                T ur = F[e].x - pr, ui = F[e].y - pi_;
-                if (vis != T(0)) { ur /= vis; ui /= vis; } else { ur = T(0); ui = T(0); }
-                U[e] = C{ur * ig, ui * ig};
+                const double sr = nq[e] ? sbr : sar, si = nq[e] ? sbi : sai;
+                double pr = kv[e] * sr, pi_ = kv[e] * si;
+                if (k2 != 0.0) { pr /= k2; pi_ /= k2; } else { pr = 0.0; pi_ = 0.0; }
+                double ur = F[e][0] - pr, ui = F[e][1] - pi_;
+                if (vis != 0.0) { ur /= vis; ui /= vis; } else { ur = 0.0; ui = 0.0; }
+                U[e] = C{(T)(ur * kp.inv_g), (T)(ui * kp.inv_g)};
             }
             v0[m] = U[0];
             v1[m] = U[1];
