@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--single", action="store_true", help="skip the other precision")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="run one device-to-device copy of this size first (a kernel of known "
+                         "traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -96,6 +99,13 @@ def main():
 
     import torch
     import fipi_amd
+
+    if args.calibration_copy_mib > 0:
+        src = torch.ones(args.calibration_copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     prec = fipi_amd.FP64 if args.precision == "fp64" else fipi_amd.FP32_MIXED
     other = fipi_amd.FP32_MIXED if prec == fipi_amd.FP64 else fipi_amd.FP64
