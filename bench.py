@@ -164,7 +164,8 @@ def main():
         if os.path.exists(tr_path):
             with open(tr_path) as f:
                 tr = json.load(f)
-            traffic = tr.get(f"{args.precision}:{dom['name']}")
+            rec = tr.get(f"{args.precision}:{dom['name']}")
+            traffic = rec["bytes"] if rec else None
         roofline = {
             "bound": "hbm",
             "kernel": dom["name"],

@@ -22,15 +22,18 @@
 
 namespace fipi {
 
+// nxh = nx/2 + 1 spectral columns; nxp >= nxh is the row pitch of every
+// half-spectrum array, a multiple of 128 bytes so that the 16-column tiles
+// of the y and z passes start on cache-line boundaries.
 struct Dims {
-    int nx, ny, nz, nxh;
+    int nx, ny, nz, nxh, nxp;
 };
 
 constexpr int TILE = 16;  // kx columns per y/z workgroup; line pairs per x workgroup
 
 __device__ __forceinline__ size_t hidx(const Dims &d, int kx, int y, int z)
 {
-    return (size_t)kx + (size_t)d.nxh * ((size_t)y + (size_t)d.ny * z);
+    return (size_t)kx + (size_t)d.nxp * ((size_t)y + (size_t)d.ny * z);
 }
 
 // ---------------------------------------------------------------------------
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_x_forward(Prod prod, 
             __syncthreads();
         }
         fft_line<T, N, S::N1, -1>(v, L, t, tw);
-        store_r2c<T, N, S::N1>(v, L, f == 0 ? out0 : out1, (size_t)rl * d.nxh, (size_t)(rl + 1) * d.nxh, t);
+        store_r2c<T, N, S::N1>(v, L, f == 0 ? out0 : out1, (size_t)rl * d.nxp, (size_t)(rl + 1) * d.nxp, t);
     }
 }
 
@@ -218,7 +221,7 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_x_inverse(const cplx<
         const C *in = f == 0 ? in0 : (f == 1 ? in1 : in2);
         T *o = f == 0 ? o0 : (f == 1 ? o1 : o2);
         C v[S::M];
-        load_c2r<T, N, false>(v, in, (size_t)rl * d.nxh, (size_t)(rl + 1) * d.nxh, t, nullptr);
+        load_c2r<T, N, false>(v, in, (size_t)rl * d.nxp, (size_t)(rl + 1) * d.nxp, t, nullptr);
         fft_line<T, N, S::N1, +1>(v, L, t, tw);
         if (t < S::N2) {
 #pragma unroll
@@ -246,7 +249,7 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_x_cgrad_f(
     __shared__ C lds[TILE * S::LDS];
     const int lp = threadIdx.x / S::P, t = threadIdx.x % S::P;
     const int rl = (blockIdx.x * TILE + lp) * 2;
-    const size_t rA = (size_t)rl * d.nxh, rB = rA + d.nxh;
+    const size_t rA = (size_t)rl * d.nxp, rB = rA + d.nxp;
     const size_t xA = (size_t)rl * d.nx, xB = xA + d.nx;
     C *L = lds + lp * S::LDS;
     C cv[S::M];
@@ -305,40 +308,42 @@ struct SpreadEntry {
     T v[3];
 };
 
+// Spread sum of one grid node: x on its line and the three force components.
+template <typename T>
+struct SpreadNode {
+    int x;
+    T v[3];
+};
+
+// P9 (see the pass list above).  nodes[line_start[l] + k], k < node_cnt[l],
+// are the per-node spread sums of real line l (unique x per line), each
+// already summed in ascending particle order; every thread adds whole
+// nodes, so there is one add per node and no ordering between threads.
+// Buoyancy enters the z component only: g = (0, 0, gravity)
+// (field.cpp:122-133).
 template <typename T, int N>
 __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_x_force(
     cplx<T> *__restrict__ mu_io /* in: mu_hat (X); out: F_0 (X) */, cplx<T> *__restrict__ f1,
     cplx<T> *__restrict__ f2, const T *__restrict__ c, const T *__restrict__ gx, const T *__restrict__ gy,
-    const T *__restrict__ gz, const int *__restrict__ spr_start, const SpreadEntry<T> *__restrict__ spr,
-    int use_spread, T rho1, T rho2, T gravity, Dims d, const cplx<T> *__restrict__ tw)
+    const T *__restrict__ gz, const int *__restrict__ line_start, const int *__restrict__ node_cnt,
+    const SpreadNode<T> *__restrict__ nodes, int use_spread, T rho1, T rho2, T gravity, Dims d,
+    const cplx<T> *__restrict__ tw)
 {
     using S = LineShape<N>;
     using C = cplx<T>;
     __shared__ C lds[TILE * S::LDS];
     const int lp = threadIdx.x / S::P, t = threadIdx.x % S::P;
     const int rl = (blockIdx.x * TILE + lp) * 2;
-    const size_t rA = (size_t)rl * d.nxh, rB = rA + d.nxh;
+    const size_t rA = (size_t)rl * d.nxp, rB = rA + d.nxp;
     const size_t xA = (size_t)rl * d.nx, xB = xA + d.nx;
     C *L = lds + lp * S::LDS;
     C muv[S::M];
     load_c2r<T, N, false>(muv, mu_io, rA, rB, t, nullptr);
     fft_line<T, N, S::N1, +1>(muv, L, t, tw);
     // thread t < N2 holds mu at x = t + N2*m for lines rl (x part) and rl+1 (y part)
-    C bv[S::M];
-    if (t < S::N2) {
-#pragma unroll
-        for (int m = 0; m < S::N1; m++) {
-            const int x = t + S::N2 * m;
-            const T ca = c[xA + x], cb = c[xB + x];
-            // ((1+c)/2*rho1 + (1-c)/2*rho2), src/main.cpp:172
-            bv[m] = C{(ca + T(1)) / T(2) * rho1 + (ca * T(-1) + T(1)) / T(2) * rho2,
-                      (cb + T(1)) / T(2) * rho1 + (cb * T(-1) + T(1)) / T(2) * rho2};
-        }
-    }
 #pragma unroll
     for (int f = 0; f < 3; f++) {
         const T *g = f == 0 ? gx : (f == 1 ? gy : gz);
-        const T gdir = f == 2 ? gravity : T(0);
         C v[S::M];
         if (t < S::N2) {
 #pragma unroll
@@ -348,9 +353,7 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_x_force(
             }
         }
         if (use_spread) {
-            // natural-order line buffer, one thread per real line adds the
-            // per-node spread sums sequentially (deterministic order)
-            T *Lr = reinterpret_cast<T *>(L);  // 2N reals: line rl then rl+1
+            T *Lr = reinterpret_cast<T *>(L);  // natural-order buffer: line rl then rl+1
             if (t < S::N2) {
 #pragma unroll
                 for (int m = 0; m < S::N1; m++) {
@@ -360,15 +363,12 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_x_force(
                 }
             }
             __syncthreads();
-            if (t < 2) {
-                const int line = rl + t;
-                const int e0 = spr_start[line], e1 = spr_start[line + 1];
-                int e = e0;
-                while (e < e1) {
-                    const int x = spr[e].x;
-                    T s = T(0);
-                    for (; e < e1 && spr[e].x == x; e++) s += spr[e].v[f];
-                    Lr[t * N + x] += s;
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int b = line_start[rl + q], cnt = node_cnt[rl + q];
+                for (int k = t; k < cnt; k += S::P) {
+                    const SpreadNode<T> nd = nodes[b + k];
+                    Lr[q * N + nd.x] += nd.v[f];
                 }
             }
             __syncthreads();
@@ -381,9 +381,16 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_x_force(
             }
             __syncthreads();
         }
-        if (t < S::N2) {
+        if (f == 2 && t < S::N2) {
+            // + ((1+c)/2*rho1 + (1-c)/2*rho2) * gravity, src/main.cpp:172
 #pragma unroll
-            for (int m = 0; m < S::N1; m++) v[m] = C{v[m].x + bv[m].x * gdir, v[m].y + bv[m].y * gdir};
+            for (int m = 0; m < S::N1; m++) {
+                const int x = t + S::N2 * m;
+                const T ca = c[xA + x], cb = c[xB + x];
+                const T ba = (ca + T(1)) / T(2) * rho1 + (ca * T(-1) + T(1)) / T(2) * rho2;
+                const T bb = (cb + T(1)) / T(2) * rho1 + (cb * T(-1) + T(1)) / T(2) * rho2;
+                v[m] = C{v[m].x + ba * gravity, v[m].y + bb * gravity};
+            }
         }
         fft_line<T, N, S::N2, -1>(v, L, t, tw);
         store_r2c<T, N, S::N2>(v, L, f == 0 ? mu_io : (f == 1 ? f1 : f2), rA, rB, t);
@@ -408,7 +415,7 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_y(cplx<T> *__restrict
     const bool ok = kx < d.nxh;
     C *L = lds + lane * S::LDS;
     const size_t base = hidx(d, ok ? kx : 0, 0, z);
-    const size_t st = d.nxh;
+    const size_t st = d.nxp;
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         C *a = f == 0 ? a0 : (f == 1 ? a1 : a2);
@@ -507,7 +514,7 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_hub_ch(
     const bool ok = kx < d.nxh;
     C *L = lds + lane * S::LDS;
     const size_t base = hidx(d, ok ? kx : 0, y, 0);
-    const size_t st = (size_t)d.nxh * d.ny;
+    const size_t st = (size_t)d.nxp * d.ny;
     C v[S::M];
     if (kp.use_conv) {
         hub_load<T, N>(v, conv_io, base, st, t, ok);
@@ -602,7 +609,7 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_hub_init(cplx<T> *__r
     const bool ok = kx < d.nxh;
     C *L = lds + lane * S::LDS;
     const size_t base = hidx(d, ok ? kx : 0, y, 0);
-    const size_t st = (size_t)d.nxh * d.ny;
+    const size_t st = (size_t)d.nxp * d.ny;
     C fv[S::M];
     hub_load<T, N>(fv, f_io, base, st, t, ok);
     fft_line<T, N, S::N1, -1>(fv, L, t, tw);
@@ -649,7 +656,7 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_hub_mu(cplx<T> *__res
     const bool ok = kx < d.nxh;
     C *L = lds + lane * S::LDS;
     const size_t base = hidx(d, ok ? kx : 0, y, 0);
-    const size_t st = (size_t)d.nxh * d.ny;
+    const size_t st = (size_t)d.nxp * d.ny;
     C v[S::M];
     hub_load<T, N>(v, f_io, base, st, t, ok);
     fft_line<T, N, S::N1, -1>(v, L, t, tw);
@@ -701,13 +708,14 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_hub_stokes(cplx<T> *_
     const bool ok = kx < d.nxh;
     C *L = lds + lane * S::LDS;
     const size_t base = hidx(d, ok ? kx : 0, y, 0);
-    const size_t st = (size_t)d.nxh * d.ny;
+    const size_t st = (size_t)d.nxp * d.ny;
     C v0[S::M], v1[S::M], v2[S::M];
+    // all three loads in flight before the first transform
     hub_load<T, N>(v0, a0, base, st, t, ok);
-    fft_line<T, N, S::N1, -1>(v0, L, t, tw);
     hub_load<T, N>(v1, a1, base, st, t, ok);
-    fft_line<T, N, S::N1, -1>(v1, L, t, tw);
     hub_load<T, N>(v2, a2, base, st, t, ok);
+    fft_line<T, N, S::N1, -1>(v0, L, t, tw);
+    fft_line<T, N, S::N1, -1>(v1, L, t, tw);
     fft_line<T, N, S::N1, -1>(v2, L, t, tw);
     if (t < S::N2) {
         const int kxi = ok ? kx : 0;
@@ -769,7 +777,7 @@ __global__ __launch_bounds__(TILE *LineShape<N>::P) void k_z(cplx<T> *__restrict
     const bool ok = kx < d.nxh;
     C *L = lds + lane * S::LDS;
     const size_t base = hidx(d, ok ? kx : 0, y, 0);
-    const size_t st = (size_t)d.nxh * d.ny;
+    const size_t st = (size_t)d.nxp * d.ny;
     C v[S::M];
     hub_load<T, N>(v, a, base, st, t, ok);
     fft_line<T, N, S::N1, SIGN>(v, L, t, tw);

@@ -76,7 +76,7 @@ struct fipi_ctx {
     int ncell = 0;
     int *cell_cnt = nullptr, *cell_start = nullptr, *cell_ids = nullptr, *cell_cursor = nullptr;
     int nlines = 0;
-    int *line_cnt = nullptr, *line_start = nullptr, *line_cursor = nullptr;
+    int *line_cnt = nullptr, *line_start = nullptr, *line_cursor = nullptr, *node_cnt = nullptr;
     void *spr = nullptr, *spr_tmp = nullptr;
     int *bsum = nullptr;       // scan block totals
     double *ymax = nullptr;    // argmax-y block partials
@@ -337,6 +337,12 @@ struct Impl {
             hipLaunchKernelGGL((k_spread_rank<T>), grid1(nl, 4), dim3(256), 0, st, nl, ctx->line_start,
                                (const SpreadEntry<T> *)ctx->spr_tmp, (SpreadEntry<T> *)ctx->spr);
         }
+        {
+            // per-node sums into the (now free) scatter buffer
+            ProfScope ps_(ctx, "spread_nodes", 8.0 * n * E + 8.0 * n * sizeof(SpreadNode<T>) + 8.0 * nl);
+            hipLaunchKernelGGL((k_spread_nodes<T>), grid1(nl, 4), dim3(256), 0, st, nl, ctx->line_start,
+                               (const SpreadEntry<T> *)ctx->spr, (SpreadNode<T> *)ctx->spr_tmp, ctx->node_cnt);
+        }
     }
 
     static void cells_build(fipi_ctx *ctx, const PartParams &pp, const PartState &ps, int *ids, int *start,
@@ -472,9 +478,9 @@ struct Impl {
                     hipLaunchKernelGGL((k_x_force<T, N>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
                                        dim3(TILE * LineShape<N>::P), 0, st, h0, h1, h2, (const T *)R(ctx->c),
                                        (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
-                                       (const int *)ctx->line_start, (const SpreadEntry<T> *)ctx->spr,
-                                       parts ? 1 : 0, (T)ctx->p.rho1, (T)ctx->p.rho2, (T)ctx->p.gravity, d,
-                                       (const C *)ctx->twx);
+                                       (const int *)ctx->line_start, (const int *)ctx->node_cnt,
+                                       (const SpreadNode<T> *)ctx->spr_tmp, parts ? 1 : 0, (T)ctx->p.rho1,
+                                       (T)ctx->p.rho2, (T)ctx->p.gravity, d, (const C *)ctx->twx);
                 });
             }
             {
@@ -505,9 +511,18 @@ struct Impl {
         ctx->t++;
     }
 
+    // unit entries: the caller's half spectrum is unpadded ([nz][ny][nx/2+1])
+    struct Unpadded {
+        fipi_ctx *ctx;
+        int saved;
+        explicit Unpadded(fipi_ctx *c) : ctx(c), saved(c->d.nxp) { ctx->d.nxp = ctx->d.nxh; }
+        ~Unpadded() { ctx->d.nxp = saved; }
+    };
+
     // unit entry: R2C of one field
     static void r2c(fipi_ctx *ctx, const T *in, C *out)
     {
+        Unpadded u(ctx);
         x_forward<1>(ctx, ProdCopy<T>{in}, out, nullptr);
         y_pass<-1, 1, false>(ctx, out, nullptr, nullptr, nullptr);
         z_pass<-1>(ctx, out);
@@ -515,6 +530,7 @@ struct Impl {
 
     static void c2r(fipi_ctx *ctx, C *in, T *out)
     {
+        Unpadded u(ctx);
         z_pass<+1>(ctx, in);
         y_pass<+1, 1, false>(ctx, in, nullptr, nullptr, nullptr);
         x_inverse<1>(ctx, in, nullptr, nullptr, out, nullptr, nullptr);
@@ -685,6 +701,7 @@ int setup_buffers(fipi_ctx *ctx)
     ALLOC(ctx->line_cnt, sizeof(int) * (ctx->nlines + 1));
     ALLOC(ctx->line_start, sizeof(int) * (ctx->nlines + 1));
     ALLOC(ctx->line_cursor, sizeof(int) * (ctx->nlines + 1));
+    ALLOC(ctx->node_cnt, sizeof(int) * (ctx->nlines + 1));
     HIPCHK(ctx, hipMemset(ctx->line_start, 0, sizeof(int) * (ctx->nlines + 1)));
     const size_t n = ctx->n > 0 ? ctx->n : 1;
     ALLOC(ctx->cell_ids, sizeof(int) * n);
@@ -830,9 +847,13 @@ int fipi_create(const fipi_params *params, fipi_ctx **out)
     fipi_ctx *ctx = new fipi_ctx();
     ctx->p = *params;
     ctx->fp64 = params->precision == FIPI_FP64;
-    ctx->d = Dims{params->nx, params->ny, params->nz, params->nx / 2 + 1};
+    {
+        const int nxh = params->nx / 2 + 1;
+        const int q = params->precision == FIPI_FP64 ? 8 : 16;  // complex values per 128 bytes
+        ctx->d = Dims{params->nx, params->ny, params->nz, nxh, (nxh + q - 1) / q * q};
+    }
     ctx->G = (size_t)params->nx * params->ny * params->nz;
-    ctx->GH = (size_t)ctx->d.nxh * params->ny * params->nz;
+    ctx->GH = (size_t)ctx->d.nxp * params->ny * params->nz;
     ctx->n = params->switch_particle ? params->n_particles : 0;
     hipError_t e = hipGetDevice(&ctx->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -1251,13 +1272,13 @@ int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32
     if (ctx->fp64) {
         Impl<double>::spread_build(ctx, pp, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], f[0], f[1], f[2]);
         hipLaunchKernelGGL((k_spread_dense<double>), grid1(ctx->nlines, 256), dim3(256), 0, ctx->stream, ctx->nlines,
-                           ctx->p.nx, ctx->G, ctx->line_start, (const SpreadEntry<double> *)ctx->spr,
-                           (double *)d_grid3);
+                           ctx->p.nx, ctx->G, ctx->line_start, (const int *)ctx->node_cnt,
+                           (const SpreadNode<double> *)ctx->spr_tmp, (double *)d_grid3);
     } else {
         Impl<float>::spread_build(ctx, pp, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], f[0], f[1], f[2]);
         hipLaunchKernelGGL((k_spread_dense<float>), grid1(ctx->nlines, 256), dim3(256), 0, ctx->stream, ctx->nlines,
-                           ctx->p.nx, ctx->G, ctx->line_start, (const SpreadEntry<float> *)ctx->spr,
-                           (float *)d_grid3);
+                           ctx->p.nx, ctx->G, ctx->line_start, (const int *)ctx->node_cnt,
+                           (const SpreadNode<float> *)ctx->spr_tmp, (float *)d_grid3);
     }
     return check_flags(ctx);
 }

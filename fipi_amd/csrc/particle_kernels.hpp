@@ -727,45 +727,63 @@ __global__ __launch_bounds__(256) void k_spread_rank(int nlines, const int *__re
     }
 }
 
+// Per-node sums of the sorted entries of each line: one wave per line; the
+// lane at the head of each run of equal x sums the run in ascending particle
+// order (the reference's accumulation order, cluster.cpp:355-407) and the
+// heads are compacted with a wave ballot + popcount.  Node k of line l is
+// stored at nodes[line_start[l] + k]; node_cnt[l] gets the count.
 template <typename T>
-__global__ void k_spread_sort(int nlines, const int *__restrict__ line_start, SpreadEntry<T> *__restrict__ e)
+__global__ __launch_bounds__(256) void k_spread_nodes(int nlines, const int *__restrict__ line_start,
+                                                      const SpreadEntry<T> *__restrict__ e,
+                                                      SpreadNode<T> *__restrict__ nodes, int *__restrict__ node_cnt)
 {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nlines) return;
-    const int b = line_start[l], en = line_start[l + 1];
-    for (int i = b + 1; i < en; i++) {
-        const SpreadEntry<T> v = e[i];
-        int j = i - 1;
-        while (j >= b && (e[j].x > v.x || (e[j].x == v.x && e[j].pid > v.pid))) {
-            e[j + 1] = e[j];
-            j--;
+    const int line = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (line >= nlines) return;
+    const int b = line_start[line], en = line_start[line + 1];
+    int base = 0;
+    for (int c0 = b; c0 < en; c0 += 64) {
+        const int i = c0 + lane;
+        const bool valid = i < en;
+        const bool head = valid && (i == b || e[i - 1].x != e[i].x);
+        const unsigned long long mask = __ballot(head);
+        if (head) {
+            const int x = e[i].x;
+            T s0 = T(0), s1 = T(0), s2 = T(0);
+            for (int j = i; j < en && e[j].x == x; j++) {
+                s0 += e[j].v[0];
+                s1 += e[j].v[1];
+                s2 += e[j].v[2];
+            }
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            SpreadNode<T> nd;
+            nd.x = x;
+            nd.v[0] = s0;
+            nd.v[1] = s1;
+            nd.v[2] = s2;
+            nodes[b + base + rank] = nd;
         }
-        e[j + 1] = v;
+        base += __popcll(mask);
     }
+    if (lane == 0) node_cnt[line] = base;
 }
 
-// Dense spread (unit entry point): one thread per grid line sums its sorted
-// entries per node into a zeroed 3-component grid.
+// Dense spread (unit entry point): one thread per grid line adds its
+// per-node sums into a zeroed 3-component grid.
 template <typename T>
 __global__ void k_spread_dense(int nlines, int nx, size_t G, const int *__restrict__ line_start,
-                               const SpreadEntry<T> *__restrict__ e, T *__restrict__ grid3)
+                               const int *__restrict__ node_cnt, const SpreadNode<T> *__restrict__ nodes,
+                               T *__restrict__ grid3)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nlines) return;
-    int i = line_start[l];
-    const int en = line_start[l + 1];
-    while (i < en) {
-        const int x = e[i].x;
-        T s0 = T(0), s1 = T(0), s2 = T(0);
-        for (; i < en && e[i].x == x; i++) {
-            s0 += e[i].v[0];
-            s1 += e[i].v[1];
-            s2 += e[i].v[2];
-        }
-        const size_t node = (size_t)l * nx + x;
-        grid3[node] += s0;
-        grid3[G + node] += s1;
-        grid3[2 * G + node] += s2;
+    const int b = line_start[l], cnt = node_cnt[l];
+    for (int k = 0; k < cnt; k++) {
+        const SpreadNode<T> nd = nodes[b + k];
+        const size_t node = (size_t)l * nx + nd.x;
+        grid3[node] += nd.v[0];
+        grid3[G + node] += nd.v[1];
+        grid3[2 * G + node] += nd.v[2];
     }
 }
 
