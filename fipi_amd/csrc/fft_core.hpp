@@ -134,54 +134,90 @@ template <int R, int SIGN, typename C> __device__ __forceinline__ void dft(C *v)
     }
 }
 
-// Factorisation of each supported line length: N = N1 * N2.
-template <int N> struct Split;
-template <> struct Split<16> { static constexpr int N1 = 4, N2 = 4; };
-template <> struct Split<32> { static constexpr int N1 = 8, N2 = 4; };
-template <> struct Split<64> { static constexpr int N1 = 8, N2 = 8; };
-template <> struct Split<128> { static constexpr int N1 = 16, N2 = 8; };
-template <> struct Split<256> { static constexpr int N1 = 16, N2 = 16; };
+// ---------------------------------------------------------------------------
+// Line transforms with E values per thread.
+//
+// A length-N line is handled by P = N/E threads.  Distribution D: thread t
+// holds elements t + q*P, q < E.  The transform is a Stockham auto-sort
+// sequence of radix stages (Govindaraju et al., "High performance discrete
+// Fourier transforms on graphics processors", SC'08): stage s with radix R
+// and span Ns takes butterfly j from x[j + q N/R], multiplies by
+// W_{Ns R}^{(j mod Ns) q}, runs a DFT-R and writes
+// y[(j/Ns) Ns R + (j mod Ns) + q Ns].  The first and last stages have radix
+// E, so the first stage reads D and the last stage leaves the result in D:
+// a forward transform, a pointwise operation and an inverse transform chain
+// in registers, and loads/stores in D are coalesced (t fastest along a
+// contiguous line, or 128-byte tiles of neighbouring lines).  Middle stages
+// exchange through LDS (one padded slot per E values).
+// Plans: 256 = 8.4.8, 128 = 8.2.8, 64 = 8.8, 32 = 4.2.4, 16 = 4.4.
+// ---------------------------------------------------------------------------
+template <int N> struct Plan;
+template <> struct Plan<16> { static constexpr int E = 4, M = 0; };
+template <> struct Plan<32> { static constexpr int E = 4, M = 2; };
+template <> struct Plan<64> { static constexpr int E = 8, M = 0; };
+template <> struct Plan<128> { static constexpr int E = 8, M = 2; };
+template <> struct Plan<256> { static constexpr int E = 8, M = 4; };
 
-template <int A, int B> struct cmax { static constexpr int v = A > B ? A : B; };
-
-// Threads per line and register slots per thread for a length-N line.
-template <int N> struct LineShape {
-    static constexpr int N1 = Split<N>::N1;
-    static constexpr int N2 = Split<N>::N2;
-    static constexpr int P = cmax<N1, N2>::v;      // threads per line
-    static constexpr int M = cmax<N1, N2>::v;      // register slots per thread
-    // complex LDS slots per line: padded transpose buffer for either split
-    static constexpr int LDS = cmax<N1 * (N2 + 1), N2 * (N1 + 1)>::v;
-    static_assert(LDS >= N, "line buffer must hold a natural-order line");
+template <int N> struct Line {
+    static constexpr int E = Plan<N>::E;   // values per thread
+    static constexpr int P = N / E;        // threads per line
+    static constexpr int LL = N + N / E;   // padded LDS slots per line
 };
 
-// Two-factor transform with split (A, B) = (first-stage thread count,
-// second-stage thread count), A*B = N.  Input: thread t < A holds
-// v[m] = x[t + A*m], m < B.  Output: thread t < B holds v[m] = X[t + B*m],
-// m < A.  tw[j] = exp(-2 pi i j / N) (forward table; conjugated for SIGN>0).
-// Must be called by every thread of the block (contains barriers).
-template <typename T, int N, int A, int SIGN>
-__device__ __forceinline__ void fft_line(cplx<T> *v, cplx<T> *lds, int t, const cplx<T> *__restrict__ tw)
+template <int E> __device__ __forceinline__ int lpad(int i) { return i + i / E; }
+
+// W_N^(SIGN idx) from the forward table (conjugated for SIGN > 0)
+template <int SIGN, typename C>
+__device__ __forceinline__ C tw_at(const C *__restrict__ tw, int idx)
 {
-    constexpr int B = N / A;
+    C w = tw[idx];
+    if constexpr (SIGN > 0) w.y = -w.y;
+    return w;
+}
+
+template <typename T, int N, int SIGN, int RM, int NS>
+__device__ __forceinline__ void mid_stage(cplx<T> *lds, int t, const cplx<T> *__restrict__ tw)
+{
+    constexpr int E = Line<N>::E, P = Line<N>::P, NB = E / RM;
     using C = cplx<T>;
-    if (t < A) {
-        dft<B, SIGN>(v);
+    C u[E];
 #pragma unroll
-        for (int k2 = 1; k2 < B; k2++) {
-            C w = tw[t * k2];
-            if constexpr (SIGN > 0) w.y = -w.y;
-            v[k2] = cmul(v[k2], w);
-        }
+    for (int b = 0; b < NB; b++) {
+        const int j = t + b * P;
 #pragma unroll
-        for (int k2 = 0; k2 < B; k2++) lds[t * (B + 1) + k2] = v[k2];
+        for (int q = 0; q < RM; q++) u[b * RM + q] = lds[lpad<E>(j + q * (N / RM))];
     }
     __syncthreads();
-    if (t < B) {
 #pragma unroll
-        for (int n1 = 0; n1 < A; n1++) v[n1] = lds[n1 * (B + 1) + t];
-        dft<A, SIGN>(v);
+    for (int b = 0; b < NB; b++) {
+        const int j = t + b * P;
+        const int k = j % NS;
+#pragma unroll
+        for (int q = 1; q < RM; q++) u[b * RM + q] = cmul(u[b * RM + q], tw_at<SIGN>(tw, k * q * (N / (NS * RM))));
+        dft<RM, SIGN>(u + b * RM);
+        const int base = (j / NS) * NS * RM + k;
+#pragma unroll
+        for (int q = 0; q < RM; q++) lds[lpad<E>(base + q * NS)] = u[b * RM + q];
     }
+    __syncthreads();
+}
+
+// In-place transform of one line held in D by threads t < P (every thread of
+// the line must call it: it contains barriers).  lds: LL slots of this line.
+template <typename T, int N, int SIGN>
+__device__ __forceinline__ void fft_dist(cplx<T> *v, cplx<T> *lds, int t, const cplx<T> *__restrict__ tw)
+{
+    constexpr int E = Line<N>::E, P = Line<N>::P;
+    dft<E, SIGN>(v);
+#pragma unroll
+    for (int q = 0; q < E; q++) lds[lpad<E>(t * E + q)] = v[q];
+    __syncthreads();
+    if constexpr (Plan<N>::M > 0) mid_stage<T, N, SIGN, Plan<N>::M, E>(lds, t, tw);
+#pragma unroll
+    for (int q = 0; q < E; q++) v[q] = lds[lpad<E>(t + q * P)];
+#pragma unroll
+    for (int q = 1; q < E; q++) v[q] = cmul(v[q], tw_at<SIGN>(tw, t * q));
+    dft<E, SIGN>(v);
     __syncthreads();
 }
 

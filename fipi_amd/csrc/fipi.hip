@@ -147,12 +147,6 @@ std::vector<T> twiddles(int n)
     return w;
 }
 
-int threads_per_line(int n)
-{
-    int r = 0;
-    with_n(n, [&](auto NC) { r = LineShape<decltype(NC)::value>::P; });
-    return r;
-}
 
 // Timing scope around one launch site when profiling is on: HIP events on
 // the context stream, plus the site's algorithmic bytes (each array the
@@ -187,6 +181,20 @@ struct ProfScope {
 
 // at least one block: every kernel bounds-checks its index
 inline dim3 grid1(size_t n, int b) { return dim3((unsigned)(n == 0 ? 1 : (n + b - 1) / b)); }
+
+// x-pass grid: XT threads per workgroup, XT / P complex lines (2 real lines each)
+template <int N>
+dim3 xgrid(const Dims &d)
+{
+    return dim3((unsigned)(d.ny * d.nz / (2 * (XT / Line<N>::P))));
+}
+
+int xblocks(const Dims &d)
+{
+    int r = 0;
+    with_n(d.nx, [&](auto NC) { r = (int)xgrid<decltype(NC)::value>(d).x; });
+    return r;
+}
 
 // Exclusive scan of n counts into start[0..n] on the context stream;
 // bsum holds ceil(n/1024) block totals.  Optionally reduces argmax-y
@@ -231,8 +239,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_forward<T, N, NF, Prod>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
-                               dim3(TILE * LineShape<N>::P), 0, ctx->stream, prod, o0, o1, d,
+            hipLaunchKernelGGL((k_x_forward<T, N, NF, Prod>), xgrid<N>(d), dim3(XT), 0, ctx->stream, prod, o0, o1, d,
                                (const C *)ctx->twx, part);
         });
     }
@@ -243,8 +250,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_inverse<T, N, NF>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
-                               dim3(TILE * LineShape<N>::P), 0, ctx->stream, i0, i1, i2, o0, o1, o2, d,
+            hipLaunchKernelGGL((k_x_inverse<T, N, NF>), xgrid<N>(d), dim3(XT), 0, ctx->stream, i0, i1, i2, o0, o1, o2, d,
                                (const C *)ctx->twx);
         });
     }
@@ -255,15 +261,15 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.ny, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD>), dim3((unsigned)((d.nxh + TILE - 1) / TILE), d.nz),
-                               dim3(TILE * LineShape<N>::P), 0, ctx->stream, a0, a1, a2, extra, d, ctx->ky,
+            hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD>), dim3((unsigned)((d.nxh + Tile<T>::L - 1) / Tile<T>::L), d.nz),
+                               dim3(Tile<T>::L * Line<N>::P), 0, ctx->stream, a0, a1, a2, extra, d, ctx->ky,
                                (const C *)ctx->twy);
         });
     }
 
     static dim3 zgrid(const fipi_ctx *ctx)
     {
-        return dim3((unsigned)((ctx->d.nxh + TILE - 1) / TILE), ctx->d.ny);
+        return dim3((unsigned)((ctx->d.nxh + Tile<T>::L - 1) / Tile<T>::L), ctx->d.ny);
     }
 
     template <int SIGN>
@@ -272,7 +278,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nz, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_z<T, N, SIGN>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, ctx->stream, a, d,
+            hipLaunchKernelGGL((k_z<T, N, SIGN>), zgrid(ctx), dim3(Tile<T>::L * Line<N>::P), 0, ctx->stream, a, d,
                                (const C *)ctx->twz);
         });
     }
@@ -287,7 +293,7 @@ struct Impl {
         const KParams kp = kparams(ctx);
         with_n(d.nz, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_hub_init<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, ctx->stream,
+            hipLaunchKernelGGL((k_hub_init<T, N>), zgrid(ctx), dim3(Tile<T>::L * Line<N>::P), 0, ctx->stream,
                                H(ctx->h[0]), H(ctx->h[1]), H(ctx->ck[ctx->cur]), H(ctx->fk), kp, d, ctx->kz,
                                (const C *)ctx->twz);
         });
@@ -300,8 +306,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_cgrad_f<T, N>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
-                               dim3(TILE * LineShape<N>::P), 0, ctx->stream, H(ctx->h[0]), H(ctx->h[2]),
+            hipLaunchKernelGGL((k_x_cgrad_f<T, N>), xgrid<N>(d), dim3(XT), 0, ctx->stream, H(ctx->h[0]), H(ctx->h[2]),
                                H(ctx->h[1]), R(ctx->c), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), H(ctx->h[0]),
                                fscale(ctx), d, ctx->kx, (const C *)ctx->twx);
         });
@@ -421,9 +426,10 @@ struct Impl {
             }
             if (mixed) {
                 // FP64 conv(0) = sum of conv over the grid, fixed order
-                ProfScope ps_(ctx, "k0_sum", 8.0 * (d.ny * d.nz / (2 * TILE)));
-                hipLaunchKernelGGL(k_sum_strided, dim3(1), dim3(256), 0, st, (const double *)ctx->conv_part,
-                                   d.ny * d.nz / (2 * TILE), 1, 1, ctx->conv0);
+                const int nb = xblocks(d);
+                ProfScope ps_(ctx, "k0_sum", 8.0 * nb);
+                hipLaunchKernelGGL(k_sum_strided, dim3(1), dim3(256), 0, st, (const double *)ctx->conv_part, nb, 1,
+                                   1, ctx->conv0);
             }
             ProfScope ps_(ctx, "y_fwd_1", 2 * Hb);
             y_pass<-1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
@@ -434,7 +440,7 @@ struct Impl {
             ProfScope ps_(ctx, "z_hub_ch", (ns ? 8 : 7) * Hb);
             with_n(d.nz, [&](auto NC) {
                 constexpr int N = decltype(NC)::value;
-                hipLaunchKernelGGL((k_hub_ch<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0, h1,
+                hipLaunchKernelGGL((k_hub_ch<T, N>), zgrid(ctx), dim3(Tile<T>::L * Line<N>::P), 0, st, h0, h1,
                                    (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]), (const C *)ctx->fk,
                                    H(ctx->hist), kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz,
                                    K0{mixed ? ctx->k0 : nullptr, ctx->conv0, ctx->cur});
@@ -457,7 +463,7 @@ struct Impl {
             ProfScope ps_(ctx, "z_hub_mu", 4 * Hb);
             with_n(d.nz, [&](auto NC) {
                 constexpr int N = decltype(NC)::value;
-                hipLaunchKernelGGL((k_hub_mu<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0,
+                hipLaunchKernelGGL((k_hub_mu<T, N>), zgrid(ctx), dim3(Tile<T>::L * Line<N>::P), 0, st, h0,
                                    H(ctx->fk), (const C *)ctx->ck[ctx->cur], kp, d, ctx->kx, ctx->ky, ctx->kz,
                                    (const C *)ctx->twz);
             });
@@ -475,8 +481,7 @@ struct Impl {
                 ProfScope ps_(ctx, "x_inv_force_fwd", 4 * Hb + 4 * Rb + (parts ? 8.0 * ctx->n * sizeof(SpreadEntry<T>) : 0.0));
                 with_n(d.nx, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_x_force<T, N>), dim3((unsigned)(d.ny * d.nz / (2 * TILE))),
-                                       dim3(TILE * LineShape<N>::P), 0, st, h0, h1, h2, (const T *)R(ctx->c),
+                    hipLaunchKernelGGL((k_x_force<T, N>), xgrid<N>(d), dim3(XT), 0, st, h0, h1, h2, (const T *)R(ctx->c),
                                        (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
                                        (const int *)ctx->line_start, (const int *)ctx->node_cnt,
                                        (const SpreadNode<T> *)ctx->spr_tmp, parts ? 1 : 0, (T)ctx->p.rho1,
@@ -491,7 +496,7 @@ struct Impl {
                 ProfScope ps_(ctx, "z_hub_stokes", 6 * Hb);
                 with_n(d.nz, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_hub_stokes<T, N>), zgrid(ctx), dim3(TILE * LineShape<N>::P), 0, st, h0,
+                    hipLaunchKernelGGL((k_hub_stokes<T, N>), zgrid(ctx), dim3(Tile<T>::L * Line<N>::P), 0, st, h0,
                                        h1, h2, kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
                 });
             }
@@ -616,6 +621,13 @@ int check_params(const fipi_params *p)
     if (!supported_n(p->nx) || !supported_n(p->ny) || !supported_n(p->nz))
         return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "grid sizes must each be one of 16, 32, 64, 128, 256");
     if (p->switch_ch != 1) return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "switch_ch must be 1");
+    {
+        // the x pass takes 2*XT/P real lines per workgroup
+        int lines = 0;
+        with_n(p->nx, [&](auto NC) { lines = 2 * (XT / Line<decltype(NC)::value>::P); });
+        if (((int64_t)p->ny * p->nz) % lines != 0)
+            return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "ny*nz must be a multiple of the x-pass line count");
+    }
     if (p->precision != FIPI_FP64 && p->precision != FIPI_FP32_MIXED)
         return set_err(nullptr, FIPI_ERR_ARG, "precision must be FIPI_FP64 or FIPI_FP32_MIXED");
     if (p->n_particles < 0) return set_err(nullptr, FIPI_ERR_ARG, "n_particles < 0");
@@ -661,7 +673,7 @@ int setup_buffers(fipi_ctx *ctx)
     ALLOC(ctx->d_red, sizeof(double) * (3 * DIAG_BLOCKS + 16));
     ALLOC(ctx->k0, sizeof(double) * 4);
     ALLOC(ctx->conv0, sizeof(double) * 2);
-    ALLOC(ctx->conv_part, sizeof(double) * (p.ny * p.nz / (2 * TILE) + 1));
+    ALLOC(ctx->conv_part, sizeof(double) * (p.ny * p.nz / 2 + 1));
     auto kx = wave_numbers(p.nx, p.pi, p.lx), ky = wave_numbers(p.ny, p.pi, p.ly), kz = wave_numbers(p.nz, p.pi, p.lz);
     HIPCHK(ctx, hipMemcpy(ctx->kx, kx.data(), sizeof(double) * p.nx, hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->ky, ky.data(), sizeof(double) * p.ny, hipMemcpyHostToDevice));
