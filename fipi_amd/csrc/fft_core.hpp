@@ -1,21 +1,12 @@
 // Device FFT building blocks for gfx950 (wave64, 160 KB LDS per CU).
 //
-// A length-N line transform is split N = N1 * N2 ("four-step" with one LDS
-// transpose).  Index n = n1 + N1*n2 on input, k = k2 + N2*k1 on output:
-//
-//   X[k2 + N2 k1] = sum_n1 W_N^{n1 k2} W_N1^{n1 k1} sum_n2 x[n1 + N1 n2] W_N2^{n2 k2}
-//
-// Stage A: thread n1 (< N1) holds x[n1 + N1*n2] for n2 < N2 in registers,
-//          runs a DFT-N2, multiplies by W_N^{n1 k2}, writes to LDS.
-// Stage B: thread k2 (< N2) reads the N1 values of its column, runs a
-//          DFT-N1 and holds X[k2 + N2*k1] for k1 < N1.
-// The output "distribution" (thread t holds elements t + N2*m) is the input
-// distribution of a transform split the other way round (N1' = N2), so a
-// forward transform, a pointwise k-space operation and an inverse transform
-// chain in registers with exactly one LDS transpose per transform.  Loads
-// and stores in this distribution are coalesced when consecutive threads
-// take consecutive t (contiguous lines) or when 16 neighbouring lines are
-// processed side by side (strided lines, 16-wide tiles).
+// A length-N line is held by P = N/E threads with E values each, in the
+// distribution D: thread t holds elements t + q*P, q < E.  The transform is
+// a Stockham auto-sort sequence of radix stages whose first and last stages
+// have radix E, so a line enters and leaves every transform in D.  A forward
+// transform, a pointwise k-space operation and an inverse transform
+// therefore chain in registers, and loads/stores in D are coalesced (t
+// fastest along a contiguous line, or neighbouring lines side by side).
 //
 // Twiddles W_N^t come from a host-computed table (long-double accurate).
 // The small DFTs use the exact constants cos/sin(2 pi j/16).
@@ -137,31 +128,50 @@ template <int R, int SIGN, typename C> __device__ __forceinline__ void dft(C *v)
 // ---------------------------------------------------------------------------
 // Line transforms with E values per thread.
 //
-// A length-N line is handled by P = N/E threads.  Distribution D: thread t
-// holds elements t + q*P, q < E.  The transform is a Stockham auto-sort
-// sequence of radix stages (Govindaraju et al., "High performance discrete
-// Fourier transforms on graphics processors", SC'08): stage s with radix R
-// and span Ns takes butterfly j from x[j + q N/R], multiplies by
-// W_{Ns R}^{(j mod Ns) q}, runs a DFT-R and writes
-// y[(j/Ns) Ns R + (j mod Ns) + q Ns].  The first and last stages have radix
-// E, so the first stage reads D and the last stage leaves the result in D:
-// a forward transform, a pointwise operation and an inverse transform chain
-// in registers, and loads/stores in D are coalesced (t fastest along a
-// contiguous line, or 128-byte tiles of neighbouring lines).  Middle stages
-// exchange through LDS (one padded slot per E values).
-// Plans: 256 = 8.4.8, 128 = 8.2.8, 64 = 8.8, 32 = 4.2.4, 16 = 4.4.
+// Stockham stage s with radix R and span Ns (Govindaraju et al., "High
+// performance discrete Fourier transforms on graphics processors", SC'08)
+// takes butterfly j from x[j + q N/R], multiplies by W_{Ns R}^{(j mod Ns) q},
+// runs a DFT-R and writes y[(j/Ns) Ns R + (j mod Ns) + q Ns].  With first
+// and last radix E the first stage reads D and the last leaves the result in
+// D.  Middle stages exchange through LDS.
+//
+// LDS slot of element i of a line: lpad(i) * S, with lpad(i) = i + i/E (one
+// spare slot per E values, so the stride-E writes of the first stage spread
+// over the banks) and S lines interleaved slot by slot (the y and z tiles
+// put their S = 128-byte-wide columns side by side, so a wave's accesses
+// are runs of S consecutive values).
+//
+// Plans (E.M.E): 256 = 8.4.8 or 16.16, 128 = 8.2.8, 64 = 8.8, 32 = 4.2.4,
+// 16 = 4.4.  FP64 lines of 256 use E = 8 (fewer registers for kernels that
+// keep three fields in flight), FP32 lines of 256 use E = 16 (one LDS
+// exchange instead of two).
 // ---------------------------------------------------------------------------
-template <int N> struct Plan;
-template <> struct Plan<16> { static constexpr int E = 4, M = 0; };
-template <> struct Plan<32> { static constexpr int E = 4, M = 2; };
-template <> struct Plan<64> { static constexpr int E = 8, M = 0; };
-template <> struct Plan<128> { static constexpr int E = 8, M = 2; };
-template <> struct Plan<256> { static constexpr int E = 8, M = 4; };
+template <int N, int E> struct PlanE;
+template <> struct PlanE<16, 4> { static constexpr int M = 0; };
+template <> struct PlanE<32, 4> { static constexpr int M = 2; };
+template <> struct PlanE<64, 8> { static constexpr int M = 0; };
+template <> struct PlanE<128, 8> { static constexpr int M = 2; };
+template <> struct PlanE<256, 8> { static constexpr int M = 4; };
+template <> struct PlanE<256, 16> { static constexpr int M = 0; };
 
-template <int N> struct Line {
-    static constexpr int E = Plan<N>::E;   // values per thread
-    static constexpr int P = N / E;        // threads per line
-    static constexpr int LL = N + N / E;   // padded LDS slots per line
+#ifndef FIPI_E256_F64
+#define FIPI_E256_F64 8
+#endif
+#ifndef FIPI_E256_F32
+#define FIPI_E256_F32 8
+#endif
+
+template <typename T, int N> struct PlanE_for {
+    static constexpr int E = N <= 32 ? 4 : 8;
+};
+template <> struct PlanE_for<double, 256> { static constexpr int E = FIPI_E256_F64; };
+template <> struct PlanE_for<float, 256> { static constexpr int E = FIPI_E256_F32; };
+
+template <typename T, int N> struct Line {
+    static constexpr int E = PlanE_for<T, N>::E;   // values per thread
+    static constexpr int M = PlanE<N, E>::M;       // middle radix (0: none)
+    static constexpr int P = N / E;                // threads per line
+    static constexpr int LL = N + N / E;           // padded LDS slots per line
 };
 
 template <int E> __device__ __forceinline__ int lpad(int i) { return i + i / E; }
@@ -175,17 +185,17 @@ __device__ __forceinline__ C tw_at(const C *__restrict__ tw, int idx)
     return w;
 }
 
-template <typename T, int N, int SIGN, int RM, int NS>
+template <typename T, int N, int SIGN, int S>
 __device__ __forceinline__ void mid_stage(cplx<T> *lds, int t, const cplx<T> *__restrict__ tw)
 {
-    constexpr int E = Line<N>::E, P = Line<N>::P, NB = E / RM;
+    constexpr int E = Line<T, N>::E, P = Line<T, N>::P, RM = Line<T, N>::M, NS = E, NB = E / RM;
     using C = cplx<T>;
     C u[E];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
         const int j = t + b * P;
 #pragma unroll
-        for (int q = 0; q < RM; q++) u[b * RM + q] = lds[lpad<E>(j + q * (N / RM))];
+        for (int q = 0; q < RM; q++) u[b * RM + q] = lds[lpad<E>(j + q * (N / RM)) * S];
     }
     __syncthreads();
 #pragma unroll
@@ -197,24 +207,25 @@ __device__ __forceinline__ void mid_stage(cplx<T> *lds, int t, const cplx<T> *__
         dft<RM, SIGN>(u + b * RM);
         const int base = (j / NS) * NS * RM + k;
 #pragma unroll
-        for (int q = 0; q < RM; q++) lds[lpad<E>(base + q * NS)] = u[b * RM + q];
+        for (int q = 0; q < RM; q++) lds[lpad<E>(base + q * NS) * S] = u[b * RM + q];
     }
     __syncthreads();
 }
 
 // In-place transform of one line held in D by threads t < P (every thread of
-// the line must call it: it contains barriers).  lds: LL slots of this line.
-template <typename T, int N, int SIGN>
+// the workgroup must call it: it contains barriers).  lds: slot 0 of this
+// line; slots are S values apart.
+template <typename T, int N, int SIGN, int S = 1>
 __device__ __forceinline__ void fft_dist(cplx<T> *v, cplx<T> *lds, int t, const cplx<T> *__restrict__ tw)
 {
-    constexpr int E = Line<N>::E, P = Line<N>::P;
+    constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
     dft<E, SIGN>(v);
 #pragma unroll
-    for (int q = 0; q < E; q++) lds[lpad<E>(t * E + q)] = v[q];
+    for (int q = 0; q < E; q++) lds[lpad<E>(t * E + q) * S] = v[q];
     __syncthreads();
-    if constexpr (Plan<N>::M > 0) mid_stage<T, N, SIGN, Plan<N>::M, E>(lds, t, tw);
+    if constexpr (Line<T, N>::M > 0) mid_stage<T, N, SIGN, S>(lds, t, tw);
 #pragma unroll
-    for (int q = 0; q < E; q++) v[q] = lds[lpad<E>(t + q * P)];
+    for (int q = 0; q < E; q++) v[q] = lds[lpad<E>(t + q * P) * S];
 #pragma unroll
     for (int q = 1; q < E; q++) v[q] = cmul(v[q], tw_at<SIGN>(tw, t * q));
     dft<E, SIGN>(v);

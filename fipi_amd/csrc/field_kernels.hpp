@@ -58,7 +58,7 @@ template <typename T, int N, bool DERIV>
 __device__ __forceinline__ void load_c2r(cplx<T> *v, const cplx<T> *__restrict__ in, size_t rowA, size_t rowB,
                                          int t, const double *__restrict__ kxv)
 {
-    constexpr int E = Line<N>::E, P = Line<N>::P;
+    constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
     using C = cplx<T>;
 #pragma unroll
     for (int q = 0; q < E; q++) {
@@ -83,7 +83,7 @@ template <typename T, int N>
 __device__ __forceinline__ void store_r2c(const cplx<T> *v, cplx<T> *L, cplx<T> *__restrict__ out, size_t rowA,
                                           size_t rowB, int t)
 {
-    constexpr int E = Line<N>::E, P = Line<N>::P;
+    constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
     using C = cplx<T>;
 #pragma unroll
     for (int q = 0; q < E; q++) L[t + q * P] = v[q];
@@ -100,7 +100,7 @@ __device__ __forceinline__ void store_r2c(const cplx<T> *v, cplx<T> *L, cplx<T> 
 
 // common x-pass geometry: complex line lp of the workgroup = real lines rl, rl+1
 #define FIPI_XPASS_SETUP(T, N)                                      \
-    constexpr int E = Line<N>::E, P = Line<N>::P, LL = Line<N>::LL; \
+    constexpr int E = Line<T, N>::E, P = Line<T, N>::P, LL = Line<T, N>::LL; \
     constexpr int LPW = XT / P;                                     \
     using C = cplx<T>;                                              \
     __shared__ C lds[LPW * LL];                                     \
@@ -370,21 +370,21 @@ __global__ __launch_bounds__(XT) void k_x_force(cplx<T> *__restrict__ mu_io /* i
 // t = position in the line (distribution D along y or z).
 // ---------------------------------------------------------------------------
 #define FIPI_TILE_SETUP(T, N, IDX2)                                   \
-    constexpr int E = Line<N>::E, P = Line<N>::P, LL = Line<N>::LL;   \
+    constexpr int E = Line<T, N>::E, P = Line<T, N>::P, LL = Line<T, N>::LL;   \
     constexpr int TL = Tile<T>::L;                                    \
     using C = cplx<T>;                                                \
     __shared__ C lds[TL * LL];                                        \
     const int lane = threadIdx.x % TL, t = threadIdx.x / TL;          \
     const int kx = blockIdx.x * TL + lane;                            \
     const bool ok = kx < d.nxh;                                       \
-    C *L = lds + lane * LL;                                           \
+    C *L = lds + lane;                                                \
     (void)E; (void)P;
 
 template <typename T, int N>
 __device__ __forceinline__ void tile_load(cplx<T> *v, const cplx<T> *__restrict__ a, size_t base, size_t st, int t,
                                           bool ok)
 {
-    constexpr int E = Line<N>::E, P = Line<N>::P;
+    constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
 #pragma unroll
     for (int q = 0; q < E; q++) v[q] = ok ? a[base + st * (t + q * P)] : cplx<T>{0, 0};
 }
@@ -393,7 +393,7 @@ template <typename T, int N>
 __device__ __forceinline__ void tile_store(const cplx<T> *v, cplx<T> *__restrict__ a, size_t base, size_t st, int t,
                                            bool ok)
 {
-    constexpr int E = Line<N>::E, P = Line<N>::P;
+    constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
     if (ok) {
 #pragma unroll
         for (int q = 0; q < E; q++) a[base + st * (t + q * P)] = v[q];
@@ -403,7 +403,7 @@ __device__ __forceinline__ void tile_store(const cplx<T> *v, cplx<T> *__restrict
 // y pass: transforms along y of NF fields in place.  GRAD (inverse only):
 // field 0 also yields i*ky*field0 into `extra` (zero on ky = NY/2).
 template <typename T, int N, int SIGN, int NF, bool GRAD>
-__global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_y(cplx<T> *__restrict__ a0, cplx<T> *__restrict__ a1,
+__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_y(cplx<T> *__restrict__ a0, cplx<T> *__restrict__ a1,
                                                               cplx<T> *__restrict__ a2, cplx<T> *__restrict__ extra,
                                                               Dims d, const double *__restrict__ kyv,
                                                               const cplx<T> *__restrict__ tw)
@@ -426,10 +426,10 @@ __global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_y(cplx<T> *__restric
                 const T kv = y == N / 2 ? T(0) : (T)kyv[y];
                 w[q] = C{-kv * v[0][q].y, kv * v[0][q].x};
             }
-            fft_dist<T, N, SIGN>(w, L, t, tw);
+            fft_dist<T, N, SIGN, TL>(w, L, t, tw);
             tile_store<T, N>(w, extra, base, st, t, ok);
         }
-        fft_dist<T, N, SIGN>(v[f], L, t, tw);
+        fft_dist<T, N, SIGN, TL>(v[f], L, t, tw);
         tile_store<T, N>(v[f], a, base, st, t, ok);
     }
 }
@@ -458,7 +458,7 @@ struct K0 {
 // Outputs c^{n+1} (K), the new history (K), and c^{n+1}/G, i kz c^{n+1}/G
 // inverse-z transformed (XY) for the gradient and real-space c.
 template <typename T, int N>
-__global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_ch(
+__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_ch(
     cplx<T> *__restrict__ conv_io /* in conv XY, out c XY */, cplx<T> *__restrict__ dzc_out,
     const cplx<T> *__restrict__ c_now, cplx<T> *__restrict__ c_next, const cplx<T> *__restrict__ f_now,
     cplx<T> *__restrict__ hist, KParams kp, Dims d, const double *__restrict__ kxv, const double *__restrict__ kyv,
@@ -474,7 +474,7 @@ __global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_ch(
     tile_load<T, N>(fn, f_now, base, st, t, ok);
     tile_load<T, N>(hh, hist, base, st, t, ok);
     if (kp.use_conv) {
-        fft_dist<T, N, -1>(v, L, t, tw);
+        fft_dist<T, N, -1, TL>(v, L, t, tw);
     } else {
 #pragma unroll
         for (int q = 0; q < E; q++) v[q] = C{0, 0};
@@ -540,15 +540,15 @@ __global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_ch(
         const T kz = z == N / 2 ? T(0) : (T)kzz;
         w[q] = C{-kz * v[q].y, kz * v[q].x};
     }
-    fft_dist<T, N, +1>(v, L, t, tw);
+    fft_dist<T, N, +1, TL>(v, L, t, tw);
     tile_store<T, N>(v, conv_io, base, st, t, ok);
-    fft_dist<T, N, +1>(w, L, t, tw);
+    fft_dist<T, N, +1, TL>(w, L, t, tw);
     tile_store<T, N>(w, dzc_out, base, st, t, ok);
 }
 
 // Initial hub: (c XY, f XY) -> c_hat^0, f_hat^0 (K) and c/G, i kz c/G (XY).
 template <typename T, int N>
-__global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_init(cplx<T> *__restrict__ c_io,
+__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_init(cplx<T> *__restrict__ c_io,
                                                                      cplx<T> *__restrict__ f_io /* out: dz c XY */,
                                                                      cplx<T> *__restrict__ c_k,
                                                                      cplx<T> *__restrict__ f_k, KParams kp, Dims d,
@@ -562,9 +562,9 @@ __global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_init(cplx<T> *__
     C fv[E], v[E], w[E];
     tile_load<T, N>(fv, f_io, base, st, t, ok);
     tile_load<T, N>(v, c_io, base, st, t, ok);
-    fft_dist<T, N, -1>(fv, L, t, tw);
+    fft_dist<T, N, -1, TL>(fv, L, t, tw);
     tile_store<T, N>(fv, f_k, base, st, t, ok);
-    fft_dist<T, N, -1>(v, L, t, tw);
+    fft_dist<T, N, -1, TL>(v, L, t, tw);
     tile_store<T, N>(v, c_k, base, st, t, ok);
     const T ig = (T)kp.inv_g;
 #pragma unroll
@@ -574,16 +574,16 @@ __global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_init(cplx<T> *__
         const T kz = z == N / 2 ? T(0) : (T)kzv[z];
         w[q] = C{-kz * v[q].y, kz * v[q].x};
     }
-    fft_dist<T, N, +1>(v, L, t, tw);
+    fft_dist<T, N, +1, TL>(v, L, t, tw);
     tile_store<T, N>(v, c_io, base, st, t, ok);
-    fft_dist<T, N, +1>(w, L, t, tw);
+    fft_dist<T, N, +1, TL>(w, L, t, tw);
     tile_store<T, N>(w, f_io, base, st, t, ok);
 }
 
 // P7: f (XY) -> f_hat^{n+1} (K); mu_hat = lambda k^2 c_hat + f_hat
 // (src/main.cpp:157) -> mu/G (XY, in place of f).
 template <typename T, int N>
-__global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_mu(cplx<T> *__restrict__ f_io,
+__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_mu(cplx<T> *__restrict__ f_io,
                                                                    cplx<T> *__restrict__ f_k,
                                                                    const cplx<T> *__restrict__ c_k, KParams kp,
                                                                    Dims d, const double *__restrict__ kxv,
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_mu(cplx<T> *__re
     C v[E], cc[E];
     tile_load<T, N>(v, f_io, base, st, t, ok);
     tile_load<T, N>(cc, c_k, base, st, t, ok);
-    fft_dist<T, N, -1>(v, L, t, tw);
+    fft_dist<T, N, -1, TL>(v, L, t, tw);
     tile_store<T, N>(v, f_k, base, st, t, ok);
     const double kxx = kxv[ok ? kx : 0], kyy = kyv[y];
 #pragma unroll
@@ -614,7 +614,7 @@ __global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_mu(cplx<T> *__re
         v[q] = C{(T)((ml * (double)cc[q].x + (double)v[q].x) * kp.inv_g),
                  (T)((ml * (double)cc[q].y + (double)v[q].y) * kp.inv_g)};
     }
-    fft_dist<T, N, +1>(v, L, t, tw);
+    fft_dist<T, N, +1, TL>(v, L, t, tw);
     tile_store<T, N>(v, f_io, base, st, t, ok);
 }
 
@@ -626,7 +626,7 @@ __global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_mu(cplx<T> *__re
 // inverse of the Hermitian part, which drops the k_d k_e cross terms
 // between Nyquist and non-Nyquist components; that is what is formed here.
 template <typename T, int N>
-__global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_stokes(cplx<T> *__restrict__ a0,
+__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_stokes(cplx<T> *__restrict__ a0,
                                                                        cplx<T> *__restrict__ a1,
                                                                        cplx<T> *__restrict__ a2, KParams kp, Dims d,
                                                                        const double *__restrict__ kxv,
@@ -642,9 +642,9 @@ __global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_stokes(cplx<T> *
     tile_load<T, N>(v[0], a0, base, st, t, ok);
     tile_load<T, N>(v[1], a1, base, st, t, ok);
     tile_load<T, N>(v[2], a2, base, st, t, ok);
-    fft_dist<T, N, -1>(v[0], L, t, tw);
-    fft_dist<T, N, -1>(v[1], L, t, tw);
-    fft_dist<T, N, -1>(v[2], L, t, tw);
+    fft_dist<T, N, -1, TL>(v[0], L, t, tw);
+    fft_dist<T, N, -1, TL>(v[1], L, t, tw);
+    fft_dist<T, N, -1, TL>(v[2], L, t, tw);
     const int kxi = ok ? kx : 0;
     const double kxx = kxv[kxi], kyy = kyv[y];
     const bool nx_ = kxi == d.nx / 2, ny_ = y == d.ny / 2;
@@ -678,17 +678,17 @@ __global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_hub_stokes(cplx<T> *
             v[e][q] = C{(T)(ur * kp.inv_g), (T)(ui * kp.inv_g)};
         }
     }
-    fft_dist<T, N, +1>(v[0], L, t, tw);
+    fft_dist<T, N, +1, TL>(v[0], L, t, tw);
     tile_store<T, N>(v[0], a0, base, st, t, ok);
-    fft_dist<T, N, +1>(v[1], L, t, tw);
+    fft_dist<T, N, +1, TL>(v[1], L, t, tw);
     tile_store<T, N>(v[1], a1, base, st, t, ok);
-    fft_dist<T, N, +1>(v[2], L, t, tw);
+    fft_dist<T, N, +1, TL>(v[2], L, t, tw);
     tile_store<T, N>(v[2], a2, base, st, t, ok);
 }
 
 // Plain forward / inverse z transform of one field in place (unit entries).
 template <typename T, int N, int SIGN>
-__global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_z(cplx<T> *__restrict__ a, Dims d,
+__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_z(cplx<T> *__restrict__ a, Dims d,
                                                               const cplx<T> *__restrict__ tw)
 {
     FIPI_TILE_SETUP(T, N, 0)
@@ -697,7 +697,7 @@ __global__ __launch_bounds__(Tile<T>::L *Line<N>::P) void k_z(cplx<T> *__restric
     const size_t st = (size_t)d.nxp * d.ny;
     C v[E];
     tile_load<T, N>(v, a, base, st, t, ok);
-    fft_dist<T, N, SIGN>(v, L, t, tw);
+    fft_dist<T, N, SIGN, TL>(v, L, t, tw);
     tile_store<T, N>(v, a, base, st, t, ok);
 }
 

@@ -17,6 +17,7 @@
 //   two Heun particle stages                                   (main.cpp:198-217)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -183,16 +184,17 @@ struct ProfScope {
 inline dim3 grid1(size_t n, int b) { return dim3((unsigned)(n == 0 ? 1 : (n + b - 1) / b)); }
 
 // x-pass grid: XT threads per workgroup, XT / P complex lines (2 real lines each)
-template <int N>
+template <typename T, int N>
 dim3 xgrid(const Dims &d)
 {
-    return dim3((unsigned)(d.ny * d.nz / (2 * (XT / Line<N>::P))));
+    return dim3((unsigned)(d.ny * d.nz / (2 * (XT / Line<T, N>::P))));
 }
 
+template <typename T>
 int xblocks(const Dims &d)
 {
     int r = 0;
-    with_n(d.nx, [&](auto NC) { r = (int)xgrid<decltype(NC)::value>(d).x; });
+    with_n(d.nx, [&](auto NC) { r = (int)xgrid<T, decltype(NC)::value>(d).x; });
     return r;
 }
 
@@ -239,7 +241,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_forward<T, N, NF, Prod>), xgrid<N>(d), dim3(XT), 0, ctx->stream, prod, o0, o1, d,
+            hipLaunchKernelGGL((k_x_forward<T, N, NF, Prod>), (xgrid<T, N>(d)), dim3(XT), 0, ctx->stream, prod, o0, o1, d,
                                (const C *)ctx->twx, part);
         });
     }
@@ -250,7 +252,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_inverse<T, N, NF>), xgrid<N>(d), dim3(XT), 0, ctx->stream, i0, i1, i2, o0, o1, o2, d,
+            hipLaunchKernelGGL((k_x_inverse<T, N, NF>), (xgrid<T, N>(d)), dim3(XT), 0, ctx->stream, i0, i1, i2, o0, o1, o2, d,
                                (const C *)ctx->twx);
         });
     }
@@ -262,7 +264,7 @@ struct Impl {
         with_n(d.ny, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
             hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD>), dim3((unsigned)((d.nxh + Tile<T>::L - 1) / Tile<T>::L), d.nz),
-                               dim3(Tile<T>::L * Line<N>::P), 0, ctx->stream, a0, a1, a2, extra, d, ctx->ky,
+                               dim3((Tile<T>::L * Line<T, N>::P)), 0, ctx->stream, a0, a1, a2, extra, d, ctx->ky,
                                (const C *)ctx->twy);
         });
     }
@@ -278,7 +280,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nz, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_z<T, N, SIGN>), zgrid(ctx), dim3(Tile<T>::L * Line<N>::P), 0, ctx->stream, a, d,
+            hipLaunchKernelGGL((k_z<T, N, SIGN>), zgrid(ctx), dim3((Tile<T>::L * Line<T, N>::P)), 0, ctx->stream, a, d,
                                (const C *)ctx->twz);
         });
     }
@@ -293,7 +295,7 @@ struct Impl {
         const KParams kp = kparams(ctx);
         with_n(d.nz, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_hub_init<T, N>), zgrid(ctx), dim3(Tile<T>::L * Line<N>::P), 0, ctx->stream,
+            hipLaunchKernelGGL((k_hub_init<T, N>), zgrid(ctx), dim3((Tile<T>::L * Line<T, N>::P)), 0, ctx->stream,
                                H(ctx->h[0]), H(ctx->h[1]), H(ctx->ck[ctx->cur]), H(ctx->fk), kp, d, ctx->kz,
                                (const C *)ctx->twz);
         });
@@ -306,7 +308,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_cgrad_f<T, N>), xgrid<N>(d), dim3(XT), 0, ctx->stream, H(ctx->h[0]), H(ctx->h[2]),
+            hipLaunchKernelGGL((k_x_cgrad_f<T, N>), (xgrid<T, N>(d)), dim3(XT), 0, ctx->stream, H(ctx->h[0]), H(ctx->h[2]),
                                H(ctx->h[1]), R(ctx->c), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), H(ctx->h[0]),
                                fscale(ctx), d, ctx->kx, (const C *)ctx->twx);
         });
@@ -426,7 +428,7 @@ struct Impl {
             }
             if (mixed) {
                 // FP64 conv(0) = sum of conv over the grid, fixed order
-                const int nb = xblocks(d);
+                const int nb = xblocks<T>(d);
                 ProfScope ps_(ctx, "k0_sum", 8.0 * nb);
                 hipLaunchKernelGGL(k_sum_strided, dim3(1), dim3(256), 0, st, (const double *)ctx->conv_part, nb, 1,
                                    1, ctx->conv0);
@@ -440,7 +442,7 @@ struct Impl {
             ProfScope ps_(ctx, "z_hub_ch", (ns ? 8 : 7) * Hb);
             with_n(d.nz, [&](auto NC) {
                 constexpr int N = decltype(NC)::value;
-                hipLaunchKernelGGL((k_hub_ch<T, N>), zgrid(ctx), dim3(Tile<T>::L * Line<N>::P), 0, st, h0, h1,
+                hipLaunchKernelGGL((k_hub_ch<T, N>), zgrid(ctx), dim3((Tile<T>::L * Line<T, N>::P)), 0, st, h0, h1,
                                    (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]), (const C *)ctx->fk,
                                    H(ctx->hist), kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz,
                                    K0{mixed ? ctx->k0 : nullptr, ctx->conv0, ctx->cur});
@@ -463,7 +465,7 @@ struct Impl {
             ProfScope ps_(ctx, "z_hub_mu", 4 * Hb);
             with_n(d.nz, [&](auto NC) {
                 constexpr int N = decltype(NC)::value;
-                hipLaunchKernelGGL((k_hub_mu<T, N>), zgrid(ctx), dim3(Tile<T>::L * Line<N>::P), 0, st, h0,
+                hipLaunchKernelGGL((k_hub_mu<T, N>), zgrid(ctx), dim3((Tile<T>::L * Line<T, N>::P)), 0, st, h0,
                                    H(ctx->fk), (const C *)ctx->ck[ctx->cur], kp, d, ctx->kx, ctx->ky, ctx->kz,
                                    (const C *)ctx->twz);
             });
@@ -481,7 +483,7 @@ struct Impl {
                 ProfScope ps_(ctx, "x_inv_force_fwd", 4 * Hb + 4 * Rb + (parts ? 8.0 * ctx->n * sizeof(SpreadEntry<T>) : 0.0));
                 with_n(d.nx, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_x_force<T, N>), xgrid<N>(d), dim3(XT), 0, st, h0, h1, h2, (const T *)R(ctx->c),
+                    hipLaunchKernelGGL((k_x_force<T, N>), (xgrid<T, N>(d)), dim3(XT), 0, st, h0, h1, h2, (const T *)R(ctx->c),
                                        (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
                                        (const int *)ctx->line_start, (const int *)ctx->node_cnt,
                                        (const SpreadNode<T> *)ctx->spr_tmp, parts ? 1 : 0, (T)ctx->p.rho1,
@@ -496,7 +498,7 @@ struct Impl {
                 ProfScope ps_(ctx, "z_hub_stokes", 6 * Hb);
                 with_n(d.nz, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_hub_stokes<T, N>), zgrid(ctx), dim3(Tile<T>::L * Line<N>::P), 0, st, h0,
+                    hipLaunchKernelGGL((k_hub_stokes<T, N>), zgrid(ctx), dim3((Tile<T>::L * Line<T, N>::P)), 0, st, h0,
                                        h1, h2, kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
                 });
             }
@@ -624,7 +626,10 @@ int check_params(const fipi_params *p)
     {
         // the x pass takes 2*XT/P real lines per workgroup
         int lines = 0;
-        with_n(p->nx, [&](auto NC) { lines = 2 * (XT / Line<decltype(NC)::value>::P); });
+        with_n(p->nx, [&](auto NC) {
+            constexpr int n = decltype(NC)::value;
+            lines = 2 * (XT / std::min(Line<double, n>::P, Line<float, n>::P));
+        });
         if (((int64_t)p->ny * p->nz) % lines != 0)
             return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "ny*nz must be a multiple of the x-pass line count");
     }
