@@ -137,7 +137,7 @@ def main():
             "pos": sim.particles()["p"],
             "diag": sim.diagnostics(),
         }
-        if precision == prec and args.profile_steps > 0:
+        if args.profile_steps > 0:
             res["stats"] = sim.profile_steps(args.profile_steps)
         sim.close()
         del sim
@@ -150,12 +150,21 @@ def main():
     wall = main_run["wall"]
     ms_per_step = wall * 1e3 / args.steps
     value = args.steps / wall
-    stats = main_run.get("stats", [])
-    for s in stats:
-        s["avg_ms"] = s["total_ms"] / max(s["launches"], 1)
-        s["bytes_per_launch"] = s["bytes"] / max(s["launches"], 1)
-        s["gbs"] = s["bytes_per_launch"] / (s["avg_ms"] * 1e-3) / 1e9 if s["avg_ms"] > 0 else 0.0
-    stats.sort(key=lambda s: -s["total_ms"])
+    def summarize(stats):
+        for s in stats:
+            s["avg_ms"] = s["total_ms"] / max(s["launches"], 1)
+            s["bytes_per_launch"] = s["bytes"] / max(s["launches"], 1)
+            s["gbs"] = s["bytes_per_launch"] / (s["avg_ms"] * 1e-3) / 1e9 if s["avg_ms"] > 0 else 0.0
+        stats.sort(key=lambda s: -s["total_ms"])
+        return stats
+
+    def kernel_table(stats):
+        return [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()
+                 if k in ("name", "launches", "avg_ms", "gbs")} for s in stats[:12]]
+
+    stats = summarize(main_run.get("stats", []))
+    # bytes the passes move per step (sum over launch sites of their algorithmic bytes)
+    moved = sum(s["bytes"] for s in stats) / args.profile_steps if stats else None
     roofline = None
     if stats:
         dom = stats[0]
@@ -230,12 +239,14 @@ def main():
             "bytes_per_step": bstep,
             "achieved_gbs": round(bstep * value / 1e9, 1),
             "frac": round(bstep * value / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_moved_per_step": moved,
+            "moved_gbs": round(moved * value / 1e9, 1) if moved else None,
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
-        "kernels": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()
-                     if k in ("name", "launches", "avg_ms", "gbs")} for s in stats[:12]],
+        "kernels": kernel_table(stats),
+        "kernels_other_precision": kernel_table(summarize(other_run.get("stats", []))) if other_run else None,
         "diagnostics": {k: diag[k] for k in ("t", "mean_phi", "free_energy", "interface_area")},
     }
     print(json.dumps(out), flush=True)

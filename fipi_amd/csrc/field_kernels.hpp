@@ -34,9 +34,38 @@ struct Dims {
 
 constexpr int XT = 256;  // threads per x-pass workgroup
 
-// kx columns per y/z workgroup: one 128-byte line of complex values
+#ifndef FIPI_TILE_BYTES_F64
+#define FIPI_TILE_BYTES_F64 128
+#endif
+#ifndef FIPI_TILE_BYTES_F32
+#define FIPI_TILE_BYTES_F32 128
+#endif
+#ifndef FIPI_HUB_TILE_BYTES_F64
+#define FIPI_HUB_TILE_BYTES_F64 128
+#endif
+#ifndef FIPI_HUB_TILE_BYTES_F32
+#define FIPI_HUB_TILE_BYTES_F32 128
+#endif
+// kx columns per y-pass workgroup: one 128-byte line of complex values
 template <typename T> struct Tile {
-    static constexpr int L = 128 / (int)sizeof(cplx<T>);
+    static constexpr int L = (sizeof(T) == 8 ? FIPI_TILE_BYTES_F64 : FIPI_TILE_BYTES_F32) / (int)sizeof(cplx<T>);
+};
+#ifndef FIPI_STOKES_TILE_BYTES_F64
+#define FIPI_STOKES_TILE_BYTES_F64 128
+#endif
+#ifndef FIPI_STOKES_TILE_BYTES_F32
+#define FIPI_STOKES_TILE_BYTES_F32 64
+#endif
+// kx columns per z-pass (hub) workgroup
+template <typename T> struct HubTile {
+    static constexpr int L =
+        (sizeof(T) == 8 ? FIPI_HUB_TILE_BYTES_F64 : FIPI_HUB_TILE_BYTES_F32) / (int)sizeof(cplx<T>);
+};
+// kx columns per Stokes hub workgroup (three fields in flight: the FP32
+// path runs half-width tiles, which doubles the workgroups per CU)
+template <typename T> struct StokesTile {
+    static constexpr int L =
+        (sizeof(T) == 8 ? FIPI_STOKES_TILE_BYTES_F64 : FIPI_STOKES_TILE_BYTES_F32) / (int)sizeof(cplx<T>);
 };
 
 __device__ __forceinline__ size_t hidx(const Dims &d, int kx, int y, int z)
@@ -230,6 +259,63 @@ __global__ __launch_bounds__(XT) void k_x_inverse(const cplx<T> *__restrict__ in
     }
 }
 
+// P13 + next step's P1 (FP32-mixed path; the FP64 path keeps the two
+// passes, whose register footprint allows twice the occupancy): inverse x
+// of u_hat (X) -> u (real), then
+// conv = u . grad c (src/main.cpp:91, field.cpp:392-400, same sum order as
+// ProdConv) and its forward x transform into h0 (X), in place of u_hat_x.
+// g holds grad c^{n+1}, the gradient the next step's convection uses.
+// part (optional): FP64 block sums of conv, as k_x_forward.
+template <typename T, int N>
+__global__ __launch_bounds__(XT) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *__restrict__ h1,
+                                                   const cplx<T> *__restrict__ h2, T *__restrict__ u0,
+                                                   T *__restrict__ u1, T *__restrict__ u2,
+                                                   const T *__restrict__ g0, const T *__restrict__ g1,
+                                                   const T *__restrict__ g2, Dims d, const cplx<T> *__restrict__ tw,
+                                                   double *__restrict__ part)
+{
+    FIPI_XPASS_SETUP(T, N)
+    const size_t rA = (size_t)rl * d.nxp, rB = rA + d.nxp;
+    const size_t xA = (size_t)rl * d.nx, xB = xA + d.nx;
+    C v[3][E];
+    load_c2r<T, N, false>(v[0], h0, rA, rB, t, nullptr);
+    load_c2r<T, N, false>(v[1], h1, rA, rB, t, nullptr);
+    load_c2r<T, N, false>(v[2], h2, rA, rB, t, nullptr);
+    C cv[E];
+#pragma unroll
+    for (int q = 0; q < E; q++) cv[q] = C{T(0), T(0)};
+#pragma unroll
+    for (int f = 0; f < 3; f++) {
+        T *u = f == 0 ? u0 : (f == 1 ? u1 : u2);
+        const T *g = f == 0 ? g0 : (f == 1 ? g1 : g2);
+        T ga[E], gb[E];
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            const int x = t + q * P;
+            ga[q] = g[xA + x];
+            gb[q] = g[xB + x];
+        }
+        fft_dist<T, N, +1>(v[f], L, t, tw);
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            const int x = t + q * P;
+            u[xA + x] = v[f][q].x;
+            u[xB + x] = v[f][q].y;
+            cv[q] = C{cv[q].x + v[f][q].x * ga[q], cv[q].y + v[f][q].y * gb[q]};
+        }
+    }
+    if (part != nullptr) {
+        double acc = 0.0;
+#pragma unroll
+        for (int q = 0; q < E; q++) acc += (double)cv[q].x + (double)cv[q].y;
+        const double s = block_sum_f64(acc, reinterpret_cast<double *>(lds));
+        if (threadIdx.x == 0) part[blockIdx.x] = s;
+        __syncthreads();
+    }
+    fft_dist<T, N, -1>(cv, L, t, tw);
+    store_r2c<T, N>(cv, L, h0, rA, rB, t);
+}
+
 // P5: inverse x of (c_hat, dy c_hat, dz c_hat) -> c, grad c (real), then
 // f = (lambda/eps^2)(c^3 - c) and its forward x transform (src/main.cpp:153).
 template <typename T, int N>
@@ -366,12 +452,12 @@ __global__ __launch_bounds__(XT) void k_x_force(cplx<T> *__restrict__ mu_io /* i
 }
 
 // ---------------------------------------------------------------------------
-// y and z passes: a tile of Tile<T>::L neighbouring kx columns; lane = column,
+// y and z passes: a tile of TL neighbouring kx columns; lane = column,
 // t = position in the line (distribution D along y or z).
 // ---------------------------------------------------------------------------
-#define FIPI_TILE_SETUP(T, N, IDX2)                                   \
+#define FIPI_TILE_SETUP(T, N, TILE)                                   \
     constexpr int E = Line<T, N>::E, P = Line<T, N>::P, LL = Line<T, N>::LL;   \
-    constexpr int TL = Tile<T>::L;                                    \
+    constexpr int TL = TILE<T>::L;                                    \
     using C = cplx<T>;                                                \
     __shared__ C lds[TL * LL];                                        \
     const int lane = threadIdx.x % TL, t = threadIdx.x / TL;          \
@@ -408,7 +494,7 @@ __global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_y(cplx<T> *__r
                                                               Dims d, const double *__restrict__ kyv,
                                                               const cplx<T> *__restrict__ tw)
 {
-    FIPI_TILE_SETUP(T, N, 0)
+    FIPI_TILE_SETUP(T, N, Tile)
     const int z = blockIdx.y;
     const size_t base = hidx(d, ok ? kx : 0, 0, z);
     const size_t st = d.nxp;
@@ -434,9 +520,23 @@ __global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_y(cplx<T> *__r
     }
 }
 
+// 1/a for a > 0 (0 for a == 0): hardware estimate refined by two
+// Newton-Raphson steps r <- r + r(1 - a r), each of which squares the
+// relative error; the result is within one ulp of the rounded quotient.
+__device__ __forceinline__ double recip_nr(double a)
+{
+    double r = __builtin_amdgcn_rcp(a);
+    double e = fma(-a, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-a, r, 1.0);
+    r = fma(r, e, r);
+    return a != 0.0 ? r : 0.0;
+}
+
 struct KParams {
     double dt, mobility, lambda, mu;
-    double inv_g;   // 1/(nx*ny*nz), the inverse transform's normalisation
+    double inv_g;     // 1/(nx*ny*nz), the inverse transform's normalisation
+    double inv_mu_g;  // inv_g / mu
     int first;      // 1 on the first step (first-order start, main.cpp:95-114)
     int use_conv;   // switch_ns
 };
@@ -458,13 +558,13 @@ struct K0 {
 // Outputs c^{n+1} (K), the new history (K), and c^{n+1}/G, i kz c^{n+1}/G
 // inverse-z transformed (XY) for the gradient and real-space c.
 template <typename T, int N>
-__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_ch(
+__global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_ch(
     cplx<T> *__restrict__ conv_io /* in conv XY, out c XY */, cplx<T> *__restrict__ dzc_out,
     const cplx<T> *__restrict__ c_now, cplx<T> *__restrict__ c_next, const cplx<T> *__restrict__ f_now,
     cplx<T> *__restrict__ hist, KParams kp, Dims d, const double *__restrict__ kxv, const double *__restrict__ kyv,
     const double *__restrict__ kzv, const cplx<T> *__restrict__ tw, K0 k0)
 {
-    FIPI_TILE_SETUP(T, N, 0)
+    FIPI_TILE_SETUP(T, N, HubTile)
     const int y = blockIdx.y;
     const size_t base = hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
@@ -494,24 +594,25 @@ __global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_ch(
         // mode's growth factor by ~1e-7 per step, and the bias compounds
         // over the run; rounding only the result is unbiased.
         const double cnx = cn[q].x, cny = cn[q].y, fnx = fn[q].x, fny = fn[q].y, cvx = v[q].x, cvy = v[q].y;
+        // one reciprocal of the (real) divisor per point
         C nc;
         if (kp.first) {
             const double m1 = k2 * (kp.dt * kp.mobility);
-            const double div = k2 * k2 * (kp.dt * kp.mobility * kp.lambda) + 1.0;
+            const double rdiv = recip_nr(k2 * k2 * (kp.dt * kp.mobility * kp.lambda) + 1.0);
             double r = cnx - m1 * fnx, im = cny - m1 * fny;
             r -= cvx * kp.dt;
             im -= cvy * kp.dt;
-            nc = C{(T)(r / div), (T)(im / div)};
+            nc = C{(T)(r * rdiv), (T)(im * rdiv)};
         } else {
             const double m4 = k2 * (4.0 * kp.dt * kp.mobility);
             const double b4 = 4.0 * kp.dt;
-            const double div = k2 * k2 * (2.0 * kp.dt * kp.mobility * kp.lambda) + 3.0;
+            const double rdiv = recip_nr(k2 * k2 * (2.0 * kp.dt * kp.mobility * kp.lambda) + 3.0);
             double r = cnx * 4.0 - m4 * fnx, im = cny * 4.0 - m4 * fny;
             r -= cvx * b4;
             im -= cvy * b4;
             r += (double)hh[q].x;
             im += (double)hh[q].y;
-            nc = C{(T)(r / div), (T)(im / div)};
+            nc = C{(T)(r * rdiv), (T)(im * rdiv)};
         }
         const double m2 = k2 * (2.0 * kp.dt * kp.mobility);
         const double b2 = 2.0 * kp.dt;
@@ -548,14 +649,14 @@ __global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_ch(
 
 // Initial hub: (c XY, f XY) -> c_hat^0, f_hat^0 (K) and c/G, i kz c/G (XY).
 template <typename T, int N>
-__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_init(cplx<T> *__restrict__ c_io,
+__global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_init(cplx<T> *__restrict__ c_io,
                                                                      cplx<T> *__restrict__ f_io /* out: dz c XY */,
                                                                      cplx<T> *__restrict__ c_k,
                                                                      cplx<T> *__restrict__ f_k, KParams kp, Dims d,
                                                                      const double *__restrict__ kzv,
                                                                      const cplx<T> *__restrict__ tw)
 {
-    FIPI_TILE_SETUP(T, N, 0)
+    FIPI_TILE_SETUP(T, N, HubTile)
     const int y = blockIdx.y;
     const size_t base = hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
@@ -583,7 +684,7 @@ __global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_init(cplx<
 // P7: f (XY) -> f_hat^{n+1} (K); mu_hat = lambda k^2 c_hat + f_hat
 // (src/main.cpp:157) -> mu/G (XY, in place of f).
 template <typename T, int N>
-__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_mu(cplx<T> *__restrict__ f_io,
+__global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_mu(cplx<T> *__restrict__ f_io,
                                                                    cplx<T> *__restrict__ f_k,
                                                                    const cplx<T> *__restrict__ c_k, KParams kp,
                                                                    Dims d, const double *__restrict__ kxv,
@@ -591,7 +692,7 @@ __global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_mu(cplx<T>
                                                                    const double *__restrict__ kzv,
                                                                    const cplx<T> *__restrict__ tw)
 {
-    FIPI_TILE_SETUP(T, N, 0)
+    FIPI_TILE_SETUP(T, N, HubTile)
     const int y = blockIdx.y;
     const size_t base = hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
@@ -626,7 +727,7 @@ __global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_mu(cplx<T>
 // inverse of the Hermitian part, which drops the k_d k_e cross terms
 // between Nyquist and non-Nyquist components; that is what is formed here.
 template <typename T, int N>
-__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_stokes(cplx<T> *__restrict__ a0,
+__global__ __launch_bounds__((StokesTile<T>::L * Line<T, N>::P)) void k_hub_stokes(cplx<T> *__restrict__ a0,
                                                                        cplx<T> *__restrict__ a1,
                                                                        cplx<T> *__restrict__ a2, KParams kp, Dims d,
                                                                        const double *__restrict__ kxv,
@@ -634,7 +735,7 @@ __global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_stokes(cpl
                                                                        const double *__restrict__ kzv,
                                                                        const cplx<T> *__restrict__ tw)
 {
-    FIPI_TILE_SETUP(T, N, 0)
+    FIPI_TILE_SETUP(T, N, StokesTile)
     const int y = blockIdx.y;
     const size_t base = hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
@@ -661,21 +762,32 @@ __global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_stokes(cpl
         const bool nq[3] = {nx_, ny_, nz_};
         const double F[3][2] = {{v[0][q].x, v[0][q].y}, {v[1][q].x, v[1][q].y}, {v[2][q].x, v[2][q].y}};
         // k.F split by Nyquist class (FP64 arithmetic on the stored values)
+        // (selects, not branches: adding an exact zero leaves a sum unchanged)
         double sar = 0, sai = 0, sbr = 0, sbi = 0;
 #pragma unroll
         for (int e = 0; e < 3; e++) {
-            if (nq[e]) { sbr += kv[e] * F[e][0]; sbi += kv[e] * F[e][1]; }
-            else { sar += kv[e] * F[e][0]; sai += kv[e] * F[e][1]; }
+            const double tr = kv[e] * F[e][0], ti = kv[e] * F[e][1];
+            sar += nq[e] ? 0.0 : tr;
+            sai += nq[e] ? 0.0 : ti;
+            sbr += nq[e] ? tr : 0.0;
+            sbi += nq[e] ? ti : 0.0;
         }
-        const double vis = k2 * kp.mu;
+        // one reciprocal per point; the velocity factor is 1/(mu k^2 G).  At
+        // k = 0 the projection and the velocity are zero (division by zero
+        // gives zero).  The branch also keeps one point's FP64 temporaries
+        // live at a time.
+        if (k2 != 0.0) {
+            const double ik2 = 1.0 / k2;
+            const double cu = ik2 * kp.inv_mu_g;
 #pragma unroll
-        for (int e = 0; e < 3; e++) {
-            const double sr = nq[e] ? sbr : sar, si = nq[e] ? sbi : sai;
-            double pr = kv[e] * sr, pi_ = kv[e] * si;
-            if (k2 != 0.0) { pr /= k2; pi_ /= k2; } else { pr = 0.0; pi_ = 0.0; }
-            double ur = F[e][0] - pr, ui = F[e][1] - pi_;
-            if (vis != 0.0) { ur /= vis; ui /= vis; } else { ur = 0.0; ui = 0.0; }
-            v[e][q] = C{(T)(ur * kp.inv_g), (T)(ui * kp.inv_g)};
+            for (int e = 0; e < 3; e++) {
+                const double sr = nq[e] ? sbr : sar, si = nq[e] ? sbi : sai;
+                const double pr = kv[e] * sr * ik2, pi_ = kv[e] * si * ik2;
+                v[e][q] = C{(T)((F[e][0] - pr) * cu), (T)((F[e][1] - pi_) * cu)};
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 3; e++) v[e][q] = C{T(0), T(0)};
         }
     }
     fft_dist<T, N, +1, TL>(v[0], L, t, tw);
@@ -688,10 +800,10 @@ __global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_hub_stokes(cpl
 
 // Plain forward / inverse z transform of one field in place (unit entries).
 template <typename T, int N, int SIGN>
-__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_z(cplx<T> *__restrict__ a, Dims d,
+__global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_z(cplx<T> *__restrict__ a, Dims d,
                                                               const cplx<T> *__restrict__ tw)
 {
-    FIPI_TILE_SETUP(T, N, 0)
+    FIPI_TILE_SETUP(T, N, HubTile)
     const int y = blockIdx.y;
     const size_t base = hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;

@@ -68,6 +68,9 @@ struct fipi_ctx {
     void *h[3] = {};
     void *ck[2] = {}, *fk = nullptr, *hist = nullptr;
     int cur = 0;
+    // h[0] already holds the next step's conv (X) and conv_part its FP64
+    // block sums: the previous step's last pass formed u.grad c
+    bool conv_ready = false;
     double *kx = nullptr, *ky = nullptr, *kz = nullptr;
     void *twx = nullptr, *twy = nullptr, *twz = nullptr;
     // particles
@@ -228,6 +231,7 @@ struct Impl {
         kp.lambda = ctx->p.lambda;
         kp.mu = ctx->p.mu;
         kp.inv_g = 1.0 / (double)((int64_t)ctx->p.nx * ctx->p.ny * ctx->p.nz);
+        kp.inv_mu_g = ctx->p.mu != 0.0 ? kp.inv_g / ctx->p.mu : 0.0;  // u = 0 when mu = 0 (division by zero gives zero)
         kp.first = ctx->t == 0;
         kp.use_conv = ctx->p.switch_ns != 0;
         return kp;
@@ -269,9 +273,10 @@ struct Impl {
         });
     }
 
+    template <template <typename> class TileT = HubTile>
     static dim3 zgrid(const fipi_ctx *ctx)
     {
-        return dim3((unsigned)((ctx->d.nxh + Tile<T>::L - 1) / Tile<T>::L), ctx->d.ny);
+        return dim3((unsigned)((ctx->d.nxh + TileT<T>::L - 1) / TileT<T>::L), ctx->d.ny);
     }
 
     template <int SIGN>
@@ -280,7 +285,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nz, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_z<T, N, SIGN>), zgrid(ctx), dim3((Tile<T>::L * Line<T, N>::P)), 0, ctx->stream, a, d,
+            hipLaunchKernelGGL((k_z<T, N, SIGN>), zgrid(ctx), dim3((HubTile<T>::L * Line<T, N>::P)), 0, ctx->stream, a, d,
                                (const C *)ctx->twz);
         });
     }
@@ -295,7 +300,7 @@ struct Impl {
         const KParams kp = kparams(ctx);
         with_n(d.nz, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_hub_init<T, N>), zgrid(ctx), dim3((Tile<T>::L * Line<T, N>::P)), 0, ctx->stream,
+            hipLaunchKernelGGL((k_hub_init<T, N>), zgrid(ctx), dim3((HubTile<T>::L * Line<T, N>::P)), 0, ctx->stream,
                                H(ctx->h[0]), H(ctx->h[1]), H(ctx->ck[ctx->cur]), H(ctx->fk), kp, d, ctx->kz,
                                (const C *)ctx->twz);
         });
@@ -421,11 +426,12 @@ struct Impl {
         const double Rb = (double)ctx->G * sizeof(T), Hb = (double)ctx->GH * 2 * sizeof(T);
         const bool mixed = !std::is_same<T, double>::value;
         if (ns) {
-            ProdConv<T> pr{R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2])};
-            {
+            if (!ctx->conv_ready) {
+                ProdConv<T> pr{R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2])};
                 ProfScope ps_(ctx, "x_fwd_conv", 6 * Rb + Hb);
                 x_forward<1>(ctx, pr, h0, nullptr, mixed ? ctx->conv_part : nullptr);
             }
+            ctx->conv_ready = false;
             if (mixed) {
                 // FP64 conv(0) = sum of conv over the grid, fixed order
                 const int nb = xblocks<T>(d);
@@ -442,7 +448,7 @@ struct Impl {
             ProfScope ps_(ctx, "z_hub_ch", (ns ? 8 : 7) * Hb);
             with_n(d.nz, [&](auto NC) {
                 constexpr int N = decltype(NC)::value;
-                hipLaunchKernelGGL((k_hub_ch<T, N>), zgrid(ctx), dim3((Tile<T>::L * Line<T, N>::P)), 0, st, h0, h1,
+                hipLaunchKernelGGL((k_hub_ch<T, N>), zgrid(ctx), dim3((HubTile<T>::L * Line<T, N>::P)), 0, st, h0, h1,
                                    (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]), (const C *)ctx->fk,
                                    H(ctx->hist), kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz,
                                    K0{mixed ? ctx->k0 : nullptr, ctx->conv0, ctx->cur});
@@ -465,7 +471,7 @@ struct Impl {
             ProfScope ps_(ctx, "z_hub_mu", 4 * Hb);
             with_n(d.nz, [&](auto NC) {
                 constexpr int N = decltype(NC)::value;
-                hipLaunchKernelGGL((k_hub_mu<T, N>), zgrid(ctx), dim3((Tile<T>::L * Line<T, N>::P)), 0, st, h0,
+                hipLaunchKernelGGL((k_hub_mu<T, N>), zgrid(ctx), dim3((HubTile<T>::L * Line<T, N>::P)), 0, st, h0,
                                    H(ctx->fk), (const C *)ctx->ck[ctx->cur], kp, d, ctx->kx, ctx->ky, ctx->kz,
                                    (const C *)ctx->twz);
             });
@@ -498,7 +504,7 @@ struct Impl {
                 ProfScope ps_(ctx, "z_hub_stokes", 6 * Hb);
                 with_n(d.nz, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_hub_stokes<T, N>), zgrid(ctx), dim3((Tile<T>::L * Line<T, N>::P)), 0, st, h0,
+                    hipLaunchKernelGGL((k_hub_stokes<T, N>), zgrid<StokesTile>(ctx), dim3((StokesTile<T>::L * Line<T, N>::P)), 0, st, h0,
                                        h1, h2, kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
                 });
             }
@@ -506,9 +512,20 @@ struct Impl {
                 ProfScope ps_(ctx, "y_inv_3", 6 * Hb);
                 y_pass<+1, 3, false>(ctx, h0, h1, h2, nullptr);
             }
-            {
+            if constexpr (std::is_same<T, double>::value) {
                 ProfScope ps_(ctx, "x_inv_u", 3 * Hb + 3 * Rb);
                 x_inverse<3>(ctx, h0, h1, h2, R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]));
+            } else {
+                // u^{n+1} and, for the next step, conv = u^{n+1}.grad c^{n+1} (X) in h[0]
+                ProfScope ps_(ctx, "x_inv_u_conv", 4 * Hb + 6 * Rb);
+                with_n(d.nx, [&](auto NC) {
+                    constexpr int N = decltype(NC)::value;
+                    hipLaunchKernelGGL((k_x_inv_conv<T, N>), (xgrid<T, N>(d)), dim3(XT), 0, st, h0, (const C *)h1,
+                                       (const C *)h2, R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]),
+                                       (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]), d,
+                                       (const C *)ctx->twx, mixed ? ctx->conv_part : nullptr);
+                });
+                ctx->conv_ready = true;
             }
         }
         if (parts) {
@@ -919,6 +936,7 @@ int fipi_set_state(fipi_ctx *ctx, const double *phi, const double *pos, const do
     HIPCHK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4, ctx->stream));
     ctx->t = 0;
     ctx->cur = 0;
+    ctx->conv_ready = false;
     {
         // FP64 k = 0 mode: c_hat(0) = sum of phi (used by the FP32-mixed path)
         long double s = 0.0L;
@@ -980,6 +998,7 @@ int fipi_set_state(fipi_ctx *ctx, const double *phi, const double *pos, const do
 int fipi_set_velocity(fipi_ctx *ctx, const double *u)
 {
     if (!ctx || !u) return set_err(ctx, FIPI_ERR_ARG, "ctx and u are required");
+    ctx->conv_ready = false;
     for (int a = 0; a < 3; a++) {
         int rc = copy_in_real(ctx, ctx->u[a], u + (size_t)a * ctx->G, ctx->G);
         if (rc) return rc;
@@ -997,6 +1016,7 @@ int fipi_get_phi(fipi_ctx *ctx, double *phi)
 int fipi_get_velocity(fipi_ctx *ctx, double *u)
 {
     if (!ctx || !u) return set_err(ctx, FIPI_ERR_ARG, "ctx and u are required");
+    ctx->conv_ready = false;
     for (int a = 0; a < 3; a++) {
         int rc = copy_out_real(ctx, u + (size_t)a * ctx->G, ctx->u[a], ctx->G);
         if (rc) return rc;
