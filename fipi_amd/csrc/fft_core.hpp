@@ -109,9 +109,57 @@ template <int SIGN, typename C> __device__ __forceinline__ void dft16(C *v)
     }
 }
 
+// cos/sin(2 pi k / R) for the odd-radix stages (closed forms: sqrt(3)/2,
+// (sqrt(5) - 1)/4, sqrt(10 + 2 sqrt(5))/4, -(sqrt(5) + 1)/4,
+// sqrt(10 - 2 sqrt(5))/4), k = 0..R-1
+template <int R> struct WR;
+template <> struct WR<3> {
+    static constexpr double c[3] = {1.0, -0.5, -0.5};
+    static constexpr double s[3] = {0.0, 0.86602540378443864676372317075293618347,
+                                    -0.86602540378443864676372317075293618347};
+};
+template <> struct WR<5> {
+    static constexpr double c[5] = {1.0, 0.30901699437494742410229341718281905886,
+                                    -0.80901699437494742410229341718281905886,
+                                    -0.80901699437494742410229341718281905886,
+                                    0.30901699437494742410229341718281905886};
+    static constexpr double s[5] = {0.0, 0.95105651629515357211643933337938214341,
+                                    0.58778525229247312916870595463907276860,
+                                    -0.58778525229247312916870595463907276860,
+                                    -0.95105651629515357211643933337938214341};
+};
+template <> struct WR<6> {
+    static constexpr double c[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+    static constexpr double s[6] = {0.0, 0.86602540378443864676372317075293618347,
+                                    0.86602540378443864676372317075293618347, 0.0,
+                                    -0.86602540378443864676372317075293618347,
+                                    -0.86602540378443864676372317075293618347};
+};
+
+// direct DFT of radix R: out[m] = sum_q v[q] W_R^(SIGN q m)
+template <int R, int SIGN, typename C> __device__ __forceinline__ void dft_direct(C *v)
+{
+    using T = decltype(v[0].x);
+    C o[R];
+#pragma unroll
+    for (int m = 0; m < R; m++) {
+        C a = v[0];
+#pragma unroll
+        for (int q = 1; q < R; q++) {
+            const int k = (q * m) % R;
+            a = cadd(a, k == 0 ? v[q] : mul_cs<SIGN>(v[q], (T)WR<R>::c[k], (T)WR<R>::s[k]));
+        }
+        o[m] = a;
+    }
+#pragma unroll
+    for (int m = 0; m < R; m++) v[m] = o[m];
+}
+
 template <int R, int SIGN, typename C> __device__ __forceinline__ void dft(C *v)
 {
-    if constexpr (R == 1) {
+    if constexpr (R == 3 || R == 5 || R == 6) {
+        dft_direct<R, SIGN>(v);
+    } else if constexpr (R == 1) {
     } else if constexpr (R == 2) {
         dft2<SIGN>(v[0], v[1]);
     } else if constexpr (R == 4) {
@@ -141,15 +189,20 @@ template <int R, int SIGN, typename C> __device__ __forceinline__ void dft(C *v)
 // put their S = 128-byte-wide columns side by side, so a wave's accesses
 // are runs of S consecutive values).
 //
-// Plans (E.M.E): 256 = 8.4.8 or 16.16, 128 = 8.2.8, 64 = 8.8, 32 = 4.2.4,
-// 16 = 4.4.  FP64 lines of 256 use E = 8 (fewer registers for kernels that
-// keep three fields in flight), FP32 lines of 256 use E = 16 (one LDS
-// exchange instead of two).
+// Plans (E.M.E): 256 = 8.4.8 or 16.16, 128 = 8.2.8, 96 = 4.6.4,
+// 80 = 4.5.4, 64 = 8.8, 48 = 4.3.4, 32 = 4.2.4, 16 = 4.4.  When M does not
+// divide E the N/M butterflies of the middle stage are spread over the P
+// threads unevenly (some threads take one more).  Lines of 256 use E = 8
+// by default (fewer registers for kernels that keep three fields in
+// flight; E = 16 would need one LDS exchange instead of two).
 // ---------------------------------------------------------------------------
 template <int N, int E> struct PlanE;
 template <> struct PlanE<16, 4> { static constexpr int M = 0; };
 template <> struct PlanE<32, 4> { static constexpr int M = 2; };
+template <> struct PlanE<48, 4> { static constexpr int M = 3; };
 template <> struct PlanE<64, 8> { static constexpr int M = 0; };
+template <> struct PlanE<80, 4> { static constexpr int M = 5; };
+template <> struct PlanE<96, 4> { static constexpr int M = 6; };
 template <> struct PlanE<128, 8> { static constexpr int M = 2; };
 template <> struct PlanE<256, 8> { static constexpr int M = 4; };
 template <> struct PlanE<256, 16> { static constexpr int M = 0; };
@@ -162,7 +215,7 @@ template <> struct PlanE<256, 16> { static constexpr int M = 0; };
 #endif
 
 template <typename T, int N> struct PlanE_for {
-    static constexpr int E = N <= 32 ? 4 : 8;
+    static constexpr int E = (N == 64 || N == 128) ? 8 : 4;
 };
 template <> struct PlanE_for<double, 256> { static constexpr int E = FIPI_E256_F64; };
 template <> struct PlanE_for<float, 256> { static constexpr int E = FIPI_E256_F32; };
@@ -188,26 +241,33 @@ __device__ __forceinline__ C tw_at(const C *__restrict__ tw, int idx)
 template <typename T, int N, int SIGN, int S>
 __device__ __forceinline__ void mid_stage(cplx<T> *lds, int t, const cplx<T> *__restrict__ tw)
 {
-    constexpr int E = Line<T, N>::E, P = Line<T, N>::P, RM = Line<T, N>::M, NS = E, NB = E / RM;
+    constexpr int E = Line<T, N>::E, P = Line<T, N>::P, RM = Line<T, N>::M, NS = E;
+    constexpr int B = N / RM;               // butterflies of the stage
+    constexpr int NB = (B + P - 1) / P;     // per thread (the last round may be partial)
     using C = cplx<T>;
-    C u[E];
+    C u[NB * RM];
 #pragma unroll
     for (int b = 0; b < NB; b++) {
         const int j = t + b * P;
+        if (B % P == 0 || j < B) {
 #pragma unroll
-        for (int q = 0; q < RM; q++) u[b * RM + q] = lds[lpad<E>(j + q * (N / RM)) * S];
+            for (int q = 0; q < RM; q++) u[b * RM + q] = lds[lpad<E>(j + q * (N / RM)) * S];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < NB; b++) {
         const int j = t + b * P;
-        const int k = j % NS;
+        if (B % P == 0 || j < B) {
+            const int k = j % NS;
 #pragma unroll
-        for (int q = 1; q < RM; q++) u[b * RM + q] = cmul(u[b * RM + q], tw_at<SIGN>(tw, k * q * (N / (NS * RM))));
-        dft<RM, SIGN>(u + b * RM);
-        const int base = (j / NS) * NS * RM + k;
+            for (int q = 1; q < RM; q++)
+                u[b * RM + q] = cmul(u[b * RM + q], tw_at<SIGN>(tw, k * q * (N / (NS * RM))));
+            dft<RM, SIGN>(u + b * RM);
+            const int base = (j / NS) * NS * RM + k;
 #pragma unroll
-        for (int q = 0; q < RM; q++) lds[lpad<E>(base + q * NS) * S] = u[b * RM + q];
+            for (int q = 0; q < RM; q++) lds[lpad<E>(base + q * NS) * S] = u[b * RM + q];
+        }
     }
     __syncthreads();
 }

@@ -28,11 +28,21 @@ namespace fipi {
 // nxh = nx/2 + 1 spectral columns; nxp >= nxh is the row pitch of every
 // half-spectrum array, a multiple of 128 bytes so that the 16-column tiles
 // of the y and z passes start on cache-line boundaries.
+// A context holds a slab of equal boxes: box b's arrays start b*rs (real)
+// and b*hs (half spectrum) elements after box 0's; every field kernel takes
+// its box from blockIdx.z.
 struct Dims {
     int nx, ny, nz, nxh, nxp;
+    size_t rs, hs;
 };
 
-constexpr int XT = 256;  // threads per x-pass workgroup
+constexpr int XT = 256;  // threads per x-pass workgroup (at most)
+
+// x-pass workgroup shape: LPW complex lines (2 real lines each) of P threads
+template <typename T, int N> struct XShape {
+    static constexpr int LPW = XT / Line<T, N>::P;
+    static constexpr int THREADS = LPW * Line<T, N>::P;
+};
 
 #ifndef FIPI_TILE_BYTES_F64
 #define FIPI_TILE_BYTES_F64 128
@@ -110,14 +120,14 @@ __device__ __forceinline__ void load_c2r(cplx<T> *v, const cplx<T> *__restrict__
 // rows rowA, rowB.
 template <typename T, int N>
 __device__ __forceinline__ void store_r2c(const cplx<T> *v, cplx<T> *L, cplx<T> *__restrict__ out, size_t rowA,
-                                          size_t rowB, int t)
+                                          size_t rowB, int t, bool lok)
 {
     constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
     using C = cplx<T>;
 #pragma unroll
     for (int q = 0; q < E; q++) L[t + q * P] = v[q];
     __syncthreads();
-    for (int k = t; k <= N / 2; k += P) {
+    for (int k = t; lok && k <= N / 2; k += P) {
         const C zk = L[k], zm = L[(N - k) % N];
         const C A{(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
         const C B{(zk.y + zm.y) * T(0.5), (zm.x - zk.x) * T(0.5)};
@@ -127,16 +137,30 @@ __device__ __forceinline__ void store_r2c(const cplx<T> *v, cplx<T> *L, cplx<T> 
     __syncthreads();
 }
 
-// common x-pass geometry: complex line lp of the workgroup = real lines rl, rl+1
-#define FIPI_XPASS_SETUP(T, N)                                      \
+// common x-pass geometry: complex line lp of the workgroup = real lines
+// rl, rl+1 of box blockIdx.z.  The last workgroup may be partial: its
+// surplus lines (lok false) load line 0, join every barrier and store
+// nothing.
+#define FIPI_XPASS_SETUP(T, N)                                               \
     constexpr int E = Line<T, N>::E, P = Line<T, N>::P, LL = Line<T, N>::LL; \
-    constexpr int LPW = XT / P;                                     \
-    using C = cplx<T>;                                              \
-    __shared__ C lds[LPW * LL];                                     \
-    const int lp = threadIdx.x / P, t = threadIdx.x % P;            \
-    const int rl = (blockIdx.x * LPW + lp) * 2;                     \
-    C *L = lds + lp * LL;                                           \
-    (void)E;
+    constexpr int LPW = XShape<T, N>::LPW;                                   \
+    using C = cplx<T>;                                                       \
+    __shared__ C lds[LPW * LL];                                              \
+    const int lp = threadIdx.x / P, t = threadIdx.x % P;                     \
+    const int rl0 = (blockIdx.x * LPW + lp) * 2;                             \
+    const bool lok = rl0 < d.ny * d.nz;                                      \
+    const int rl = lok ? rl0 : 0;                                            \
+    const size_t zb = blockIdx.z;                                            \
+    C *L = lds + lp * LL;                                                    \
+    (void)E;                                                                 \
+    (void)zb;
+
+// pointer of box zb in a slab with per-box stride st (null stays null)
+template <typename P_>
+__device__ __forceinline__ P_ *boxp(P_ *p, size_t zb, size_t st)
+{
+    return p ? p + zb * st : p;
+}
 
 // ---------------------------------------------------------------------------
 // P1: conv = u . grad c in real space (src/main.cpp:91, field.cpp:392-400),
@@ -175,23 +199,28 @@ struct ProdCF {
     }
 };
 
-// Fixed-order FP64 sum of one value per thread over the block; the result
-// is valid in thread 0.  `red` holds blockDim.x doubles.
+// Fixed-order FP64 sum of one value per thread over the block (at most 256
+// threads, at least 128); the result is valid in thread 0.  `red` holds 256
+// doubles.
 __device__ __forceinline__ double block_sum_f64(double v, double *red)
 {
     red[threadIdx.x] = v;
+    if ((int)threadIdx.x < 256 - (int)blockDim.x) red[blockDim.x + threadIdx.x] = 0.0;
     __syncthreads();
-    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
     return red[0];
 }
 
-// Fixed-order FP64 sums: out[q] = sum_i part[i*stride + q], q < nq (one block).
+// Fixed-order FP64 sums: out[q] = sum_i part[i*stride + q], q < nq; one
+// block per box (box b: part + b*nb*stride, out + b*nq).
 __global__ __launch_bounds__(256) void k_sum_strided(const double *part, int nb, int stride, int nq, double *out)
 {
     __shared__ double s[256];
+    part += (size_t)blockIdx.x * nb * stride;
+    out += (size_t)blockIdx.x * nq;
     for (int q = 0; q < nq; q++) {
         double a = 0;
         for (int i = threadIdx.x; i < nb; i += 256) a += part[(size_t)i * stride + q];
@@ -220,16 +249,17 @@ __global__ __launch_bounds__(XT) void k_x_forward(Prod prod, cplx<T> *__restrict
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int x = t + q * P;
-            v[q] = C{prod(f, (size_t)rl * d.nx + x), prod(f, (size_t)(rl + 1) * d.nx + x)};
-            acc += (double)v[q].x + (double)v[q].y;
+            v[q] = C{prod(f, zb * d.rs + (size_t)rl * d.nx + x), prod(f, zb * d.rs + (size_t)(rl + 1) * d.nx + x)};
+            if (lok) acc += (double)v[q].x + (double)v[q].y;
         }
         if (f == 0 && part != nullptr) {
             const double s = block_sum_f64(acc, reinterpret_cast<double *>(lds));
-            if (threadIdx.x == 0) part[blockIdx.x] = s;
+            if (threadIdx.x == 0) part[zb * gridDim.x + blockIdx.x] = s;
             __syncthreads();
         }
         fft_dist<T, N, -1>(v, L, t, tw);
-        store_r2c<T, N>(v, L, f == 0 ? out0 : out1, (size_t)rl * d.nxp, (size_t)(rl + 1) * d.nxp, t);
+        store_r2c<T, N>(v, L, boxp(f == 0 ? out0 : out1, zb, d.hs), (size_t)rl * d.nxp, (size_t)(rl + 1) * d.nxp,
+                        t, lok);
     }
 }
 
@@ -241,8 +271,8 @@ __global__ __launch_bounds__(XT) void k_x_inverse(const cplx<T> *__restrict__ in
                                                   const cplx<T> *__restrict__ tw)
 {
     FIPI_XPASS_SETUP(T, N)
-    const size_t rA = (size_t)rl * d.nxp, rB = rA + d.nxp;
-    const size_t xA = (size_t)rl * d.nx, xB = xA + d.nx;
+    const size_t rA = zb * d.hs + (size_t)rl * d.nxp, rB = rA + d.nxp;
+    const size_t xA = zb * d.rs + (size_t)rl * d.nx, xB = xA + d.nx;
     C v[NF][E];
 #pragma unroll
     for (int f = 0; f < NF; f++) load_c2r<T, N, false>(v[f], f == 0 ? in0 : (f == 1 ? in1 : in2), rA, rB, t, nullptr);
@@ -250,11 +280,13 @@ __global__ __launch_bounds__(XT) void k_x_inverse(const cplx<T> *__restrict__ in
     for (int f = 0; f < NF; f++) {
         T *o = f == 0 ? o0 : (f == 1 ? o1 : o2);
         fft_dist<T, N, +1>(v[f], L, t, tw);
+        if (lok) {
 #pragma unroll
-        for (int q = 0; q < E; q++) {
-            const int x = t + q * P;
-            o[xA + x] = v[f][q].x;
-            o[xB + x] = v[f][q].y;
+            for (int q = 0; q < E; q++) {
+                const int x = t + q * P;
+                o[xA + x] = v[f][q].x;
+                o[xB + x] = v[f][q].y;
+            }
         }
     }
 }
@@ -275,8 +307,8 @@ __global__ __launch_bounds__(XT) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *_
                                                    double *__restrict__ part)
 {
     FIPI_XPASS_SETUP(T, N)
-    const size_t rA = (size_t)rl * d.nxp, rB = rA + d.nxp;
-    const size_t xA = (size_t)rl * d.nx, xB = xA + d.nx;
+    const size_t rA = zb * d.hs + (size_t)rl * d.nxp, rB = rA + d.nxp;
+    const size_t xA = zb * d.rs + (size_t)rl * d.nx, xB = xA + d.nx;
     C v[3][E];
     load_c2r<T, N, false>(v[0], h0, rA, rB, t, nullptr);
     load_c2r<T, N, false>(v[1], h1, rA, rB, t, nullptr);
@@ -299,8 +331,10 @@ __global__ __launch_bounds__(XT) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *_
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int x = t + q * P;
-            u[xA + x] = v[f][q].x;
-            u[xB + x] = v[f][q].y;
+            if (lok) {
+                u[xA + x] = v[f][q].x;
+                u[xB + x] = v[f][q].y;
+            }
             cv[q] = C{cv[q].x + v[f][q].x * ga[q], cv[q].y + v[f][q].y * gb[q]};
         }
     }
@@ -308,12 +342,12 @@ __global__ __launch_bounds__(XT) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *_
         double acc = 0.0;
 #pragma unroll
         for (int q = 0; q < E; q++) acc += (double)cv[q].x + (double)cv[q].y;
-        const double s = block_sum_f64(acc, reinterpret_cast<double *>(lds));
-        if (threadIdx.x == 0) part[blockIdx.x] = s;
+        const double s = block_sum_f64(lok ? acc : 0.0, reinterpret_cast<double *>(lds));
+        if (threadIdx.x == 0) part[zb * gridDim.x + blockIdx.x] = s;
         __syncthreads();
     }
     fft_dist<T, N, -1>(cv, L, t, tw);
-    store_r2c<T, N>(cv, L, h0, rA, rB, t);
+    store_r2c<T, N>(cv, L, h0, rA, rB, t, lok);
 }
 
 // P5: inverse x of (c_hat, dy c_hat, dz c_hat) -> c, grad c (real), then
@@ -326,8 +360,8 @@ __global__ __launch_bounds__(XT) void k_x_cgrad_f(const cplx<T> *__restrict__ ch
                                                   const double *__restrict__ kxv, const cplx<T> *__restrict__ tw)
 {
     FIPI_XPASS_SETUP(T, N)
-    const size_t rA = (size_t)rl * d.nxp, rB = rA + d.nxp;
-    const size_t xA = (size_t)rl * d.nx, xB = xA + d.nx;
+    const size_t rA = zb * d.hs + (size_t)rl * d.nxp, rB = rA + d.nxp;
+    const size_t xA = zb * d.rs + (size_t)rl * d.nx, xB = xA + d.nx;
     // all inputs in flight first
     C cv[E], v[3][E];
     load_c2r<T, N, false>(cv, ch, rA, rB, t, kxv);
@@ -335,21 +369,25 @@ __global__ __launch_bounds__(XT) void k_x_cgrad_f(const cplx<T> *__restrict__ ch
     load_c2r<T, N, false>(v[1], dyc, rA, rB, t, kxv);
     load_c2r<T, N, false>(v[2], dzc, rA, rB, t, kxv);
     fft_dist<T, N, +1>(cv, L, t, tw);
+    if (lok) {
 #pragma unroll
-    for (int q = 0; q < E; q++) {
-        const int x = t + q * P;
-        c[xA + x] = cv[q].x;
-        c[xB + x] = cv[q].y;
+        for (int q = 0; q < E; q++) {
+            const int x = t + q * P;
+            c[xA + x] = cv[q].x;
+            c[xB + x] = cv[q].y;
+        }
     }
 #pragma unroll
     for (int f = 0; f < 3; f++) {
         T *g = f == 0 ? gx : (f == 1 ? gy : gz);
         fft_dist<T, N, +1>(v[f], L, t, tw);
+        if (lok) {
 #pragma unroll
-        for (int q = 0; q < E; q++) {
-            const int x = t + q * P;
-            g[xA + x] = v[f][q].x;
-            g[xB + x] = v[f][q].y;
+            for (int q = 0; q < E; q++) {
+                const int x = t + q * P;
+                g[xA + x] = v[f][q].x;
+                g[xB + x] = v[f][q].y;
+            }
         }
     }
 #pragma unroll
@@ -358,7 +396,7 @@ __global__ __launch_bounds__(XT) void k_x_cgrad_f(const cplx<T> *__restrict__ ch
         cv[q] = C{(a * a * a - a) * fscale, (b * b * b - b) * fscale};
     }
     fft_dist<T, N, -1>(cv, L, t, tw);
-    store_r2c<T, N>(cv, L, fout, rA, rB, t);
+    store_r2c<T, N>(cv, L, fout, rA, rB, t, lok);
 }
 
 template <typename T>
@@ -393,8 +431,11 @@ __global__ __launch_bounds__(XT) void k_x_force(cplx<T> *__restrict__ mu_io /* i
                                                 T rho2, T gravity, Dims d, const cplx<T> *__restrict__ tw)
 {
     FIPI_XPASS_SETUP(T, N)
-    const size_t rA = (size_t)rl * d.nxp, rB = rA + d.nxp;
-    const size_t xA = (size_t)rl * d.nx, xB = xA + d.nx;
+    const size_t rA = zb * d.hs + (size_t)rl * d.nxp, rB = rA + d.nxp;
+    const size_t xA = zb * d.rs + (size_t)rl * d.nx, xB = xA + d.nx;
+    // spread CSR over the lines of all boxes: box zb's lines follow box 0's
+    line_start = boxp(line_start, zb, (size_t)(d.ny * d.nz));
+    node_cnt = boxp(node_cnt, zb, (size_t)(d.ny * d.nz));
     C muv[E];
     T ga[3][E], gb[3][E], ca[E], cb[E];
     load_c2r<T, N, false>(muv, mu_io, rA, rB, t, nullptr);
@@ -447,7 +488,7 @@ __global__ __launch_bounds__(XT) void k_x_force(cplx<T> *__restrict__ mu_io /* i
             }
         }
         fft_dist<T, N, -1>(v, L, t, tw);
-        store_r2c<T, N>(v, L, f == 0 ? mu_io : (f == 1 ? f1 : f2), rA, rB, t);
+        store_r2c<T, N>(v, L, f == 0 ? mu_io : (f == 1 ? f1 : f2), rA, rB, t, lok);
     }
 }
 
@@ -463,6 +504,7 @@ __global__ __launch_bounds__(XT) void k_x_force(cplx<T> *__restrict__ mu_io /* i
     const int lane = threadIdx.x % TL, t = threadIdx.x / TL;          \
     const int kx = blockIdx.x * TL + lane;                            \
     const bool ok = kx < d.nxh;                                       \
+    const size_t zb = blockIdx.z;                                     \
     C *L = lds + lane;                                                \
     (void)E; (void)P;
 
@@ -496,7 +538,7 @@ __global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_y(cplx<T> *__r
 {
     FIPI_TILE_SETUP(T, N, Tile)
     const int z = blockIdx.y;
-    const size_t base = hidx(d, ok ? kx : 0, 0, z);
+    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, 0, z);
     const size_t st = d.nxp;
     C v[NF][E];
 #pragma unroll
@@ -544,6 +586,8 @@ struct KParams {
 // FP64 k = 0 mode (FP32-mixed path): k0[cur] = c_hat^n(0), k0[cur^1] gets
 // c_hat^{n+1}(0), k0[2] = history(0); conv0 = FP64 sum of conv (its k = 0
 // transform).  k0 == nullptr: the k = 0 mode is carried like every other.
+// Box b's values are at k0 + K0_STRIDE*b and conv0 + b.
+constexpr int K0_STRIDE = 4;
 struct K0 {
     double *k0;
     const double *conv0;
@@ -566,7 +610,7 @@ __global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_ch(
 {
     FIPI_TILE_SETUP(T, N, HubTile)
     const int y = blockIdx.y;
-    const size_t base = hidx(d, ok ? kx : 0, y, 0);
+    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
     C v[E], cn[E], fn[E], hh[E];
     if (kp.use_conv) tile_load<T, N>(v, conv_io, base, st, t, ok);
@@ -620,15 +664,16 @@ __global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_ch(
         const T ig = (T)kp.inv_g;
         if (k0.k0 != nullptr && ok && kx == 0 && y == 0 && z == 0) {
             // mass mode in FP64: k^2 = 0, so only c, conv and history enter
-            const double c0 = k0.k0[k0.cur], h0 = k0.k0[2];
-            const double cv0 = kp.use_conv ? *k0.conv0 : 0.0;
+            double *kb = k0.k0 + K0_STRIDE * zb;
+            const double c0 = kb[k0.cur], h0 = kb[2];
+            const double cv0 = kp.use_conv ? k0.conv0[zb] : 0.0;
             double n0;
             if (kp.first) n0 = c0 - cv0 * kp.dt;
             else n0 = (c0 * 4.0 - cv0 * (4.0 * kp.dt) + h0) / 3.0;
-            k0.k0[k0.cur ^ 1] = n0;
-            k0.k0[2] = -c0 + cv0 * (2.0 * kp.dt);
+            kb[k0.cur ^ 1] = n0;
+            kb[2] = -c0 + cv0 * (2.0 * kp.dt);
             nc = C{(T)n0, T(0)};
-            nh = C{(T)k0.k0[2], T(0)};
+            nh = C{(T)kb[2], T(0)};
             v[q] = C{(T)(n0 * kp.inv_g), T(0)};
         } else {
             v[q] = C{nc.x * ig, nc.y * ig};
@@ -658,7 +703,7 @@ __global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_init(cp
 {
     FIPI_TILE_SETUP(T, N, HubTile)
     const int y = blockIdx.y;
-    const size_t base = hidx(d, ok ? kx : 0, y, 0);
+    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
     C fv[E], v[E], w[E];
     tile_load<T, N>(fv, f_io, base, st, t, ok);
@@ -694,7 +739,7 @@ __global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_mu(cplx
 {
     FIPI_TILE_SETUP(T, N, HubTile)
     const int y = blockIdx.y;
-    const size_t base = hidx(d, ok ? kx : 0, y, 0);
+    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
     C v[E], cc[E];
     tile_load<T, N>(v, f_io, base, st, t, ok);
@@ -737,7 +782,7 @@ __global__ __launch_bounds__((StokesTile<T>::L * Line<T, N>::P)) void k_hub_stok
 {
     FIPI_TILE_SETUP(T, N, StokesTile)
     const int y = blockIdx.y;
-    const size_t base = hidx(d, ok ? kx : 0, y, 0);
+    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
     C v[3][E];
     tile_load<T, N>(v[0], a0, base, st, t, ok);
@@ -805,7 +850,7 @@ __global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_z(cplx<T> *
 {
     FIPI_TILE_SETUP(T, N, HubTile)
     const int y = blockIdx.y;
-    const size_t base = hidx(d, ok ? kx : 0, y, 0);
+    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
     C v[E];
     tile_load<T, N>(v, a, base, st, t, ok);

@@ -41,14 +41,20 @@ bool with_n(int n, F &&f)
     switch (n) {
     case 16: f(std::integral_constant<int, 16>{}); return true;
     case 32: f(std::integral_constant<int, 32>{}); return true;
+    case 48: f(std::integral_constant<int, 48>{}); return true;
     case 64: f(std::integral_constant<int, 64>{}); return true;
+    case 80: f(std::integral_constant<int, 80>{}); return true;
+    case 96: f(std::integral_constant<int, 96>{}); return true;
     case 128: f(std::integral_constant<int, 128>{}); return true;
     case 256: f(std::integral_constant<int, 256>{}); return true;
     default: return false;
     }
 }
 
-bool supported_n(int n) { return n == 16 || n == 32 || n == 64 || n == 128 || n == 256; }
+bool supported_n(int n)
+{
+    return n == 16 || n == 32 || n == 48 || n == 64 || n == 80 || n == 96 || n == 128 || n == 256;
+}
 
 }  // namespace
 
@@ -60,6 +66,8 @@ struct fipi_ctx {
     bool fp64 = true;
     bool has_state = false;
     int t = 0;
+    int *t_dev = nullptr;  // device copy of t (read by kernels a captured graph replays)
+    int nbox = 1;          // equal boxes in the slab; G, GH and n_box_particles are per box
     Dims d{};
     size_t G = 0, GH = 0;
     size_t esz = 8;  // bytes per real element
@@ -186,18 +194,20 @@ struct ProfScope {
 // at least one block: every kernel bounds-checks its index
 inline dim3 grid1(size_t n, int b) { return dim3((unsigned)(n == 0 ? 1 : (n + b - 1) / b)); }
 
-// x-pass grid: XT threads per workgroup, XT / P complex lines (2 real lines each)
+// x-pass grid: LPW complex lines (2 real lines each) per workgroup, the
+// last one possibly partial; one slab of workgroups per box
 template <typename T, int N>
-dim3 xgrid(const Dims &d)
+dim3 xgrid(const Dims &d, int nbox)
 {
-    return dim3((unsigned)(d.ny * d.nz / (2 * (XT / Line<T, N>::P))));
+    constexpr int LPW = XShape<T, N>::LPW;
+    return dim3((unsigned)((d.ny * d.nz / 2 + LPW - 1) / LPW), 1, (unsigned)nbox);
 }
 
 template <typename T>
 int xblocks(const Dims &d)
 {
     int r = 0;
-    with_n(d.nx, [&](auto NC) { r = (int)xgrid<T, decltype(NC)::value>(d).x; });
+    with_n(d.nx, [&](auto NC) { r = (int)xgrid<T, decltype(NC)::value>(d, 1).x; });
     return r;
 }
 
@@ -211,6 +221,12 @@ void scan(fipi_ctx *ctx, const int *cnt, int *start, int n, int *bsum, const dou
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, ctx->stream, cnt, start, n, bsum);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, ctx->stream, bsum, nb, start, n, ymax, yidx, nparts, mark);
     hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, ctx->stream, start, n, (const int *)bsum);
+}
+
+// advance the device step counter (one thread)
+__global__ void k_tick(int *t)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *t += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -245,7 +261,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_forward<T, N, NF, Prod>), (xgrid<T, N>(d)), dim3(XT), 0, ctx->stream, prod, o0, o1, d,
+            hipLaunchKernelGGL((k_x_forward<T, N, NF, Prod>), (xgrid<T, N>(d, ctx->nbox)), dim3(XShape<T, N>::THREADS), 0, ctx->stream, prod, o0, o1, d,
                                (const C *)ctx->twx, part);
         });
     }
@@ -256,7 +272,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_inverse<T, N, NF>), (xgrid<T, N>(d)), dim3(XT), 0, ctx->stream, i0, i1, i2, o0, o1, o2, d,
+            hipLaunchKernelGGL((k_x_inverse<T, N, NF>), (xgrid<T, N>(d, ctx->nbox)), dim3(XShape<T, N>::THREADS), 0, ctx->stream, i0, i1, i2, o0, o1, o2, d,
                                (const C *)ctx->twx);
         });
     }
@@ -267,7 +283,8 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.ny, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD>), dim3((unsigned)((d.nxh + Tile<T>::L - 1) / Tile<T>::L), d.nz),
+            hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD>),
+                               dim3((unsigned)((d.nxh + Tile<T>::L - 1) / Tile<T>::L), d.nz, ctx->nbox),
                                dim3((Tile<T>::L * Line<T, N>::P)), 0, ctx->stream, a0, a1, a2, extra, d, ctx->ky,
                                (const C *)ctx->twy);
         });
@@ -276,7 +293,7 @@ struct Impl {
     template <template <typename> class TileT = HubTile>
     static dim3 zgrid(const fipi_ctx *ctx)
     {
-        return dim3((unsigned)((ctx->d.nxh + TileT<T>::L - 1) / TileT<T>::L), ctx->d.ny);
+        return dim3((unsigned)((ctx->d.nxh + TileT<T>::L - 1) / TileT<T>::L), ctx->d.ny, ctx->nbox);
     }
 
     template <int SIGN>
@@ -313,14 +330,15 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_cgrad_f<T, N>), (xgrid<T, N>(d)), dim3(XT), 0, ctx->stream, H(ctx->h[0]), H(ctx->h[2]),
+            hipLaunchKernelGGL((k_x_cgrad_f<T, N>), (xgrid<T, N>(d, ctx->nbox)), dim3(XShape<T, N>::THREADS), 0, ctx->stream, H(ctx->h[0]), H(ctx->h[2]),
                                H(ctx->h[1]), R(ctx->c), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), H(ctx->h[0]),
                                fscale(ctx), d, ctx->kx, (const C *)ctx->twx);
         });
     }
 
     static void spread_build(fipi_ctx *ctx, const PartParams &pp, const double *px, const double *py,
-                             const double *pz, const double *f0, const double *f1, const double *f2)
+                             const double *pz, const double *f0, const double *f1, const double *f2,
+                             const int *box)
     {
         hipStream_t st = ctx->stream;
         const int nl = ctx->nlines;
@@ -332,8 +350,8 @@ struct Impl {
         }
         {
             ProfScope ps_(ctx, "spread_count", 24.0 * n + 32.0 * n);
-            hipLaunchKernelGGL(k_spread_count, grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, ctx->line_cnt,
-                               ctx->d_flags);
+            hipLaunchKernelGGL(k_spread_count, grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, box,
+                               ctx->line_cnt, ctx->d_flags);
         }
         {
             ProfScope ps_(ctx, "spread_scan", 8.0 * nl);
@@ -342,7 +360,7 @@ struct Impl {
         {
             ProfScope ps_(ctx, "spread_scatter", 48.0 * n + 8.0 * n * E);
             hipLaunchKernelGGL((k_spread_scatter<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, f0, f1,
-                               f2, ctx->line_start, ctx->line_cursor, (SpreadEntry<T> *)ctx->spr_tmp);
+                               f2, box, ctx->line_start, ctx->line_cursor, (SpreadEntry<T> *)ctx->spr_tmp);
         }
         {
             ProfScope ps_(ctx, "spread_rank", 16.0 * n * E + 4.0 * nl);
@@ -374,7 +392,7 @@ struct Impl {
                                R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), ctx->cell_cnt, ymax, yidx);
         } else {
             hipLaunchKernelGGL(k_cell_count, grid1(pp.n, 256), dim3(256), 0, st, pp, ps.p[0], ps.p[1], ps.p[2],
-                               ps.cell, ctx->cell_cnt, ps.err, ymax, yidx);
+                               ps.box, ps.cell, ctx->cell_cnt, ps.err, ymax, yidx);
         }
         {
             ProfScope ps_(ctx, "cells_scan_mark", 8.0 * nc + 12.0 * (pp.n / 256 + 1));
@@ -408,7 +426,7 @@ struct Impl {
         {
             ProfScope ps_(ctx, "contact", pp.n * (24.0 + 4.0 + 24.0 + 8.0 + 24.0));
             hipLaunchKernelGGL(k_contact, grid1(pp.n, 128), dim3(128), 0, st, pp, ctx->ps, ctx->cell_start,
-                               ctx->cell_ids, rmp, ctx->d_flags + 1);
+                               ctx->cell_ids, rmp, (const int *)ctx->t_dev, ctx->p.ramp_t, ctx->d_flags + 1);
         }
         {
             ProfScope ps_(ctx, "advect", pp.n * (5 * 24.0 + 8.0 + 2 * 24.0));
@@ -436,8 +454,8 @@ struct Impl {
                 // FP64 conv(0) = sum of conv over the grid, fixed order
                 const int nb = xblocks<T>(d);
                 ProfScope ps_(ctx, "k0_sum", 8.0 * nb);
-                hipLaunchKernelGGL(k_sum_strided, dim3(1), dim3(256), 0, st, (const double *)ctx->conv_part, nb, 1,
-                                   1, ctx->conv0);
+                hipLaunchKernelGGL(k_sum_strided, dim3(ctx->nbox), dim3(256), 0, st, (const double *)ctx->conv_part,
+                                   nb, 1, 1, ctx->conv0);
             }
             ProfScope ps_(ctx, "y_fwd_1", 2 * Hb);
             y_pass<-1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
@@ -483,13 +501,13 @@ struct Impl {
             }
             if (parts) {
                 const PartState &ps = ctx->ps;
-                spread_build(ctx, ctx->pp, ps.p[0], ps.p[1], ps.p[2], ps.fpp[0], ps.fpp[1], ps.fpp[2]);
+                spread_build(ctx, ctx->pp, ps.p[0], ps.p[1], ps.p[2], ps.fpp[0], ps.fpp[1], ps.fpp[2], ps.box);
             }
             {
                 ProfScope ps_(ctx, "x_inv_force_fwd", 4 * Hb + 4 * Rb + (parts ? 8.0 * ctx->n * sizeof(SpreadEntry<T>) : 0.0));
                 with_n(d.nx, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_x_force<T, N>), (xgrid<T, N>(d)), dim3(XT), 0, st, h0, h1, h2, (const T *)R(ctx->c),
+                    hipLaunchKernelGGL((k_x_force<T, N>), (xgrid<T, N>(d, ctx->nbox)), dim3(XShape<T, N>::THREADS), 0, st, h0, h1, h2, (const T *)R(ctx->c),
                                        (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
                                        (const int *)ctx->line_start, (const int *)ctx->node_cnt,
                                        (const SpreadNode<T> *)ctx->spr_tmp, parts ? 1 : 0, (T)ctx->p.rho1,
@@ -520,7 +538,7 @@ struct Impl {
                 ProfScope ps_(ctx, "x_inv_u_conv", 4 * Hb + 6 * Rb);
                 with_n(d.nx, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_x_inv_conv<T, N>), (xgrid<T, N>(d)), dim3(XT), 0, st, h0, (const C *)h1,
+                    hipLaunchKernelGGL((k_x_inv_conv<T, N>), (xgrid<T, N>(d, ctx->nbox)), dim3(XShape<T, N>::THREADS), 0, st, h0, (const C *)h1,
                                        (const C *)h2, R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]),
                                        (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]), d,
                                        (const C *)ctx->twx, mixed ? ctx->conv_part : nullptr);
@@ -532,6 +550,7 @@ struct Impl {
             particle_stage(ctx, 1);
             particle_stage(ctx, 2);
         }
+        hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, ctx->t_dev);
         ctx->t++;
     }
 
@@ -638,18 +657,9 @@ int check_params(const fipi_params *p)
 {
     if (!p) return set_err(nullptr, FIPI_ERR_ARG, "params is NULL");
     if (!supported_n(p->nx) || !supported_n(p->ny) || !supported_n(p->nz))
-        return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "grid sizes must each be one of 16, 32, 64, 128, 256");
+        return set_err(nullptr, FIPI_ERR_UNSUPPORTED,
+                       "grid sizes must each be one of 16, 32, 48, 64, 80, 96, 128, 256");
     if (p->switch_ch != 1) return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "switch_ch must be 1");
-    {
-        // the x pass takes 2*XT/P real lines per workgroup
-        int lines = 0;
-        with_n(p->nx, [&](auto NC) {
-            constexpr int n = decltype(NC)::value;
-            lines = 2 * (XT / std::min(Line<double, n>::P, Line<float, n>::P));
-        });
-        if (((int64_t)p->ny * p->nz) % lines != 0)
-            return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "ny*nz must be a multiple of the x-pass line count");
-    }
     if (p->precision != FIPI_FP64 && p->precision != FIPI_FP32_MIXED)
         return set_err(nullptr, FIPI_ERR_ARG, "precision must be FIPI_FP64 or FIPI_FP32_MIXED");
     if (p->n_particles < 0) return set_err(nullptr, FIPI_ERR_ARG, "n_particles < 0");
@@ -671,7 +681,8 @@ int setup_buffers(fipi_ctx *ctx)
 {
     const fipi_params &p = ctx->p;
     ctx->esz = sizeof(T);
-    const size_t G = ctx->G, GH = ctx->GH;
+    const size_t nb = (size_t)ctx->nbox;
+    const size_t G = ctx->G * nb, GH = ctx->GH * nb;  // whole slab
     int rc;
 #define ALLOC(ptr, bytes) \
     if ((rc = dalloc(ctx, (void **)&(ptr), (bytes))) != FIPI_OK) return rc
@@ -692,10 +703,11 @@ int setup_buffers(fipi_ctx *ctx)
     ALLOC(ctx->twy, sizeof(T) * 2 * p.ny);
     ALLOC(ctx->twz, sizeof(T) * 2 * p.nz);
     ALLOC(ctx->d_flags, sizeof(int) * 4);
+    ALLOC(ctx->t_dev, sizeof(int));
     ALLOC(ctx->d_red, sizeof(double) * (3 * DIAG_BLOCKS + 16));
-    ALLOC(ctx->k0, sizeof(double) * 4);
-    ALLOC(ctx->conv0, sizeof(double) * 2);
-    ALLOC(ctx->conv_part, sizeof(double) * (p.ny * p.nz / 2 + 1));
+    ALLOC(ctx->k0, sizeof(double) * K0_STRIDE * nb);
+    ALLOC(ctx->conv0, sizeof(double) * (nb + 1));
+    ALLOC(ctx->conv_part, sizeof(double) * nb * (size_t)(xblocks<T>(ctx->d) + 1));
     auto kx = wave_numbers(p.nx, p.pi, p.lx), ky = wave_numbers(p.ny, p.pi, p.ly), kz = wave_numbers(p.nz, p.pi, p.lz);
     HIPCHK(ctx, hipMemcpy(ctx->kx, kx.data(), sizeof(double) * p.nx, hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemcpy(ctx->ky, ky.data(), sizeof(double) * p.ny, hipMemcpyHostToDevice));
@@ -727,8 +739,12 @@ int setup_buffers(fipi_ctx *ctx)
     pp.tanh2 = p.tanh2;
     pp.sqrt2 = std::pow(2.0, 0.5);
     pp.has_ctheta = 0;
-    ctx->ncell = (nc[0] > 0 && nc[1] > 0 && nc[2] > 0) ? nc[0] * nc[1] * nc[2] : 1;
-    ctx->nlines = p.ny * p.nz;
+    pp.nbox = ctx->nbox;
+    pp.ncell_box = (nc[0] > 0 && nc[1] > 0 && nc[2] > 0) ? nc[0] * nc[1] * nc[2] : 1;
+    pp.nlines_box = p.ny * p.nz;
+    pp.rs = ctx->G;
+    ctx->ncell = pp.ncell_box * ctx->nbox;
+    ctx->nlines = pp.nlines_box * ctx->nbox;
     ALLOC(ctx->cell_cnt, sizeof(int) * (ctx->ncell + 1));
     ALLOC(ctx->cell_start, sizeof(int) * (ctx->ncell + 1));
     ALLOC(ctx->cell_cursor, sizeof(int) * (ctx->ncell + 1));
@@ -762,6 +778,11 @@ int setup_buffers(fipi_ctx *ctx)
     ALLOC(ps.cloc, sizeof(double) * n);
     ALLOC(ps.tau_i, sizeof(double) * n);
     ALLOC(ps.cell, sizeof(int) * n);
+    if (ctx->nbox > 1) {
+        int *bx;
+        ALLOC(bx, sizeof(int) * n);
+        ps.box = bx;
+    }
     ps.err = ctx->d_flags;
     ALLOC(ps.mark, sizeof(int));
 #undef ALLOC
@@ -872,14 +893,19 @@ int unit_cells(fipi_ctx *ctx, TmpParts &tp, int *start, int *ids)
 // ===========================================================================
 extern "C" {
 
-int fipi_create(const fipi_params *params, fipi_ctx **out)
+}  // extern "C"
+
+// A context of nbox equal boxes (the slab) with n_total particles in all.
+int create_ctx(const fipi_params *params, int nbox, int n_total, fipi_ctx **out)
 {
     if (!out) return set_err(nullptr, FIPI_ERR_ARG, "out is NULL");
     *out = nullptr;
     int rc = check_params(params);
     if (rc != FIPI_OK) return rc;
+    if (nbox < 1) return set_err(nullptr, FIPI_ERR_ARG, "nbox < 1");
     fipi_ctx *ctx = new fipi_ctx();
     ctx->p = *params;
+    ctx->nbox = nbox;
     ctx->fp64 = params->precision == FIPI_FP64;
     {
         const int nxh = params->nx / 2 + 1;
@@ -888,7 +914,9 @@ int fipi_create(const fipi_params *params, fipi_ctx **out)
     }
     ctx->G = (size_t)params->nx * params->ny * params->nz;
     ctx->GH = (size_t)ctx->d.nxp * params->ny * params->nz;
-    ctx->n = params->switch_particle ? params->n_particles : 0;
+    ctx->d.rs = ctx->G;
+    ctx->d.hs = ctx->GH;
+    ctx->n = params->switch_particle ? n_total : 0;
     hipError_t e = hipGetDevice(&ctx->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -903,6 +931,13 @@ int fipi_create(const fipi_params *params, fipi_ctx **out)
     }
     *out = ctx;
     return FIPI_OK;
+}
+
+extern "C" {
+
+int fipi_create(const fipi_params *params, fipi_ctx **out)
+{
+    return create_ctx(params, 1, params ? params->n_particles : 0, out);
 }
 
 int fipi_destroy(fipi_ctx *ctx)
@@ -924,33 +959,50 @@ const char *fipi_last_error(const fipi_ctx *ctx)
 
 void *fipi_stream(fipi_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
-int fipi_set_state(fipi_ctx *ctx, const double *phi, const double *pos, const double *radius,
-                   const double *cos_theta)
+}  // extern "C"
+
+// State of the whole slab: phi[nbox*G] (box-major), n particles (pos[3n])
+// with their boxes (box may be null for one box).
+int set_state_slab(fipi_ctx *ctx, const double *phi, const double *pos, const double *radius,
+                   const double *cos_theta, const int *box)
 {
     if (!ctx || !phi) return set_err(ctx, FIPI_ERR_ARG, "ctx and phi are required");
     if (ctx->n > 0 && !pos) return set_err(ctx, FIPI_ERR_ARG, "pos is required when particles are on");
+    if (ctx->n > 0 && ctx->nbox > 1 && !box) return set_err(ctx, FIPI_ERR_ARG, "particle boxes are required");
+    const size_t GS = ctx->G * ctx->nbox, GHS = ctx->GH * ctx->nbox;
     int rc;
-    if ((rc = copy_in_real(ctx, ctx->c, phi, ctx->G))) return rc;
-    for (int a = 0; a < 3; a++) HIPCHK(ctx, hipMemsetAsync(ctx->u[a], 0, ctx->G * ctx->esz, ctx->stream));
-    HIPCHK(ctx, hipMemsetAsync(ctx->hist, 0, ctx->GH * 2 * ctx->esz, ctx->stream));
+    if ((rc = copy_in_real(ctx, ctx->c, phi, GS))) return rc;
+    for (int a = 0; a < 3; a++) HIPCHK(ctx, hipMemsetAsync(ctx->u[a], 0, GS * ctx->esz, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->hist, 0, GHS * 2 * ctx->esz, ctx->stream));
     HIPCHK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->t_dev, 0, sizeof(int), ctx->stream));
     ctx->t = 0;
     ctx->cur = 0;
     ctx->conv_ready = false;
     {
-        // FP64 k = 0 mode: c_hat(0) = sum of phi (used by the FP32-mixed path)
-        long double s = 0.0L;
-        for (size_t i = 0; i < ctx->G; i++) s += (long double)phi[i];
-        const double k0h[4] = {(double)s, 0.0, 0.0, 0.0};
-        HIPCHK(ctx, hipMemcpyAsync(ctx->k0, k0h, sizeof(k0h), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(ctx, hipMemsetAsync(ctx->conv0, 0, sizeof(double) * 2, ctx->stream));
+        // FP64 k = 0 mode of each box: c_hat(0) = sum of phi (used by the FP32-mixed path)
+        std::vector<double> k0h((size_t)K0_STRIDE * ctx->nbox, 0.0);
+        for (int b = 0; b < ctx->nbox; b++) {
+            long double s = 0.0L;
+            const double *pb = phi + (size_t)b * ctx->G;
+            for (size_t i = 0; i < ctx->G; i++) s += (long double)pb[i];
+            k0h[(size_t)K0_STRIDE * b] = (double)s;
+        }
+        HIPCHK(ctx, hipMemcpyAsync(ctx->k0, k0h.data(), sizeof(double) * k0h.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        HIPCHK(ctx, hipMemsetAsync(ctx->conv0, 0, sizeof(double) * (ctx->nbox + 1), ctx->stream));
         HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     }
     if (ctx->fp64) Impl<double>::init_fields(ctx);
     else Impl<float>::init_fields(ctx);
     // keep phi exactly as given (the transform round trip above rewrote it)
-    if ((rc = copy_in_real(ctx, ctx->c, phi, ctx->G))) return rc;
+    if ((rc = copy_in_real(ctx, ctx->c, phi, GS))) return rc;
     const int n = ctx->n;
+    if (n > 0 && ctx->ps.box) {
+        for (int i = 0; i < n; i++)
+            if (box[i] < 0 || box[i] >= ctx->nbox) return set_err(ctx, FIPI_ERR_ARG, "particle box out of range");
+        HIPCHK(ctx, hipMemcpyAsync((void *)ctx->ps.box, box, sizeof(int) * n, hipMemcpyHostToDevice, ctx->stream));
+    }
     if (n > 0) {
         PartState &ps = ctx->ps;
         double *tmp;
@@ -993,6 +1045,14 @@ int fipi_set_state(fipi_ctx *ctx, const double *phi, const double *pos, const do
     HIPCHK(ctx, hipGetLastError());
     ctx->has_state = true;
     return FIPI_OK;
+}
+
+extern "C" {
+
+int fipi_set_state(fipi_ctx *ctx, const double *phi, const double *pos, const double *radius,
+                   const double *cos_theta)
+{
+    return set_state_slab(ctx, phi, pos, radius, cos_theta, nullptr);
 }
 
 int fipi_set_velocity(fipi_ctx *ctx, const double *u)
@@ -1062,28 +1122,33 @@ int fipi_get_time(fipi_ctx *ctx, int32_t *t)
     return FIPI_OK;
 }
 
-int fipi_diagnostics(fipi_ctx *ctx, fipi_diag *out)
+}  // extern "C"
+
+// Diagnostics of box b; its particles are [p0, p1) of the slab's list.
+int diag_box(fipi_ctx *ctx, int b, int p0, int p1, fipi_diag *out)
 {
     if (!ctx || !out) return set_err(ctx, FIPI_ERR_ARG, "ctx and out are required");
+    if (b < 0 || b >= ctx->nbox) return set_err(ctx, FIPI_ERR_ARG, "box out of range");
     hipStream_t st = ctx->stream;
-    const double dx = ctx->p.lx / ctx->p.nx, dy = ctx->p.ly / ctx->p.ny, dz = ctx->p.lz / ctx->p.nz;
-    (void)dy;
-    (void)dz;
+    const double dx = ctx->p.lx / ctx->p.nx;
+    const size_t off = (size_t)b * ctx->G * ctx->esz;
+    auto at = [&](void *p) { return (const void *)((const char *)p + off); };
     if (ctx->fp64)
-        hipLaunchKernelGGL((k_diag_partial<double>), dim3(DIAG_BLOCKS), dim3(256), 0, st, (const double *)ctx->c,
-                           (const double *)ctx->g[0], (const double *)ctx->g[1], (const double *)ctx->g[2], ctx->G,
-                           ctx->p.lambda, ctx->p.epsilon, ctx->d_red);
+        hipLaunchKernelGGL((k_diag_partial<double>), dim3(DIAG_BLOCKS), dim3(256), 0, st, (const double *)at(ctx->c),
+                           (const double *)at(ctx->g[0]), (const double *)at(ctx->g[1]),
+                           (const double *)at(ctx->g[2]), ctx->G, ctx->p.lambda, ctx->p.epsilon, ctx->d_red);
     else
-        hipLaunchKernelGGL((k_diag_partial<float>), dim3(DIAG_BLOCKS), dim3(256), 0, st, (const float *)ctx->c,
-                           (const float *)ctx->g[0], (const float *)ctx->g[1], (const float *)ctx->g[2], ctx->G,
-                           ctx->p.lambda, ctx->p.epsilon, ctx->d_red);
+        hipLaunchKernelGGL((k_diag_partial<float>), dim3(DIAG_BLOCKS), dim3(256), 0, st, (const float *)at(ctx->c),
+                           (const float *)at(ctx->g[0]), (const float *)at(ctx->g[1]), (const float *)at(ctx->g[2]),
+                           ctx->G, ctx->p.lambda, ctx->p.epsilon, ctx->d_red);
     hipLaunchKernelGGL(k_sum_strided, dim3(1), dim3(256), 0, st, ctx->d_red, DIAG_BLOCKS, 3, 3,
                        ctx->d_red + 3 * DIAG_BLOCKS);
-    if (ctx->n > 0)
-        hipLaunchKernelGGL(k_sum_strided, dim3(1), dim3(256), 0, st, ctx->ps.tau_i, ctx->n, 1, 1,
+    const int np = p1 - p0;
+    if (np > 0)
+        hipLaunchKernelGGL(k_sum_strided, dim3(1), dim3(256), 0, st, ctx->ps.tau_i + p0, np, 1, 1,
                            ctx->d_red + 3 * DIAG_BLOCKS + 3);
     double h[4] = {0, 0, 0, 0};
-    HIPCHK(ctx, hipMemcpyAsync(h, ctx->d_red + 3 * DIAG_BLOCKS, sizeof(double) * (ctx->n > 0 ? 4 : 3),
+    HIPCHK(ctx, hipMemcpyAsync(h, ctx->d_red + 3 * DIAG_BLOCKS, sizeof(double) * (np > 0 ? 4 : 3),
                                hipMemcpyDeviceToHost, st));
     int mark = -1;
     if (ctx->n > 0) HIPCHK(ctx, hipMemcpyAsync(&mark, ctx->ps.mark, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1095,13 +1160,21 @@ int fipi_diagnostics(fipi_ctx *ctx, fipi_diag *out)
     if (!ctx->fp64) {
         // FP32-mixed: the mass is carried in FP64 as the k = 0 mode
         double k0h[3];
-        HIPCHK(ctx, hipMemcpy(k0h, ctx->k0, sizeof(k0h), hipMemcpyDeviceToHost));
+        HIPCHK(ctx, hipMemcpy(k0h, ctx->k0 + (size_t)K0_STRIDE * b, sizeof(k0h), hipMemcpyDeviceToHost));
         out->mean_phi = k0h[ctx->cur] / (double)ctx->G;
     }
     out->free_energy = h[1] * cell;
     out->interface_area = h[2] * cell;
     out->tau = h[3];
     return FIPI_OK;
+}
+
+extern "C" {
+
+int fipi_diagnostics(fipi_ctx *ctx, fipi_diag *out)
+{
+    if (!ctx || !out) return set_err(ctx, FIPI_ERR_ARG, "ctx and out are required");
+    return diag_box(ctx, 0, 0, ctx->n, out);
 }
 
 int fipi_step_async(fipi_ctx *ctx, int32_t nsteps)
@@ -1258,7 +1331,7 @@ int fipi_contact_forces(fipi_ctx *ctx, const double *d_pos, int32_t n, int32_t t
     const double rmp = t < 0 ? 0.0 : (t < thr ? t / thr : 1.0);
     if (n > 0) {
         hipLaunchKernelGGL(k_contact, grid1(n, 128), dim3(128), 0, ctx->stream, pp, tp.ps, ctx->cell_start, ids, rmp,
-                           ctx->d_flags + 1);
+                           (const int *)nullptr, 0.0, ctx->d_flags + 1);
         hipLaunchKernelGGL(k_soa_to_aos, grid1(n, 256), dim3(256), 0, ctx->stream, tp.ps.fpp[0], tp.ps.fpp[1],
                            tp.ps.fpp[2], d_f_pp, n);
     }
@@ -1307,12 +1380,12 @@ int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32
     pp.n = n;
     HIPCHK(ctx, hipMemsetAsync(d_grid3, 0, 3 * ctx->G * ctx->esz, ctx->stream));
     if (ctx->fp64) {
-        Impl<double>::spread_build(ctx, pp, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], f[0], f[1], f[2]);
+        Impl<double>::spread_build(ctx, pp, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], f[0], f[1], f[2], nullptr);
         hipLaunchKernelGGL((k_spread_dense<double>), grid1(ctx->nlines, 256), dim3(256), 0, ctx->stream, ctx->nlines,
                            ctx->p.nx, ctx->G, ctx->line_start, (const int *)ctx->node_cnt,
                            (const SpreadNode<double> *)ctx->spr_tmp, (double *)d_grid3);
     } else {
-        Impl<float>::spread_build(ctx, pp, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], f[0], f[1], f[2]);
+        Impl<float>::spread_build(ctx, pp, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], f[0], f[1], f[2], nullptr);
         hipLaunchKernelGGL((k_spread_dense<float>), grid1(ctx->nlines, 256), dim3(256), 0, ctx->stream, ctx->nlines,
                            ctx->p.nx, ctx->G, ctx->line_start, (const int *)ctx->node_cnt,
                            (const SpreadNode<float> *)ctx->spr_tmp, (float *)d_grid3);

@@ -30,6 +30,11 @@ struct PartParams {
     double pi, mu, dt, epsilon, sigma, a_pi, tanh2;
     double sqrt2;                // pow(2.0, 0.5) as the reference's sqrt override
     int has_ctheta;
+    // slab of nbox equal boxes: contact cell and grid line indices are
+    // global, box*ncell_box + cell and box*nlines_box + line; box b's
+    // grid arrays start b*rs elements in
+    int nbox, ncell_box, nlines_box;
+    size_t rs;
 };
 
 struct PartState {
@@ -40,7 +45,10 @@ struct PartState {
     int *cell;                   // contact cell of each particle
     int *err;                    // device error flags (bit 0: sorting range)
     int *mark;                   // it_mark (argmax y), persists across sorts
+    const int *box;              // box of each particle (null: all in box 0)
 };
+
+__device__ __forceinline__ int part_box(const int *box, int a) { return box ? box[a] : 0; }
 
 // ---------------------------------------------------------------------------
 // wave64 inclusive scan of one int per lane
@@ -216,18 +224,19 @@ __device__ __forceinline__ int cell_of(const PartParams &P, double x, double y, 
 // Histogram of contact cells; flags particles outside the sorting range.
 __global__ __launch_bounds__(256) void k_cell_count(PartParams P, const double *__restrict__ px,
                                                     const double *__restrict__ py, const double *__restrict__ pz,
-                                                    int *__restrict__ cell, int *__restrict__ cnt,
-                                                    int *__restrict__ err, double *__restrict__ ymax,
-                                                    int *__restrict__ yidx)
+                                                    const int *__restrict__ box, int *__restrict__ cell,
+                                                    int *__restrict__ cnt, int *__restrict__ err,
+                                                    double *__restrict__ ymax, int *__restrict__ yidx)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     block_argmax_y(i < P.n ? py[i] : -1.0, i, ymax, yidx);
     if (i >= P.n) return;
     int a, b, c;
     const int cc = cell_of(P, px[i], py[i], pz[i], &a, &b, &c);
-    cell[i] = cc;
-    if (cc < 0) { atomicOr(err, 1); return; }
-    atomicAdd(&cnt[cc], 1);
+    if (cc < 0) { cell[i] = -1; atomicOr(err, 1); return; }
+    const int g = cc + part_box(box, i) * P.ncell_box;
+    cell[i] = g;
+    atomicAdd(&cnt[g], 1);
 }
 
 __global__ void k_cell_scatter(int n, const int *__restrict__ cell, const int *__restrict__ start,
@@ -327,8 +336,8 @@ constexpr int MAX_NB = 32;  // contact partners per side held for ordering
 // Reference-order force sum by repeated scans, for particles with more than
 // MAX_NB contacts on one side (not expected for spheres; kept exact).
 __device__ void contact_slow(const PartParams &P, const PartState &S, const int *__restrict__ start,
-                             const int *__restrict__ ids, int a, int o, int pp, int q, double rmp, bool tau_on,
-                             double *fo, double *tau_out)
+                             const int *__restrict__ ids, int a, int o, int pp, int q, int cb, double rmp,
+                             bool tau_on, double *fo, double *tau_out)
 {
     double fx = 0.0, fy = 0.0, fz = 0.0;
     int last = -1;
@@ -337,7 +346,7 @@ __device__ void contact_slow(const PartParams &P, const PartState &S, const int 
         for (int k = q - 1; k <= q + 1; k++)
             for (int j = pp - 1; j <= pp + 1; j++)
                 for (int i = o - 1; i <= o + 1; i++) {
-                    const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                    const int co = cb + wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
                     for (int e = start[co]; e < start[co + 1]; e++) {
                         const int b = ids[e];
                         if (b >= a || b <= last || b >= best) continue;
@@ -358,7 +367,7 @@ __device__ void contact_slow(const PartParams &P, const PartState &S, const int 
     for (int k = q - 1; k <= q + 1; k++)
         for (int j = pp - 1; j <= pp + 1; j++)
             for (int i = o - 1; i <= o + 1; i++) {
-                const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                const int co = cb + wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
                 for (int e = start[co]; e < start[co + 1]; e++) {
                     const int b = ids[e];
                     if (b <= a) continue;
@@ -384,14 +393,26 @@ __device__ void contact_slow(const PartParams &P, const PartState &S, const int 
 // One scan of the 27 cells collects both partner lists.
 // tau_i[a] = sum over b > a of 0.5 F.d when f_pi[a] and f_pi[mark] are
 // non-zero (cluster.cpp:256-259).
+// rmp: the contact ramp; when t_dev is set it is ramp(*t_dev + 1, ramp_t)
+// (primitive.cpp:122-129) from the device step counter instead.
+__device__ __forceinline__ double ramp_dev(double t, double thr)
+{
+    if (t < 0) return 0.0;
+    if (t < thr) return t / thr;
+    return 1.0;
+}
+
 __global__ __launch_bounds__(128) void k_contact(PartParams P, PartState S, const int *__restrict__ start,
                                                  const int *__restrict__ ids, double rmp,
+                                                 const int *__restrict__ t_dev, double ramp_t,
                                                  int *__restrict__ overflow)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     // only the slots the sort filled (particles outside the range are left out)
-    if (s >= P.n || s >= start[P.ncx * P.ncy * P.ncz]) return;
+    if (s >= P.n || s >= start[P.ncell_box * P.nbox]) return;
+    if (t_dev) rmp = ramp_dev((double)(*t_dev + 1), ramp_t);
     const int a = ids[s];
+    const int cb = part_box(S.box, a) * P.ncell_box;
     int o, pp, q;
     cell_of(P, S.p[0][a], S.p[1][a], S.p[2][a], &o, &pp, &q);
     const int mk = *S.mark;
@@ -406,7 +427,7 @@ __global__ __launch_bounds__(128) void k_contact(PartParams P, PartState S, cons
     for (int k = q - 1; k <= q + 1; k++)
         for (int j = pp - 1; j <= pp + 1; j++)
             for (int i = o - 1; i <= o + 1; i++) {
-                const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                const int co = cb + wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
                 const int e1 = start[co + 1];
                 for (int e = start[co]; e < e1; e++) {
                     const int b = ids[e];
@@ -427,7 +448,7 @@ __global__ __launch_bounds__(128) void k_contact(PartParams P, PartState S, cons
     double fo[3], tau = 0.0;
     if (ovf) {
         atomicOr(overflow, 1);
-        contact_slow(P, S, start, ids, a, o, pp, q, rmp, tau_on, fo, &tau);
+        contact_slow(P, S, start, ids, a, o, pp, q, cb, rmp, tau_on, fo, &tau);
     } else {
         for (int x = 1; x < nl; x++) {
             const int v = low[x];
@@ -474,12 +495,13 @@ __global__ void k_pairs(PartParams P, PartState S, const int *__restrict__ start
     }
     int o, pp, q;
     cell_of(P, S.p[0][a], S.p[1][a], S.p[2][a], &o, &pp, &q);
+    const int cb = part_box(S.box, a) * P.ncell_box;
     int cnt = 0;
     const int base = pstart ? pstart[a] : 0;
     for (int k = q - 1; k <= q + 1; k++)
         for (int j = pp - 1; j <= pp + 1; j++)
             for (int i = o - 1; i <= o + 1; i++) {
-                const int co = wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                const int co = cb + wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
                 for (int e = start[co]; e < start[co + 1]; e++) {
                     const int b = ids[e];
                     if (b <= a) continue;
@@ -562,6 +584,11 @@ __global__ __launch_bounds__(256) void k_interp_fpi_count(PartParams P, PartStat
         S.cell[a] = -1;
         return;
     }
+    const int bx = part_box(S.box, a);
+    c += bx * P.rs;
+    g0 += bx * P.rs;
+    g1 += bx * P.rs;
+    g2 += bx * P.rs;
     const double phi = interp1(P, c, px, py, pz);
     const double gx = interp1(P, g0, px, py, pz);
     const double gy = interp1(P, g1, px, py, pz);
@@ -592,9 +619,10 @@ __global__ __launch_bounds__(256) void k_interp_fpi_count(PartParams P, PartStat
     S.fpi[2][a] = f[2];
     int ci, cj, ck;
     const int cc = cell_of(P, px, py, pz, &ci, &cj, &ck);
-    S.cell[a] = cc;
-    if (cc < 0) { atomicOr(S.err, 1); return; }
-    atomicAdd(&cnt[cc], 1);
+    if (cc < 0) { S.cell[a] = -1; atomicOr(S.err, 1); return; }
+    const int gc = cc + bx * P.ncell_box;
+    S.cell[a] = gc;
+    atomicAdd(&cnt[gc], 1);
 }
 
 // Heun stage (element.cpp:156-173; uf = 0, f_ext = 0) and periodic wrap
@@ -641,7 +669,8 @@ __device__ __forceinline__ void spread_corner(const PartParams &P, double px, do
 }
 
 __global__ void k_spread_count(PartParams P, const double *__restrict__ px, const double *__restrict__ py,
-                               const double *__restrict__ pz, int *__restrict__ line_cnt, int *__restrict__ err)
+                               const double *__restrict__ pz, const int *__restrict__ box,
+                               int *__restrict__ line_cnt, int *__restrict__ err)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= P.n) return;
@@ -649,6 +678,7 @@ __global__ void k_spread_count(PartParams P, const double *__restrict__ px, cons
     double o, p, q;
     spread_corner(P, px[a], py[a], pz[a], &x, &y, &z, &o, &p, &q);
     if (x < 0 || x >= P.nx || y < 0 || y >= P.ny || z < 0 || z >= P.nz) { atomicOr(err, 2); return; }
+    line_cnt += part_box(box, a) * P.nlines_box;
     const int y1 = (y + 1) % P.ny, z1 = (z + 1) % P.nz;
     atomicAdd(&line_cnt[y + P.ny * z], 2);
     atomicAdd(&line_cnt[y1 + P.ny * z], 2);
@@ -660,8 +690,8 @@ template <typename T>
 __global__ void k_spread_scatter(PartParams P, const double *__restrict__ px, const double *__restrict__ py,
                                  const double *__restrict__ pz, const double *__restrict__ f0,
                                  const double *__restrict__ f1, const double *__restrict__ f2,
-                                 const int *__restrict__ line_start, int *__restrict__ cursor,
-                                 SpreadEntry<T> *__restrict__ out)
+                                 const int *__restrict__ box, const int *__restrict__ line_start,
+                                 int *__restrict__ cursor, SpreadEntry<T> *__restrict__ out)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= P.n) return;
@@ -669,6 +699,11 @@ __global__ void k_spread_scatter(PartParams P, const double *__restrict__ px, co
     double o, p, q;
     spread_corner(P, px[a], py[a], pz[a], &x, &y, &z, &o, &p, &q);
     if (x < 0 || x >= P.nx || y < 0 || y >= P.ny || z < 0 || z >= P.nz) return;
+    {
+        const int lb = part_box(box, a) * P.nlines_box;
+        line_start += lb;
+        cursor += lb;
+    }
     // f = (f_pp + f_ext)/dx/dy/dz, f_ext = 0
     const double f[3] = {(f0[a] + 0.0) / P.dx / P.dy / P.dz, (f1[a] + 0.0) / P.dx / P.dy / P.dz,
                          (f2[a] + 0.0) / P.dx / P.dy / P.dz};

@@ -36,7 +36,7 @@ def test_struct_sizes():
 def test_param_checks_return_codes():
     L = fipi_amd.lib()
     h = C.c_void_p()
-    bad = fipi_amd._to_cparams(fipi_amd.config_params(48, 0, 2.0, 10))
+    bad = fipi_amd._to_cparams(fipi_amd.config_params(40, 0, 2.0, 10))
     assert L.fipi_create(C.byref(bad), C.byref(h)) == -3  # FIPI_ERR_UNSUPPORTED
     assert b"grid sizes" in L.fipi_last_error(None)
     p = fipi_amd.config_params(32, 0, 2.0, 10).replace(switch_ch=0)

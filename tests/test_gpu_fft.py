@@ -10,15 +10,18 @@ from fipi_amd import FP32_MIXED, FP64, config_params
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n", [16, 32, 64, 128, 256])
+@pytest.mark.parametrize("n", [16, 32, 48, 64, 80, 96, 128, 256, (48, 80, 96), (96, 32, 80)])
 @pytest.mark.parametrize("prec", [FP64, FP32_MIXED])
 def test_r2c_c2r_vs_numpy(gpu, n, prec):
+    """Radix-2/4/8 lengths and the radix-3/5/6 middle stages of 48, 80 and
+    96, cubic and mixed (nx, ny, nz)."""
     import torch
     print(f"fft n={n} prec={prec}", flush=True)
-    p = config_params(n, 0, 2.0, 10, precision=prec)
+    nx, ny, nz = (n, n, n) if isinstance(n, int) else n
+    p = config_params(nx, 0, 2.0, 10, precision=prec).replace(ny=ny, nz=nz, ly=float(ny), lz=float(nz))
     sim = fipi_amd.Simulation(p)
-    rng = np.random.default_rng(n)
-    x = rng.standard_normal((n, n, n))
+    rng = np.random.default_rng(nx * 7 + ny * 3 + nz)
+    x = rng.standard_normal((nz, ny, nx))
     dt = torch.float64 if prec == FP64 else torch.float32
     xt = torch.tensor(x, dtype=dt, device=gpu)
     X = sim.fft_r2c(xt)

@@ -53,6 +53,17 @@ def test_step_with_particles_32(gpu):
         assert rel_l2(gp["f_pi"], rp["f_pi"]) < 1e-8, (t, rel_l2(gp["f_pi"], rp["f_pi"]))
 
 
+@pytest.mark.parametrize("n,npart", [(48, 120), (80, 300), (96, 400)])
+def test_step_odd_radix(gpu, n, npart):
+    """Edges with radix-3/5/6 transforms (the ensemble's 48, 80, 96)."""
+    print(f"odd-radix {n}^3 + {npart}", flush=True)
+    p, phi0, pos = make_case(n, npart, steps=30, seed=n)
+    for t, g_phi, r_phi, g_u, r_u, gp, rp in _run_both(p, phi0, pos, 6, chunks=2):
+        assert rel_l2(g_phi, r_phi) < TOL, (t, rel_l2(g_phi, r_phi))
+        assert rel_l2(g_u, r_u) < TOL, (t, rel_l2(g_u, r_u))
+        assert rel_l2(gp["p"], rp["p"]) < TOL, (t, rel_l2(gp["p"], rp["p"]))
+
+
 def test_step_config1_64(gpu):
     """64^3 benchmark units with 2.5 dx spheres (config 1 physics), 10 steps."""
     print("config1 64^3", flush=True)
