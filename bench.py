@@ -10,6 +10,13 @@ sequential addition (gap 0.5 dx), contact angles U(60, 120) deg.
 One step = one pass of the reference time loop body (src/main.cpp:75-222):
 CH SBDF2 update, Stokes solve, two Heun particle stages.  Inputs are
 resident in HBM before the timed region.  Prints one JSON line.
+
+The line also carries the batched-ensemble measurement ("ensemble"): 512
+independent boxes with edges drawn from {32, 48, 64, 80, 96}, mean
+composition U(-0.3, 0.3) + U(-0.05, 0.05) noise, U(0, 2000) spheres of
+radius 2 dx per box (capped where random sequential addition jams),
+FP32-mixed, advanced together (one graph replay per step), against the
+same boxes stepped one at a time through the single-box path.
 """
 from __future__ import annotations
 
@@ -39,6 +46,89 @@ def workload(n, n_particles, precision, seed=1):
     theta = fipi_amd.init_uniform(n_particles, 60.0, 120.0, seed + 2)
     cos_theta = np.cos(np.deg2rad(theta))
     return p, phi0, pos, cos_theta
+
+
+EDGE_SET = (32, 48, 64, 80, 96)
+
+
+def ensemble_workload(nboxes, seed=11, max_attempts=2000):
+    """Seeded boxes of the ensemble workload: (edge, spheres placed,
+    phi0, positions) per box, the requested counts and how many boxes
+    jammed below their request."""
+    import fipi_amd
+    rng = np.random.default_rng(seed)
+    edges = rng.choice(EDGE_SET, size=nboxes)
+    means = rng.uniform(-0.3, 0.3, size=nboxes)
+    want = rng.integers(0, 2001, size=nboxes)
+    boxes, capped = [], 0
+    for b in range(nboxes):
+        n = int(edges[b])
+        phi = fipi_amd.init_phi((n, n, n), float(means[b]), 0.05, seed * 100003 + b)
+        pos = fipi_amd.init_rsa(int(want[b]), 2.0, 0.5, (n, n, n), seed * 200003 + b, max_attempts)
+        capped += int(pos.shape[0] < want[b])
+        boxes.append((n, int(pos.shape[0]), phi, pos))
+    return boxes, want, capped
+
+
+def ensemble_run(nboxes, steps, loop_sample, loop_steps):
+    """Aggregate grid-point-steps/s of the batched ensemble and of looping
+    the single-box path over a sample of the same boxes."""
+    import torch
+    import fipi_amd
+    boxes, want, capped = ensemble_workload(nboxes)
+    common = fipi_amd.config_params(32, 0, 2.0, steps, precision=fipi_amd.FP32_MIXED).replace(switch_particle=1)
+    sum_g = sum(n ** 3 for n, _, _, _ in boxes)
+    ens = fipi_amd.Ensemble(common, [b[0] for b in boxes], [b[1] for b in boxes])
+    for i, (n, k, phi, pos) in enumerate(boxes):
+        ens.set_box(i, phi, pos if k else None)
+    t0 = time.perf_counter()
+    ens.tune(3)
+    tune_s = time.perf_counter() - t0
+    ens.step(2)  # first-order start step, then graph capture + first replay
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step_async(steps)
+    ens.sync()
+    wall = time.perf_counter() - t0
+    m0 = [ens.diagnostics(b)["mean_phi"] for b in range(min(nboxes, 8))]
+    tuning = {str(n): dict(zip(("y_tile_bytes", "hub_tile_bytes", "stokes_tile_bytes"), ens.tuning(n)))
+              for n in EDGE_SET}
+    ens.close()
+    del ens
+    torch.cuda.synchronize()
+    # the same boxes one at a time through the single-box path
+    loop_g, loop_t = 0, 0.0
+    for n, k, phi, pos in boxes[:loop_sample]:
+        sim = fipi_amd.Simulation(fipi_amd.config_params(n, k, 2.0, steps, precision=fipi_amd.FP32_MIXED))
+        sim.set_state(phi, pos if k else None)
+        sim.step(2)
+        t0 = time.perf_counter()
+        sim.step_async(loop_steps)
+        sim.sync()
+        loop_t += time.perf_counter() - t0
+        loop_g += n ** 3 * loop_steps
+        sim.close()
+    rate = sum_g * steps / wall
+    loop_rate = loop_g / loop_t
+    hist = {str(n): int(sum(1 for b in boxes if b[0] == n)) for n in EDGE_SET}
+    return {
+        "workload": f"{nboxes} boxes, edges from {list(EDGE_SET)}, U(0,2000) spheres r=2dx (RSA-capped), "
+                    f"FP32-mixed, dt=0.01",
+        "boxes_per_edge": hist,
+        "grid_points": sum_g,
+        "particles": int(sum(b[1] for b in boxes)),
+        "particles_requested": int(want.sum()),
+        "boxes_capped_by_rsa": capped,
+        "steps": steps,
+        "ms_per_step": round(wall * 1e3 / steps, 4),
+        "grid_point_steps_per_s": rate,
+        "looped_single_box_grid_point_steps_per_s": loop_rate,
+        "looped_sample": f"first {loop_sample} boxes, {loop_steps} steps each after 2 untimed",
+        "speedup_vs_looped": round(rate / loop_rate, 2),
+        "tuning": tuning,
+        "tune_seconds": round(tune_s, 2),
+        "mean_phi_first_boxes": m0,
+    }
 
 
 def step_bytes(p, n_particles):
@@ -86,6 +176,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--single", action="store_true", help="skip the other precision")
+    ap.add_argument("--ensemble-boxes", type=int, default=512)
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--ensemble-loop-sample", type=int, default=64)
+    ap.add_argument("--ensemble-loop-steps", type=int, default=10)
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -207,6 +302,11 @@ def main():
             "mean_phi_abs_err": float(abs(mix["diag"]["mean_phi"] - f64["diag"]["mean_phi"])),
         }
 
+    ens = None
+    if not args.no_ensemble and args.ensemble_boxes > 0:
+        ens = ensemble_run(args.ensemble_boxes, args.ensemble_steps, args.ensemble_loop_sample,
+                           args.ensemble_loop_steps)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, phi0, pos, cos_theta, args.cpu_steps)
@@ -248,6 +348,7 @@ def main():
         "kernels": kernel_table(stats),
         "kernels_other_precision": kernel_table(summarize(other_run.get("stats", []))) if other_run else None,
         "diagnostics": {k: diag[k] for k in ("t", "mean_phi", "free_energy", "interface_area")},
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

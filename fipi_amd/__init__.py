@@ -17,7 +17,7 @@ from .params import FP32_MIXED, FP64, Params, config_params, reference_params
 
 __all__ = [
     "FP64", "FP32_MIXED", "Params", "config_params", "reference_params",
-    "Simulation", "FipiError", "lib", "LIB_PATH", "init_phi", "init_rsa", "init_uniform",
+    "Simulation", "Ensemble", "FipiError", "lib", "LIB_PATH", "init_phi", "init_rsa", "init_uniform",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -36,6 +36,10 @@ ABI_SYMBOLS = [
     "fipi_step", "fipi_step_async", "fipi_sync", "fipi_profile_steps",
     "fipi_fft_r2c", "fipi_fft_c2r", "fipi_cells_build", "fipi_contact_pairs",
     "fipi_contact_forces", "fipi_interp", "fipi_spread",
+    "fipi_ensemble_create", "fipi_ensemble_destroy", "fipi_ensemble_last_error", "fipi_ensemble_set_box",
+    "fipi_ensemble_tune", "fipi_ensemble_step", "fipi_ensemble_step_async", "fipi_ensemble_sync",
+    "fipi_ensemble_stream", "fipi_ensemble_get_phi", "fipi_ensemble_get_particles",
+    "fipi_ensemble_diagnostics", "fipi_ensemble_tuning",
     "fipi_init_phi", "fipi_init_rsa", "fipi_init_uniform",
 ]
 
@@ -64,6 +68,10 @@ class CParams(C.Structure):
         ("switch_particle", C.c_int32), ("switch_ch", C.c_int32), ("switch_ns", C.c_int32),
         ("precision", C.c_int32),
     ]
+
+
+class CBoxDesc(C.Structure):
+    _fields_ = [("n", C.c_int32), ("n_particles", C.c_int32)]
 
 
 class CDiag(C.Structure):
@@ -122,6 +130,21 @@ def lib():
                                 C.c_double, C.c_uint64, i64, dp]
     L.fipi_init_rsa.restype = i64
     L.fipi_init_uniform.argtypes = [i64, C.c_double, C.c_double, C.c_uint64, dp]
+    L.fipi_ensemble_create.argtypes = [C.POINTER(CParams), C.c_int32, C.POINTER(CBoxDesc), C.POINTER(vp)]
+    L.fipi_ensemble_destroy.argtypes = [vp]
+    L.fipi_ensemble_last_error.argtypes = [vp]
+    L.fipi_ensemble_last_error.restype = C.c_char_p
+    L.fipi_ensemble_set_box.argtypes = [vp, C.c_int32, dp, dp, dp, dp]
+    L.fipi_ensemble_tune.argtypes = [vp, C.c_int32]
+    L.fipi_ensemble_step.argtypes = [vp, C.c_int32]
+    L.fipi_ensemble_step_async.argtypes = [vp, C.c_int32]
+    L.fipi_ensemble_sync.argtypes = [vp]
+    L.fipi_ensemble_stream.argtypes = [vp]
+    L.fipi_ensemble_stream.restype = vp
+    L.fipi_ensemble_get_phi.argtypes = [vp, C.c_int32, dp]
+    L.fipi_ensemble_get_particles.argtypes = [vp, C.c_int32, dp]
+    L.fipi_ensemble_diagnostics.argtypes = [vp, C.c_int32, C.POINTER(CDiag)]
+    L.fipi_ensemble_tuning.argtypes = [vp, C.c_int32, ip, ip, ip]
     _lib = L
     return L
 
@@ -352,3 +375,85 @@ class Simulation:
         self._c(lib().fipi_spread(self._h, C.c_void_p(pos.data_ptr()), C.c_void_p(f.data_ptr()), n,
                                   C.c_void_p(grid.data_ptr())))
         return grid.cpu().numpy()
+
+
+class Ensemble:
+    """Many independent periodic cubic boxes advanced together (a
+    fipi_ensemble): common physics `params` (its lx/nx is the grid spacing
+    of every box), per-box edge lengths and particle counts."""
+
+    def __init__(self, params: Params, edges, n_particles):
+        self.params = params
+        self.edges = [int(n) for n in edges]
+        self.counts = [int(k) for k in n_particles]
+        assert len(self.edges) == len(self.counts)
+        desc = (CBoxDesc * len(self.edges))(*[CBoxDesc(n, k) for n, k in zip(self.edges, self.counts)])
+        self._h = C.c_void_p()
+        self._cp = _to_cparams(params)
+        rc = lib().fipi_ensemble_create(C.byref(self._cp), len(self.edges), desc, C.byref(self._h))
+        if rc != 0:
+            raise FipiError(f"{ERRORS.get(rc, rc)}: {lib().fipi_ensemble_last_error(None).decode()}")
+
+    def _c(self, rc):
+        if rc != 0:
+            msg = lib().fipi_ensemble_last_error(self._h)
+            raise FipiError(f"{ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().fipi_ensemble_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_box(self, b, phi, pos=None, radius=None, cos_theta=None):
+        n, k = self.edges[b], self.counts[b]
+        phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+        assert phi.size == n ** 3
+        if k:
+            pos = np.ascontiguousarray(pos, dtype=np.float64)
+            assert pos.shape == (k, 3)
+        r = None if radius is None else np.ascontiguousarray(radius, dtype=np.float64)
+        ct = None if cos_theta is None else np.ascontiguousarray(cos_theta, dtype=np.float64)
+        self._c(lib().fipi_ensemble_set_box(self._h, b, _dp(phi), _dp(pos) if k else None, _dp(r), _dp(ct)))
+
+    def tune(self, steps=3):
+        self._c(lib().fipi_ensemble_tune(self._h, steps))
+
+    def tuning(self, n):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        self._c(lib().fipi_ensemble_tuning(self._h, n, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def step(self, nsteps=1):
+        self._c(lib().fipi_ensemble_step(self._h, nsteps))
+
+    def step_async(self, nsteps=1):
+        self._c(lib().fipi_ensemble_step_async(self._h, nsteps))
+
+    def sync(self):
+        self._c(lib().fipi_ensemble_sync(self._h))
+
+    @property
+    def stream_ptr(self):
+        return lib().fipi_ensemble_stream(self._h)
+
+    def phi(self, b):
+        n = self.edges[b]
+        out = np.empty(n ** 3, np.float64)
+        self._c(lib().fipi_ensemble_get_phi(self._h, b, _dp(out)))
+        return out.reshape(n, n, n)
+
+    def particles(self, b):
+        out = np.zeros((max(self.counts[b], 1), 3), np.float64)
+        self._c(lib().fipi_ensemble_get_particles(self._h, b, _dp(out)))
+        return out[:self.counts[b]]
+
+    def diagnostics(self, b):
+        d = CDiag()
+        self._c(lib().fipi_ensemble_diagnostics(self._h, b, C.byref(d)))
+        return {k: getattr(d, k) for k, _ in CDiag._fields_}

@@ -38,44 +38,17 @@ struct Dims {
 
 constexpr int XT = 256;  // threads per x-pass workgroup (at most)
 
-// x-pass workgroup shape: LPW complex lines (2 real lines each) of P threads
+// x-pass workgroup: up to LPW complex lines (2 real lines each) of P
+// threads; a launch uses blockDim.x / P lines, a divisor of ny*nz/2, so
+// every line of every workgroup exists
 template <typename T, int N> struct XShape {
     static constexpr int LPW = XT / Line<T, N>::P;
-    static constexpr int THREADS = LPW * Line<T, N>::P;
 };
 
-#ifndef FIPI_TILE_BYTES_F64
-#define FIPI_TILE_BYTES_F64 128
-#endif
-#ifndef FIPI_TILE_BYTES_F32
-#define FIPI_TILE_BYTES_F32 128
-#endif
-#ifndef FIPI_HUB_TILE_BYTES_F64
-#define FIPI_HUB_TILE_BYTES_F64 128
-#endif
-#ifndef FIPI_HUB_TILE_BYTES_F32
-#define FIPI_HUB_TILE_BYTES_F32 128
-#endif
-// kx columns per y-pass workgroup: one 128-byte line of complex values
-template <typename T> struct Tile {
-    static constexpr int L = (sizeof(T) == 8 ? FIPI_TILE_BYTES_F64 : FIPI_TILE_BYTES_F32) / (int)sizeof(cplx<T>);
-};
-#ifndef FIPI_STOKES_TILE_BYTES_F64
-#define FIPI_STOKES_TILE_BYTES_F64 128
-#endif
-#ifndef FIPI_STOKES_TILE_BYTES_F32
-#define FIPI_STOKES_TILE_BYTES_F32 64
-#endif
-// kx columns per z-pass (hub) workgroup
-template <typename T> struct HubTile {
-    static constexpr int L =
-        (sizeof(T) == 8 ? FIPI_HUB_TILE_BYTES_F64 : FIPI_HUB_TILE_BYTES_F32) / (int)sizeof(cplx<T>);
-};
-// kx columns per Stokes hub workgroup (three fields in flight: the FP32
-// path runs half-width tiles, which doubles the workgroups per CU)
-template <typename T> struct StokesTile {
-    static constexpr int L =
-        (sizeof(T) == 8 ? FIPI_STOKES_TILE_BYTES_F64 : FIPI_STOKES_TILE_BYTES_F32) / (int)sizeof(cplx<T>);
+// kx columns per y/z-pass workgroup: a TB-byte run of complex values
+// (TB = 64 or 128, chosen per launch site and edge length)
+template <typename T, int TB> struct TileW {
+    static constexpr int L = TB / (int)sizeof(cplx<T>);
 };
 
 __device__ __forceinline__ size_t hidx(const Dims &d, int kx, int y, int z)
@@ -120,14 +93,14 @@ __device__ __forceinline__ void load_c2r(cplx<T> *v, const cplx<T> *__restrict__
 // rows rowA, rowB.
 template <typename T, int N>
 __device__ __forceinline__ void store_r2c(const cplx<T> *v, cplx<T> *L, cplx<T> *__restrict__ out, size_t rowA,
-                                          size_t rowB, int t, bool lok)
+                                          size_t rowB, int t)
 {
     constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
     using C = cplx<T>;
 #pragma unroll
     for (int q = 0; q < E; q++) L[t + q * P] = v[q];
     __syncthreads();
-    for (int k = t; lok && k <= N / 2; k += P) {
+    for (int k = t; k <= N / 2; k += P) {
         const C zk = L[k], zm = L[(N - k) % N];
         const C A{(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
         const C B{(zk.y + zm.y) * T(0.5), (zm.x - zk.x) * T(0.5)};
@@ -138,18 +111,14 @@ __device__ __forceinline__ void store_r2c(const cplx<T> *v, cplx<T> *L, cplx<T> 
 }
 
 // common x-pass geometry: complex line lp of the workgroup = real lines
-// rl, rl+1 of box blockIdx.z.  The last workgroup may be partial: its
-// surplus lines (lok false) load line 0, join every barrier and store
-// nothing.
+// rl, rl+1 of box blockIdx.z
 #define FIPI_XPASS_SETUP(T, N)                                               \
     constexpr int E = Line<T, N>::E, P = Line<T, N>::P, LL = Line<T, N>::LL; \
     constexpr int LPW = XShape<T, N>::LPW;                                   \
     using C = cplx<T>;                                                       \
     __shared__ C lds[LPW * LL];                                              \
     const int lp = threadIdx.x / P, t = threadIdx.x % P;                     \
-    const int rl0 = (blockIdx.x * LPW + lp) * 2;                             \
-    const bool lok = rl0 < d.ny * d.nz;                                      \
-    const int rl = lok ? rl0 : 0;                                            \
+    const int rl = (blockIdx.x * (blockDim.x / P) + lp) * 2;                 \
     const size_t zb = blockIdx.z;                                            \
     C *L = lds + lp * LL;                                                    \
     (void)E;                                                                 \
@@ -200,12 +169,11 @@ struct ProdCF {
 };
 
 // Fixed-order FP64 sum of one value per thread over the block (at most 256
-// threads, at least 128); the result is valid in thread 0.  `red` holds 256
-// doubles.
+// threads); the result is valid in thread 0.  `red` holds 256 doubles.
 __device__ __forceinline__ double block_sum_f64(double v, double *red)
 {
     red[threadIdx.x] = v;
-    if ((int)threadIdx.x < 256 - (int)blockDim.x) red[blockDim.x + threadIdx.x] = 0.0;
+    for (int i = blockDim.x + threadIdx.x; i < 256; i += blockDim.x) red[i] = 0.0;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
@@ -250,7 +218,7 @@ __global__ __launch_bounds__(XT) void k_x_forward(Prod prod, cplx<T> *__restrict
         for (int q = 0; q < E; q++) {
             const int x = t + q * P;
             v[q] = C{prod(f, zb * d.rs + (size_t)rl * d.nx + x), prod(f, zb * d.rs + (size_t)(rl + 1) * d.nx + x)};
-            if (lok) acc += (double)v[q].x + (double)v[q].y;
+            acc += (double)v[q].x + (double)v[q].y;
         }
         if (f == 0 && part != nullptr) {
             const double s = block_sum_f64(acc, reinterpret_cast<double *>(lds));
@@ -259,7 +227,7 @@ __global__ __launch_bounds__(XT) void k_x_forward(Prod prod, cplx<T> *__restrict
         }
         fft_dist<T, N, -1>(v, L, t, tw);
         store_r2c<T, N>(v, L, boxp(f == 0 ? out0 : out1, zb, d.hs), (size_t)rl * d.nxp, (size_t)(rl + 1) * d.nxp,
-                        t, lok);
+                        t);
     }
 }
 
@@ -280,13 +248,11 @@ __global__ __launch_bounds__(XT) void k_x_inverse(const cplx<T> *__restrict__ in
     for (int f = 0; f < NF; f++) {
         T *o = f == 0 ? o0 : (f == 1 ? o1 : o2);
         fft_dist<T, N, +1>(v[f], L, t, tw);
-        if (lok) {
 #pragma unroll
-            for (int q = 0; q < E; q++) {
-                const int x = t + q * P;
-                o[xA + x] = v[f][q].x;
-                o[xB + x] = v[f][q].y;
-            }
+        for (int q = 0; q < E; q++) {
+            const int x = t + q * P;
+            o[xA + x] = v[f][q].x;
+            o[xB + x] = v[f][q].y;
         }
     }
 }
@@ -331,10 +297,8 @@ __global__ __launch_bounds__(XT) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *_
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int x = t + q * P;
-            if (lok) {
-                u[xA + x] = v[f][q].x;
-                u[xB + x] = v[f][q].y;
-            }
+            u[xA + x] = v[f][q].x;
+            u[xB + x] = v[f][q].y;
             cv[q] = C{cv[q].x + v[f][q].x * ga[q], cv[q].y + v[f][q].y * gb[q]};
         }
     }
@@ -342,12 +306,12 @@ __global__ __launch_bounds__(XT) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *_
         double acc = 0.0;
 #pragma unroll
         for (int q = 0; q < E; q++) acc += (double)cv[q].x + (double)cv[q].y;
-        const double s = block_sum_f64(lok ? acc : 0.0, reinterpret_cast<double *>(lds));
+        const double s = block_sum_f64(acc, reinterpret_cast<double *>(lds));
         if (threadIdx.x == 0) part[zb * gridDim.x + blockIdx.x] = s;
         __syncthreads();
     }
     fft_dist<T, N, -1>(cv, L, t, tw);
-    store_r2c<T, N>(cv, L, h0, rA, rB, t, lok);
+    store_r2c<T, N>(cv, L, h0, rA, rB, t);
 }
 
 // P5: inverse x of (c_hat, dy c_hat, dz c_hat) -> c, grad c (real), then
@@ -369,25 +333,21 @@ __global__ __launch_bounds__(XT) void k_x_cgrad_f(const cplx<T> *__restrict__ ch
     load_c2r<T, N, false>(v[1], dyc, rA, rB, t, kxv);
     load_c2r<T, N, false>(v[2], dzc, rA, rB, t, kxv);
     fft_dist<T, N, +1>(cv, L, t, tw);
-    if (lok) {
 #pragma unroll
-        for (int q = 0; q < E; q++) {
-            const int x = t + q * P;
-            c[xA + x] = cv[q].x;
-            c[xB + x] = cv[q].y;
-        }
+    for (int q = 0; q < E; q++) {
+        const int x = t + q * P;
+        c[xA + x] = cv[q].x;
+        c[xB + x] = cv[q].y;
     }
 #pragma unroll
     for (int f = 0; f < 3; f++) {
         T *g = f == 0 ? gx : (f == 1 ? gy : gz);
         fft_dist<T, N, +1>(v[f], L, t, tw);
-        if (lok) {
 #pragma unroll
-            for (int q = 0; q < E; q++) {
-                const int x = t + q * P;
-                g[xA + x] = v[f][q].x;
-                g[xB + x] = v[f][q].y;
-            }
+        for (int q = 0; q < E; q++) {
+            const int x = t + q * P;
+            g[xA + x] = v[f][q].x;
+            g[xB + x] = v[f][q].y;
         }
     }
 #pragma unroll
@@ -396,7 +356,7 @@ __global__ __launch_bounds__(XT) void k_x_cgrad_f(const cplx<T> *__restrict__ ch
         cv[q] = C{(a * a * a - a) * fscale, (b * b * b - b) * fscale};
     }
     fft_dist<T, N, -1>(cv, L, t, tw);
-    store_r2c<T, N>(cv, L, fout, rA, rB, t, lok);
+    store_r2c<T, N>(cv, L, fout, rA, rB, t);
 }
 
 template <typename T>
@@ -488,7 +448,7 @@ __global__ __launch_bounds__(XT) void k_x_force(cplx<T> *__restrict__ mu_io /* i
             }
         }
         fft_dist<T, N, -1>(v, L, t, tw);
-        store_r2c<T, N>(v, L, f == 0 ? mu_io : (f == 1 ? f1 : f2), rA, rB, t, lok);
+        store_r2c<T, N>(v, L, f == 0 ? mu_io : (f == 1 ? f1 : f2), rA, rB, t);
     }
 }
 
@@ -496,9 +456,9 @@ __global__ __launch_bounds__(XT) void k_x_force(cplx<T> *__restrict__ mu_io /* i
 // y and z passes: a tile of TL neighbouring kx columns; lane = column,
 // t = position in the line (distribution D along y or z).
 // ---------------------------------------------------------------------------
-#define FIPI_TILE_SETUP(T, N, TILE)                                   \
+#define FIPI_TILE_SETUP(T, N, TLV)                                   \
     constexpr int E = Line<T, N>::E, P = Line<T, N>::P, LL = Line<T, N>::LL;   \
-    constexpr int TL = TILE<T>::L;                                    \
+    constexpr int TL = TLV;                                           \
     using C = cplx<T>;                                                \
     __shared__ C lds[TL * LL];                                        \
     const int lane = threadIdx.x % TL, t = threadIdx.x / TL;          \
@@ -530,13 +490,13 @@ __device__ __forceinline__ void tile_store(const cplx<T> *v, cplx<T> *__restrict
 
 // y pass: transforms along y of NF fields in place.  GRAD (inverse only):
 // field 0 also yields i*ky*field0 into `extra` (zero on ky = NY/2).
-template <typename T, int N, int SIGN, int NF, bool GRAD>
-__global__ __launch_bounds__((Tile<T>::L * Line<T, N>::P)) void k_y(cplx<T> *__restrict__ a0, cplx<T> *__restrict__ a1,
+template <typename T, int N, int SIGN, int NF, bool GRAD, int TB>
+__global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_y(cplx<T> *__restrict__ a0, cplx<T> *__restrict__ a1,
                                                               cplx<T> *__restrict__ a2, cplx<T> *__restrict__ extra,
                                                               Dims d, const double *__restrict__ kyv,
                                                               const cplx<T> *__restrict__ tw)
 {
-    FIPI_TILE_SETUP(T, N, Tile)
+    FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
     const int z = blockIdx.y;
     const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, 0, z);
     const size_t st = d.nxp;
@@ -601,14 +561,14 @@ struct K0 {
 //   c^1 = (c^0 - dtM k^2 f^0 - dt conv^0) / (1 + dtM lambda k^4)  (:106-113).
 // Outputs c^{n+1} (K), the new history (K), and c^{n+1}/G, i kz c^{n+1}/G
 // inverse-z transformed (XY) for the gradient and real-space c.
-template <typename T, int N>
-__global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_ch(
+template <typename T, int N, int TB>
+__global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_ch(
     cplx<T> *__restrict__ conv_io /* in conv XY, out c XY */, cplx<T> *__restrict__ dzc_out,
     const cplx<T> *__restrict__ c_now, cplx<T> *__restrict__ c_next, const cplx<T> *__restrict__ f_now,
     cplx<T> *__restrict__ hist, KParams kp, Dims d, const double *__restrict__ kxv, const double *__restrict__ kyv,
     const double *__restrict__ kzv, const cplx<T> *__restrict__ tw, K0 k0)
 {
-    FIPI_TILE_SETUP(T, N, HubTile)
+    FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
     const int y = blockIdx.y;
     const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
@@ -693,15 +653,15 @@ __global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_ch(
 }
 
 // Initial hub: (c XY, f XY) -> c_hat^0, f_hat^0 (K) and c/G, i kz c/G (XY).
-template <typename T, int N>
-__global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_init(cplx<T> *__restrict__ c_io,
+template <typename T, int N, int TB>
+__global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_init(cplx<T> *__restrict__ c_io,
                                                                      cplx<T> *__restrict__ f_io /* out: dz c XY */,
                                                                      cplx<T> *__restrict__ c_k,
                                                                      cplx<T> *__restrict__ f_k, KParams kp, Dims d,
                                                                      const double *__restrict__ kzv,
                                                                      const cplx<T> *__restrict__ tw)
 {
-    FIPI_TILE_SETUP(T, N, HubTile)
+    FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
     const int y = blockIdx.y;
     const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
@@ -728,8 +688,8 @@ __global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_init(cp
 
 // P7: f (XY) -> f_hat^{n+1} (K); mu_hat = lambda k^2 c_hat + f_hat
 // (src/main.cpp:157) -> mu/G (XY, in place of f).
-template <typename T, int N>
-__global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_mu(cplx<T> *__restrict__ f_io,
+template <typename T, int N, int TB>
+__global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_mu(cplx<T> *__restrict__ f_io,
                                                                    cplx<T> *__restrict__ f_k,
                                                                    const cplx<T> *__restrict__ c_k, KParams kp,
                                                                    Dims d, const double *__restrict__ kxv,
@@ -737,7 +697,7 @@ __global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_mu(cplx
                                                                    const double *__restrict__ kzv,
                                                                    const cplx<T> *__restrict__ tw)
 {
-    FIPI_TILE_SETUP(T, N, HubTile)
+    FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
     const int y = blockIdx.y;
     const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
@@ -771,8 +731,8 @@ __global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_hub_mu(cplx
 // Nyquist index (k_d(N/2) = +pi/dx on both sides).  The real part equals the
 // inverse of the Hermitian part, which drops the k_d k_e cross terms
 // between Nyquist and non-Nyquist components; that is what is formed here.
-template <typename T, int N>
-__global__ __launch_bounds__((StokesTile<T>::L * Line<T, N>::P)) void k_hub_stokes(cplx<T> *__restrict__ a0,
+template <typename T, int N, int TB>
+__global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_stokes(cplx<T> *__restrict__ a0,
                                                                        cplx<T> *__restrict__ a1,
                                                                        cplx<T> *__restrict__ a2, KParams kp, Dims d,
                                                                        const double *__restrict__ kxv,
@@ -780,7 +740,7 @@ __global__ __launch_bounds__((StokesTile<T>::L * Line<T, N>::P)) void k_hub_stok
                                                                        const double *__restrict__ kzv,
                                                                        const cplx<T> *__restrict__ tw)
 {
-    FIPI_TILE_SETUP(T, N, StokesTile)
+    FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
     const int y = blockIdx.y;
     const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;
@@ -844,11 +804,11 @@ __global__ __launch_bounds__((StokesTile<T>::L * Line<T, N>::P)) void k_hub_stok
 }
 
 // Plain forward / inverse z transform of one field in place (unit entries).
-template <typename T, int N, int SIGN>
-__global__ __launch_bounds__((HubTile<T>::L * Line<T, N>::P)) void k_z(cplx<T> *__restrict__ a, Dims d,
+template <typename T, int N, int SIGN, int TB>
+__global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_z(cplx<T> *__restrict__ a, Dims d,
                                                               const cplx<T> *__restrict__ tw)
 {
-    FIPI_TILE_SETUP(T, N, HubTile)
+    FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
     const int y = blockIdx.y;
     const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
     const size_t st = (size_t)d.nxp * d.ny;

@@ -51,6 +51,14 @@ bool with_n(int n, F &&f)
     }
 }
 
+// tile width in bytes (64 or 128) as a compile-time constant
+template <class F>
+void with_tb(int tb, F &&f)
+{
+    if (tb == 64) f(std::integral_constant<int, 64>{});
+    else f(std::integral_constant<int, 128>{});
+}
+
 bool supported_n(int n)
 {
     return n == 16 || n == 32 || n == 48 || n == 64 || n == 80 || n == 96 || n == 128 || n == 256;
@@ -67,7 +75,13 @@ struct fipi_ctx {
     bool has_state = false;
     int t = 0;
     int *t_dev = nullptr;  // device copy of t (read by kernels a captured graph replays)
-    int nbox = 1;          // equal boxes in the slab; G, GH and n_box_particles are per box
+    int nbox = 1;          // equal boxes in the slab; G and GH are per box
+    // tile widths (bytes) of the y passes, the CH/mu/init hubs and the
+    // Stokes hub; results do not depend on them (every line sees the same
+    // arithmetic), only the speed does
+    struct Tune {
+        int y_tb = 128, hub_tb = 128, stokes_tb = 128;
+    } tune;
     Dims d{};
     size_t G = 0, GH = 0;
     size_t esz = 8;  // bytes per real element
@@ -194,13 +208,27 @@ struct ProfScope {
 // at least one block: every kernel bounds-checks its index
 inline dim3 grid1(size_t n, int b) { return dim3((unsigned)(n == 0 ? 1 : (n + b - 1) / b)); }
 
-// x-pass grid: LPW complex lines (2 real lines each) per workgroup, the
-// last one possibly partial; one slab of workgroups per box
+// x-pass lines per workgroup: the largest divisor of ny*nz/2 (complex
+// lines, 2 real lines each) that fits in XT threads
+template <typename T, int N>
+int xlines(const Dims &d)
+{
+    const int cl = d.ny * d.nz / 2;
+    int l = XShape<T, N>::LPW;
+    while (cl % l != 0) l--;
+    return l;
+}
+
+// x-pass grid (one slab of workgroups per box) and workgroup
 template <typename T, int N>
 dim3 xgrid(const Dims &d, int nbox)
 {
-    constexpr int LPW = XShape<T, N>::LPW;
-    return dim3((unsigned)((d.ny * d.nz / 2 + LPW - 1) / LPW), 1, (unsigned)nbox);
+    return dim3((unsigned)(d.ny * d.nz / 2 / xlines<T, N>(d)), 1, (unsigned)nbox);
+}
+template <typename T, int N>
+dim3 xblock(const Dims &d)
+{
+    return dim3((unsigned)(xlines<T, N>(d) * Line<T, N>::P));
 }
 
 template <typename T>
@@ -261,7 +289,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_forward<T, N, NF, Prod>), (xgrid<T, N>(d, ctx->nbox)), dim3(XShape<T, N>::THREADS), 0, ctx->stream, prod, o0, o1, d,
+            hipLaunchKernelGGL((k_x_forward<T, N, NF, Prod>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, ctx->stream, prod, o0, o1, d,
                                (const C *)ctx->twx, part);
         });
     }
@@ -272,7 +300,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_inverse<T, N, NF>), (xgrid<T, N>(d, ctx->nbox)), dim3(XShape<T, N>::THREADS), 0, ctx->stream, i0, i1, i2, o0, o1, o2, d,
+            hipLaunchKernelGGL((k_x_inverse<T, N, NF>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, ctx->stream, i0, i1, i2, o0, o1, o2, d,
                                (const C *)ctx->twx);
         });
     }
@@ -283,17 +311,27 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.ny, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD>),
-                               dim3((unsigned)((d.nxh + Tile<T>::L - 1) / Tile<T>::L), d.nz, ctx->nbox),
-                               dim3((Tile<T>::L * Line<T, N>::P)), 0, ctx->stream, a0, a1, a2, extra, d, ctx->ky,
-                               (const C *)ctx->twy);
+            with_tb(ctx->tune.y_tb, [&](auto TBC) {
+                constexpr int TB = decltype(TBC)::value, TL = TileW<T, TB>::L;
+                hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD, TB>),
+                                   dim3((unsigned)((d.nxh + TL - 1) / TL), d.nz, ctx->nbox),
+                                   dim3((TL * Line<T, N>::P)), 0, ctx->stream, a0, a1, a2, extra, d, ctx->ky,
+                                   (const C *)ctx->twy);
+            });
         });
     }
 
-    template <template <typename> class TileT = HubTile>
+    // z-pass (hub) grid and workgroup for tile width TB
+    template <int TB>
     static dim3 zgrid(const fipi_ctx *ctx)
     {
-        return dim3((unsigned)((ctx->d.nxh + TileT<T>::L - 1) / TileT<T>::L), ctx->d.ny, ctx->nbox);
+        constexpr int TL = TileW<T, TB>::L;
+        return dim3((unsigned)((ctx->d.nxh + TL - 1) / TL), ctx->d.ny, ctx->nbox);
+    }
+    template <int TB, int N>
+    static dim3 zblock()
+    {
+        return dim3(TileW<T, TB>::L * Line<T, N>::P);
     }
 
     template <int SIGN>
@@ -302,8 +340,11 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nz, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_z<T, N, SIGN>), zgrid(ctx), dim3((HubTile<T>::L * Line<T, N>::P)), 0, ctx->stream, a, d,
-                               (const C *)ctx->twz);
+            with_tb(ctx->tune.hub_tb, [&](auto TBC) {
+                constexpr int TB = decltype(TBC)::value;
+                hipLaunchKernelGGL((k_z<T, N, SIGN, TB>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, ctx->stream, a, d,
+                                   (const C *)ctx->twz);
+            });
         });
     }
 
@@ -317,9 +358,12 @@ struct Impl {
         const KParams kp = kparams(ctx);
         with_n(d.nz, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_hub_init<T, N>), zgrid(ctx), dim3((HubTile<T>::L * Line<T, N>::P)), 0, ctx->stream,
-                               H(ctx->h[0]), H(ctx->h[1]), H(ctx->ck[ctx->cur]), H(ctx->fk), kp, d, ctx->kz,
-                               (const C *)ctx->twz);
+            with_tb(ctx->tune.hub_tb, [&](auto TBC) {
+                constexpr int TB = decltype(TBC)::value;
+                hipLaunchKernelGGL((k_hub_init<T, N, TB>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, ctx->stream,
+                                   H(ctx->h[0]), H(ctx->h[1]), H(ctx->ck[ctx->cur]), H(ctx->fk), kp, d, ctx->kz,
+                                   (const C *)ctx->twz);
+            });
         });
         y_pass<+1, 2, true>(ctx, H(ctx->h[0]), H(ctx->h[1]), nullptr, H(ctx->h[2]));
         cgrad_f(ctx);
@@ -330,7 +374,7 @@ struct Impl {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_cgrad_f<T, N>), (xgrid<T, N>(d, ctx->nbox)), dim3(XShape<T, N>::THREADS), 0, ctx->stream, H(ctx->h[0]), H(ctx->h[2]),
+            hipLaunchKernelGGL((k_x_cgrad_f<T, N>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, ctx->stream, H(ctx->h[0]), H(ctx->h[2]),
                                H(ctx->h[1]), R(ctx->c), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), H(ctx->h[0]),
                                fscale(ctx), d, ctx->kx, (const C *)ctx->twx);
         });
@@ -466,10 +510,13 @@ struct Impl {
             ProfScope ps_(ctx, "z_hub_ch", (ns ? 8 : 7) * Hb);
             with_n(d.nz, [&](auto NC) {
                 constexpr int N = decltype(NC)::value;
-                hipLaunchKernelGGL((k_hub_ch<T, N>), zgrid(ctx), dim3((HubTile<T>::L * Line<T, N>::P)), 0, st, h0, h1,
-                                   (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]), (const C *)ctx->fk,
-                                   H(ctx->hist), kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz,
-                                   K0{mixed ? ctx->k0 : nullptr, ctx->conv0, ctx->cur});
+                with_tb(ctx->tune.hub_tb, [&](auto TBC) {
+                    constexpr int TB = decltype(TBC)::value;
+                    hipLaunchKernelGGL((k_hub_ch<T, N, TB>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, st, h0, h1,
+                                       (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]), (const C *)ctx->fk,
+                                       H(ctx->hist), kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz,
+                                       K0{mixed ? ctx->k0 : nullptr, ctx->conv0, ctx->cur});
+                });
             });
         }
         ctx->cur = nxt;
@@ -489,9 +536,12 @@ struct Impl {
             ProfScope ps_(ctx, "z_hub_mu", 4 * Hb);
             with_n(d.nz, [&](auto NC) {
                 constexpr int N = decltype(NC)::value;
-                hipLaunchKernelGGL((k_hub_mu<T, N>), zgrid(ctx), dim3((HubTile<T>::L * Line<T, N>::P)), 0, st, h0,
-                                   H(ctx->fk), (const C *)ctx->ck[ctx->cur], kp, d, ctx->kx, ctx->ky, ctx->kz,
-                                   (const C *)ctx->twz);
+                with_tb(ctx->tune.hub_tb, [&](auto TBC) {
+                    constexpr int TB = decltype(TBC)::value;
+                    hipLaunchKernelGGL((k_hub_mu<T, N, TB>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, st, h0,
+                                       H(ctx->fk), (const C *)ctx->ck[ctx->cur], kp, d, ctx->kx, ctx->ky, ctx->kz,
+                                       (const C *)ctx->twz);
+                });
             });
         }
         if (ns) {
@@ -507,7 +557,7 @@ struct Impl {
                 ProfScope ps_(ctx, "x_inv_force_fwd", 4 * Hb + 4 * Rb + (parts ? 8.0 * ctx->n * sizeof(SpreadEntry<T>) : 0.0));
                 with_n(d.nx, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_x_force<T, N>), (xgrid<T, N>(d, ctx->nbox)), dim3(XShape<T, N>::THREADS), 0, st, h0, h1, h2, (const T *)R(ctx->c),
+                    hipLaunchKernelGGL((k_x_force<T, N>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, st, h0, h1, h2, (const T *)R(ctx->c),
                                        (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
                                        (const int *)ctx->line_start, (const int *)ctx->node_cnt,
                                        (const SpreadNode<T> *)ctx->spr_tmp, parts ? 1 : 0, (T)ctx->p.rho1,
@@ -522,8 +572,11 @@ struct Impl {
                 ProfScope ps_(ctx, "z_hub_stokes", 6 * Hb);
                 with_n(d.nz, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_hub_stokes<T, N>), zgrid<StokesTile>(ctx), dim3((StokesTile<T>::L * Line<T, N>::P)), 0, st, h0,
-                                       h1, h2, kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
+                    with_tb(ctx->tune.stokes_tb, [&](auto TBC) {
+                        constexpr int TB = decltype(TBC)::value;
+                        hipLaunchKernelGGL((k_hub_stokes<T, N, TB>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, st, h0,
+                                           h1, h2, kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
+                    });
                 });
             }
             {
@@ -538,7 +591,7 @@ struct Impl {
                 ProfScope ps_(ctx, "x_inv_u_conv", 4 * Hb + 6 * Rb);
                 with_n(d.nx, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_x_inv_conv<T, N>), (xgrid<T, N>(d, ctx->nbox)), dim3(XShape<T, N>::THREADS), 0, st, h0, (const C *)h1,
+                    hipLaunchKernelGGL((k_x_inv_conv<T, N>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, st, h0, (const C *)h1,
                                        (const C *)h2, R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]),
                                        (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]), d,
                                        (const C *)ctx->twx, mixed ? ctx->conv_part : nullptr);
@@ -681,6 +734,9 @@ int setup_buffers(fipi_ctx *ctx)
 {
     const fipi_params &p = ctx->p;
     ctx->esz = sizeof(T);
+    // FP32: half-width Stokes tiles (three fields in flight) double the
+    // workgroups per CU
+    if (sizeof(T) == 4) ctx->tune.stokes_tb = 64;
     const size_t nb = (size_t)ctx->nbox;
     const size_t G = ctx->G * nb, GH = ctx->GH * nb;  // whole slab
     int rc;
@@ -896,7 +952,7 @@ extern "C" {
 }  // extern "C"
 
 // A context of nbox equal boxes (the slab) with n_total particles in all.
-int create_ctx(const fipi_params *params, int nbox, int n_total, fipi_ctx **out)
+static int create_ctx(const fipi_params *params, int nbox, int n_total, fipi_ctx **out)
 {
     if (!out) return set_err(nullptr, FIPI_ERR_ARG, "out is NULL");
     *out = nullptr;
@@ -963,8 +1019,8 @@ void *fipi_stream(fipi_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 // State of the whole slab: phi[nbox*G] (box-major), n particles (pos[3n])
 // with their boxes (box may be null for one box).
-int set_state_slab(fipi_ctx *ctx, const double *phi, const double *pos, const double *radius,
-                   const double *cos_theta, const int *box)
+static int set_state_slab(fipi_ctx *ctx, const double *phi, const double *pos, const double *radius,
+                          const double *cos_theta, const int *box)
 {
     if (!ctx || !phi) return set_err(ctx, FIPI_ERR_ARG, "ctx and phi are required");
     if (ctx->n > 0 && !pos) return set_err(ctx, FIPI_ERR_ARG, "pos is required when particles are on");
@@ -1125,7 +1181,7 @@ int fipi_get_time(fipi_ctx *ctx, int32_t *t)
 }  // extern "C"
 
 // Diagnostics of box b; its particles are [p0, p1) of the slab's list.
-int diag_box(fipi_ctx *ctx, int b, int p0, int p1, fipi_diag *out)
+static int diag_box(fipi_ctx *ctx, int b, int p0, int p1, fipi_diag *out)
 {
     if (!ctx || !out) return set_err(ctx, FIPI_ERR_ARG, "ctx and out are required");
     if (b < 0 || b >= ctx->nbox) return set_err(ctx, FIPI_ERR_ARG, "box out of range");
@@ -1394,3 +1450,5 @@ int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32
 }
 
 }  // extern "C"
+
+#include "ensemble.hpp"

@@ -41,7 +41,7 @@ extern "C" {
 /* Runtime parameters.  The reference holds these as compile-time constants
  * (src/head.hpp:27-74, src/parameter.hpp:18-20); field names follow it. */
 typedef struct fipi_params {
-    int32_t nx, ny, nz;         /* grid, head.hpp:29-31 (each 16, 32, 64, 128 or 256) */
+    int32_t nx, ny, nz;         /* grid, head.hpp:29-31 (each 16, 32, 48, 64, 80, 96, 128 or 256) */
     int32_t n_particles;        /* head.hpp:63 */
     double lx, ly, lz;          /* box, head.hpp:33-35 */
     double pi;                  /* head.hpp:27: the reference uses 3.1415926 */
@@ -167,6 +167,46 @@ int fipi_interp(fipi_ctx *ctx, const void *d_grid, const double *d_pos, int32_t 
  * cluster::compute_f_r (cluster.cpp:344-410).  Deterministic: per grid node
  * the contributions are summed in ascending particle order. */
 int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32_t n, void *d_grid3);
+
+/* ---- batched ensemble of independent boxes ---------------------------- */
+/* No counterpart in the reference, which runs one box per process
+ * (src/main.cpp:19-252).  Many independent periodic cubic boxes of mixed
+ * edge lengths advance together: boxes of equal edge share one slab, every
+ * pass is one launch over all boxes of a slab, and a whole ensemble step is
+ * one replay of a captured HIP graph (after the first, first-order step).
+ * Physics parameters are common; each box has its own state. */
+typedef struct fipi_ensemble fipi_ensemble;
+typedef struct fipi_box_desc {
+    int32_t n;            /* cells per edge (16, 32, 48, 64, 80, 96, 128, 256) */
+    int32_t n_particles;  /* spheres in this box */
+} fipi_box_desc;
+/* common: physics, precision and switches; its grid spacing lx/nx is the
+ * spacing of every box (box edge = n * lx/nx); nx..nz and n_particles are
+ * ignored.  Boxes are numbered 0..n_boxes-1 in the order given. */
+int fipi_ensemble_create(const fipi_params *common, int32_t n_boxes, const fipi_box_desc *boxes,
+                         fipi_ensemble **out);
+int fipi_ensemble_destroy(fipi_ensemble *ens);
+const char *fipi_ensemble_last_error(const fipi_ensemble *ens);
+/* Initial state of one box (host arrays: phi[n^3], pos[3*n_particles],
+ * radius and cos_theta may be NULL).  Every box must be set before the
+ * first step; setting a box again restarts the whole ensemble at t = 0. */
+int fipi_ensemble_set_box(fipi_ensemble *ens, int32_t box, const double *phi, const double *pos,
+                          const double *radius, const double *cos_theta);
+/* Pick the y-pass, hub and Stokes-hub tile widths of each edge length by
+ * timing `steps` steps per candidate, then restart from the initial state.
+ * Results do not depend on the choice. */
+int fipi_ensemble_tune(fipi_ensemble *ens, int32_t steps);
+/* Advance every box nsteps steps (one graph replay per step); _async
+ * leaves the error check to fipi_ensemble_sync. */
+int fipi_ensemble_step(fipi_ensemble *ens, int32_t nsteps);
+int fipi_ensemble_step_async(fipi_ensemble *ens, int32_t nsteps);
+int fipi_ensemble_sync(fipi_ensemble *ens);
+void *fipi_ensemble_stream(fipi_ensemble *ens);
+int fipi_ensemble_get_phi(fipi_ensemble *ens, int32_t box, double *phi);
+int fipi_ensemble_get_particles(fipi_ensemble *ens, int32_t box, double *pos);
+int fipi_ensemble_diagnostics(fipi_ensemble *ens, int32_t box, fipi_diag *out);
+/* Tile widths chosen for edge n (bytes; 0 when no box has that edge). */
+int fipi_ensemble_tuning(const fipi_ensemble *ens, int32_t n, int32_t *y_tb, int32_t *hub_tb, int32_t *stokes_tb);
 
 /* ---- seeded synthetic inputs (host only, no GPU needed) ---------------- */
 /* phi[i] = mean + U(-amp, amp) from a counter-based generator. */
