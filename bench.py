@@ -17,6 +17,13 @@ composition U(-0.3, 0.3) + U(-0.05, 0.05) noise, U(0, 2000) spheres of
 radius 2 dx per box (capped where random sequential addition jams),
 FP32-mixed, advanced together (one graph replay per step), against the
 same boxes stepped one at a time through the single-box path.
+
+And the calibration adjoint ("adjoint"): 128^3 box, 20,000 spheres of
+radius 2 dx held fixed, 200 steps; target from a seeded forward run with
+M = 1.3, kappa = 0.8 and contact angles U(70, 110) deg; loss = sum of
+squared phi mismatch at the last step; gradients in M, kappa and every
+contact angle from the hand-written adjoint (checkpoints every 20 steps),
+FP32-mixed and FP64, checked against FP64 central differences (h = 1e-4).
 """
 from __future__ import annotations
 
@@ -131,6 +138,73 @@ def ensemble_run(nboxes, steps, loop_sample, loop_steps):
     }
 
 
+def adjoint_run(n=128, npart=20000, steps=200, every=20, wetting=0.5, seed=21):
+    """Gradient timing and checks of the calibration adjoint."""
+    import fipi_amd
+    from fipi_amd import FP32_MIXED, FP64
+    phi0 = fipi_amd.init_phi((n, n, n), 0.0, 0.05, seed)
+    pos = fipi_amd.init_rsa(npart, 2.0, 0.0, (n, n, n), seed + 1)
+    th_true = np.deg2rad(fipi_amd.init_uniform(pos.shape[0], 70.0, 110.0, seed + 2))
+    npart = pos.shape[0]
+    th0 = np.full(npart, np.pi / 2)
+
+    def make(prec):
+        p = fipi_amd.config_params(n, npart, 2.0, steps, precision=prec)
+        return fipi_amd.Adjoint(p, phi0, pos, wetting, steps, every)
+
+    a32 = make(FP32_MIXED)
+    _, target = a32.forward(1.3, 0.8, th_true, want_loss=False)
+    a32.set_target(target)
+    t0 = time.perf_counter()
+    a32.forward(1.0, 1.0, th0)
+    fwd32 = time.perf_counter() - t0
+    a32.gradient(1.0, 1.0, th0)  # first use of the backward kernels
+    t0 = time.perf_counter()
+    l32, m32, k32, g32 = a32.gradient(1.0, 1.0, th0)
+    grad32 = time.perf_counter() - t0
+    a32.close()
+    a64 = make(FP64)
+    a64.set_target(target)
+    t0 = time.perf_counter()
+    l64, m64, k64, g64 = a64.gradient(1.0, 1.0, th0)
+    grad64 = time.perf_counter() - t0
+    h = 1e-4
+
+    def loss(mm, kk, tt):
+        return a64.forward(mm, kk, tt)[0]
+    fd_m = (loss(1.0 + h, 1.0, th0) - loss(1.0 - h, 1.0, th0)) / (2 * h)
+    fd_k = (loss(1.0, 1.0 + h, th0) - loss(1.0, 1.0 - h, th0)) / (2 * h)
+    fd_t = []
+    for i in np.argsort(-np.abs(g64))[:3]:
+        tp, tm = th0.copy(), th0.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd_t.append((int(i), float(g64[i]), (loss(1.0, 1.0, tp) - loss(1.0, 1.0, tm)) / (2 * h)))
+    a64.close()
+    gs = float(np.abs(g64).max())
+    return {
+        "workload": f"{n}^3, {npart} spheres r=2dx fixed, {steps} steps, checkpoints every {every}, "
+                    f"wetting a_w={wetting}; target M=1.3 kappa=0.8 theta~U(70,110) deg; gradient at "
+                    f"M=1 kappa=1 theta=90 deg",
+        "loss_fp64": l64,
+        "forward_ms_fp32_mixed": round(fwd32 * 1e3, 2),
+        "gradient_ms_fp32_mixed": round(grad32 * 1e3, 2),
+        "gradient_ms_fp64": round(grad64 * 1e3, 2),
+        "grad_fp64": {"mobility": m64, "kappa": k64, "theta_norm": float(np.linalg.norm(g64))},
+        "fd_rel_err_fp64": {
+            "mobility": abs(m64 - fd_m) / abs(fd_m),
+            "kappa": abs(k64 - fd_k) / abs(fd_k),
+            "theta_top3_vs_max_grad": max(abs(a - b) for _, a, b in fd_t) / gs,
+        },
+        "mixed_vs_fp64_rel": {
+            "loss": abs(l32 - l64) / l64,
+            "mobility": abs(m32 - m64) / abs(m64),
+            "kappa": abs(k32 - k64) / abs(k64),
+            "theta": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+        },
+    }
+
+
 def step_bytes(p, n_particles):
     """Algorithmic HBM bytes of one step with the minimal dataflow
     (every array read or written once per stage): 20 real fields + 18 half
@@ -181,6 +255,7 @@ def main():
     ap.add_argument("--ensemble-loop-sample", type=int, default=64)
     ap.add_argument("--ensemble-loop-steps", type=int, default=10)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -307,6 +382,8 @@ def main():
         ens = ensemble_run(args.ensemble_boxes, args.ensemble_steps, args.ensemble_loop_sample,
                            args.ensemble_loop_steps)
 
+    adj = None if args.no_adjoint else adjoint_run()
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, phi0, pos, cos_theta, args.cpu_steps)
@@ -349,6 +426,7 @@ def main():
         "kernels_other_precision": kernel_table(summarize(other_run.get("stats", []))) if other_run else None,
         "diagnostics": {k: diag[k] for k in ("t", "mean_phi", "free_energy", "interface_area")},
         "ensemble": ens,
+        "adjoint": adj,
     }
     print(json.dumps(out), flush=True)
 

@@ -17,7 +17,7 @@ from .params import FP32_MIXED, FP64, Params, config_params, reference_params
 
 __all__ = [
     "FP64", "FP32_MIXED", "Params", "config_params", "reference_params",
-    "Simulation", "Ensemble", "FipiError", "lib", "LIB_PATH", "init_phi", "init_rsa", "init_uniform",
+    "Simulation", "Ensemble", "Adjoint", "FipiError", "lib", "LIB_PATH", "init_phi", "init_rsa", "init_uniform",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -40,6 +40,8 @@ ABI_SYMBOLS = [
     "fipi_ensemble_tune", "fipi_ensemble_step", "fipi_ensemble_step_async", "fipi_ensemble_sync",
     "fipi_ensemble_stream", "fipi_ensemble_get_phi", "fipi_ensemble_get_particles",
     "fipi_ensemble_diagnostics", "fipi_ensemble_tuning",
+    "fipi_adjoint_create", "fipi_adjoint_destroy", "fipi_adjoint_last_error", "fipi_adjoint_set_target",
+    "fipi_adjoint_forward", "fipi_adjoint_gradient",
     "fipi_init_phi", "fipi_init_rsa", "fipi_init_uniform",
 ]
 
@@ -145,6 +147,13 @@ def lib():
     L.fipi_ensemble_get_particles.argtypes = [vp, C.c_int32, dp]
     L.fipi_ensemble_diagnostics.argtypes = [vp, C.c_int32, C.POINTER(CDiag)]
     L.fipi_ensemble_tuning.argtypes = [vp, C.c_int32, ip, ip, ip]
+    L.fipi_adjoint_create.argtypes = [C.POINTER(CParams), dp, dp, C.c_double, C.c_int32, C.c_int32, C.POINTER(vp)]
+    L.fipi_adjoint_destroy.argtypes = [vp]
+    L.fipi_adjoint_last_error.argtypes = [vp]
+    L.fipi_adjoint_last_error.restype = C.c_char_p
+    L.fipi_adjoint_set_target.argtypes = [vp, dp]
+    L.fipi_adjoint_forward.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp]
+    L.fipi_adjoint_gradient.argtypes = [vp, C.c_double, C.c_double, dp, dp, dp, dp, dp]
     _lib = L
     return L
 
@@ -457,3 +466,70 @@ class Ensemble:
         d = CDiag()
         self._c(lib().fipi_ensemble_diagnostics(self._h, b, C.byref(d)))
         return {k: getattr(d, k) for k, _ in CDiag._fields_}
+
+
+class Adjoint:
+    """Calibration problem with a hand-written adjoint (a fipi_adjoint):
+    particles held at `pos`, wetting strength `wetting` (a_w), loss
+    sum (phi_T - phi_target)^2 after `steps` steps from phi0, forward state
+    checkpointed every `checkpoint_every` steps."""
+
+    def __init__(self, params: Params, phi0, pos, wetting: float, steps: int, checkpoint_every: int = 20):
+        self.params = params
+        self.n = params.n_particles if params.switch_particle else 0
+        phi0 = np.ascontiguousarray(phi0, dtype=np.float64).reshape(-1)
+        assert phi0.size == params.G
+        if self.n:
+            pos = np.ascontiguousarray(pos, dtype=np.float64)
+            assert pos.shape == (self.n, 3)
+        self._h = C.c_void_p()
+        self._cp = _to_cparams(params)
+        rc = lib().fipi_adjoint_create(C.byref(self._cp), _dp(phi0), _dp(pos) if self.n else None, wetting, steps,
+                                       checkpoint_every, C.byref(self._h))
+        if rc != 0:
+            raise FipiError(f"{ERRORS.get(rc, rc)}: {lib().fipi_adjoint_last_error(None).decode()}")
+
+    def _c(self, rc):
+        if rc != 0:
+            msg = lib().fipi_adjoint_last_error(self._h)
+            raise FipiError(f"{ERRORS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().fipi_adjoint_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _theta(self, theta):
+        if not self.n:
+            return None
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        assert th.shape == (self.n,)
+        return th
+
+    def set_target(self, phi):
+        phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
+        assert phi.size == self.params.G
+        self._c(lib().fipi_adjoint_set_target(self._h, _dp(phi)))
+
+    def forward(self, mobility, lam, theta, want_loss=True):
+        """(loss or None, final phi) at the given parameters."""
+        p = self.params
+        out = np.empty(p.G, np.float64)
+        loss = np.zeros(1)
+        self._c(lib().fipi_adjoint_forward(self._h, mobility, lam, _dp(self._theta(theta)),
+                                           _dp(loss) if want_loss else None, _dp(out)))
+        return (float(loss[0]) if want_loss else None), out.reshape(p.nz, p.ny, p.nx)
+
+    def gradient(self, mobility, lam, theta):
+        """(loss, dloss/dmobility, dloss/dlambda, dloss/dtheta)."""
+        loss, gm, gl = np.zeros(1), np.zeros(1), np.zeros(1)
+        gth = np.zeros(max(self.n, 1))
+        self._c(lib().fipi_adjoint_gradient(self._h, mobility, lam, _dp(self._theta(theta)), _dp(loss), _dp(gm),
+                                            _dp(gl), _dp(gth) if self.n else None))
+        return float(loss[0]), float(gm[0]), float(gl[0]), gth[:self.n].copy()

@@ -157,14 +157,19 @@ struct ProdCopy {
 
 // field 0: c; field 1: f = (lambda/eps^2)(c^3 - c) (src/main.cpp:81)
 template <typename T>
+// W (optional): wetting source added to f (calibration extension, zero in
+// the reference)
 struct ProdCF {
     const T *c;
     T fscale;
+    const T *W;
     __device__ __forceinline__ T operator()(int f, size_t i) const
     {
         const T v = c[i];
         if (f == 0) return v;
-        return (v * v * v - v) * fscale;
+        T r = (v * v * v - v) * fscale;
+        if (W) r += W[i];
+        return r;
     }
 };
 
@@ -321,7 +326,8 @@ __global__ __launch_bounds__(XT) void k_x_cgrad_f(const cplx<T> *__restrict__ ch
                                                   const cplx<T> *__restrict__ dzc, T *__restrict__ c,
                                                   T *__restrict__ gx, T *__restrict__ gy, T *__restrict__ gz,
                                                   cplx<T> *__restrict__ fout, T fscale, Dims d,
-                                                  const double *__restrict__ kxv, const cplx<T> *__restrict__ tw)
+                                                  const double *__restrict__ kxv, const cplx<T> *__restrict__ tw,
+                                                  const T *__restrict__ W)
 {
     FIPI_XPASS_SETUP(T, N)
     const size_t rA = zb * d.hs + (size_t)rl * d.nxp, rB = rA + d.nxp;
@@ -354,6 +360,11 @@ __global__ __launch_bounds__(XT) void k_x_cgrad_f(const cplx<T> *__restrict__ ch
     for (int q = 0; q < E; q++) {
         const T a = cv[q].x, b = cv[q].y;
         cv[q] = C{(a * a * a - a) * fscale, (b * b * b - b) * fscale};
+        if (W) {
+            const int x = t + q * P;
+            cv[q].x += W[xA + x];
+            cv[q].y += W[xB + x];
+        }
     }
     fft_dist<T, N, -1>(cv, L, t, tw);
     store_r2c<T, N>(cv, L, fout, rA, rB, t);

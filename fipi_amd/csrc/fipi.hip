@@ -93,6 +93,11 @@ struct fipi_ctx {
     // h[0] already holds the next step's conv (X) and conv_part its FP64
     // block sums: the previous step's last pass formed u.grad c
     bool conv_ready = false;
+    // calibration extension: wetting source W (real, added to f; null in the
+    // reference physics) and particles held at their positions (no Heun
+    // stages; their contact forces are spread every step unchanged)
+    void *W = nullptr;
+    bool fix_particles = false;
     double *kx = nullptr, *ky = nullptr, *kz = nullptr;
     void *twx = nullptr, *twy = nullptr, *twz = nullptr;
     // particles
@@ -352,7 +357,7 @@ struct Impl {
     static void init_fields(fipi_ctx *ctx)
     {
         const Dims d = ctx->d;
-        ProdCF<T> pr{R(ctx->c), fscale(ctx)};
+        ProdCF<T> pr{R(ctx->c), fscale(ctx), (const T *)ctx->W};
         x_forward<2>(ctx, pr, H(ctx->h[0]), H(ctx->h[1]));
         y_pass<-1, 2, false>(ctx, H(ctx->h[0]), H(ctx->h[1]), nullptr, nullptr);
         const KParams kp = kparams(ctx);
@@ -376,7 +381,7 @@ struct Impl {
             constexpr int N = decltype(NC)::value;
             hipLaunchKernelGGL((k_x_cgrad_f<T, N>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, ctx->stream, H(ctx->h[0]), H(ctx->h[2]),
                                H(ctx->h[1]), R(ctx->c), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), H(ctx->h[0]),
-                               fscale(ctx), d, ctx->kx, (const C *)ctx->twx);
+                               fscale(ctx), d, ctx->kx, (const C *)ctx->twx, (const T *)ctx->W);
         });
     }
 
@@ -599,7 +604,7 @@ struct Impl {
                 ctx->conv_ready = true;
             }
         }
-        if (parts) {
+        if (parts && !ctx->fix_particles) {
             particle_stage(ctx, 1);
             particle_stage(ctx, 2);
         }
@@ -1452,3 +1457,4 @@ int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32
 }  // extern "C"
 
 #include "ensemble.hpp"
+#include "adjoint.hpp"

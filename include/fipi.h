@@ -208,6 +208,31 @@ int fipi_ensemble_diagnostics(fipi_ensemble *ens, int32_t box, fipi_diag *out);
 /* Tile widths chosen for edge n (bytes; 0 when no box has that edge). */
 int fipi_ensemble_tuning(const fipi_ensemble *ens, int32_t n, int32_t *y_tb, int32_t *hub_tb, int32_t *stokes_tb);
 
+/* ---- adjoint of the step for calibration ------------------------------- */
+/* No counterpart in the reference.  Fits mobility, lambda and per-particle
+ * contact angles to a target field: the particles are held at their
+ * positions and wet the fluid through a source in the chemical potential,
+ *   f = (lambda/eps^2)(c^3 - c) + W,  W = -a_w sum_p cos(theta_p) S_p,
+ * S_p the trilinear spreading kernel of particle p over dx*dy*dz (W = 0 at
+ * theta = 90 deg or a_w = 0, where the step is the reference step).  The
+ * loss is sum_x (phi_T - phi_target)^2 after `steps` steps from phi0.
+ * Gradients come from a hand-written adjoint of the step with the forward
+ * state checkpointed every checkpoint_every steps and each segment
+ * recomputed.  Requires rho1 == rho2. */
+typedef struct fipi_adjoint fipi_adjoint;
+int fipi_adjoint_create(const fipi_params *params, const double *phi0, const double *pos, double wetting,
+                        int32_t steps, int32_t checkpoint_every, fipi_adjoint **out);
+int fipi_adjoint_destroy(fipi_adjoint *adj);
+const char *fipi_adjoint_last_error(const fipi_adjoint *adj);
+int fipi_adjoint_set_target(fipi_adjoint *adj, const double *phi_target);
+/* Forward run at (mobility, lambda, theta[n] in radians): the loss (if a
+ * target is set and loss != NULL) and the final field (if phi_final). */
+int fipi_adjoint_forward(fipi_adjoint *adj, double mobility, double lambda, const double *theta, double *loss,
+                         double *phi_final);
+/* Loss and dloss/dmobility, dloss/dlambda, dloss/dtheta[n]. */
+int fipi_adjoint_gradient(fipi_adjoint *adj, double mobility, double lambda, const double *theta, double *loss,
+                          double *g_mobility, double *g_lambda, double *g_theta);
+
 /* ---- seeded synthetic inputs (host only, no GPU needed) ---------------- */
 /* phi[i] = mean + U(-amp, amp) from a counter-based generator. */
 int fipi_init_phi(int64_t g, double mean, double amp, uint64_t seed, double *phi);
