@@ -36,6 +36,7 @@ ABI_SYMBOLS = [
     "fipi_step", "fipi_step_async", "fipi_sync", "fipi_profile_steps",
     "fipi_fft_r2c", "fipi_fft_c2r", "fipi_cells_build", "fipi_contact_pairs",
     "fipi_contact_forces", "fipi_interp", "fipi_spread",
+    "fipi_snapshot_size", "fipi_snapshot_save", "fipi_snapshot_load",
     "fipi_ensemble_create", "fipi_ensemble_destroy", "fipi_ensemble_last_error", "fipi_ensemble_set_box",
     "fipi_ensemble_tune", "fipi_ensemble_step", "fipi_ensemble_step_async", "fipi_ensemble_sync",
     "fipi_ensemble_stream", "fipi_ensemble_get_phi", "fipi_ensemble_get_particles",
@@ -132,6 +133,10 @@ def lib():
                                 C.c_double, C.c_uint64, i64, dp]
     L.fipi_init_rsa.restype = i64
     L.fipi_init_uniform.argtypes = [i64, C.c_double, C.c_double, C.c_uint64, dp]
+    L.fipi_snapshot_size.argtypes = [vp]
+    L.fipi_snapshot_size.restype = i64
+    L.fipi_snapshot_save.argtypes = [vp, vp, i64]
+    L.fipi_snapshot_load.argtypes = [vp, vp, i64]
     L.fipi_ensemble_create.argtypes = [C.POINTER(CParams), C.c_int32, C.POINTER(CBoxDesc), C.POINTER(vp)]
     L.fipi_ensemble_destroy.argtypes = [vp]
     L.fipi_ensemble_last_error.argtypes = [vp]
@@ -293,6 +298,20 @@ class Simulation:
         d = CDiag()
         self._c(lib().fipi_diagnostics(self._h, C.byref(d)))
         return {k: getattr(d, k) for k, _ in CDiag._fields_}
+
+    def snapshot(self) -> bytes:
+        """The whole state (fields, SBDF2 history, particles, counters)."""
+        n = lib().fipi_snapshot_size(self._h)
+        if n < 0:
+            self._c(int(n))
+        buf = C.create_string_buffer(int(n))
+        self._c(lib().fipi_snapshot_save(self._h, buf, n))
+        return buf.raw
+
+    def restore(self, snap: bytes):
+        """Continue from a snapshot of a context with the same parameters."""
+        buf = C.create_string_buffer(snap, len(snap))
+        self._c(lib().fipi_snapshot_load(self._h, buf, len(snap)))
 
     # ---- unit entry points on device memory (torch tensors) --------------
     # torch only provides device memory here; the work runs in libfipi.

@@ -479,9 +479,11 @@ __global__ __launch_bounds__(XT) void k_x_force(cplx<T> *__restrict__ mu_io /* i
     C *L = lds + lane;                                                \
     (void)E; (void)P;
 
+// 32-bit element offsets within one box (the box offset is applied to the
+// pointer, which stays uniform): half the address registers of 64-bit ones
 template <typename T, int N>
-__device__ __forceinline__ void tile_load(cplx<T> *v, const cplx<T> *__restrict__ a, size_t base, size_t st, int t,
-                                          bool ok)
+__device__ __forceinline__ void tile_load(cplx<T> *v, const cplx<T> *__restrict__ a, uint32_t base, uint32_t st,
+                                          int t, bool ok)
 {
     constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
 #pragma unroll
@@ -489,8 +491,8 @@ __device__ __forceinline__ void tile_load(cplx<T> *v, const cplx<T> *__restrict_
 }
 
 template <typename T, int N>
-__device__ __forceinline__ void tile_store(const cplx<T> *v, cplx<T> *__restrict__ a, size_t base, size_t st, int t,
-                                           bool ok)
+__device__ __forceinline__ void tile_store(const cplx<T> *v, cplx<T> *__restrict__ a, uint32_t base, uint32_t st,
+                                           int t, bool ok)
 {
     constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
     if (ok) {
@@ -508,9 +510,13 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_y(cplx<T>
                                                               const cplx<T> *__restrict__ tw)
 {
     FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
+    a0 = boxp(a0, zb, d.hs);
+    a1 = boxp(a1, zb, d.hs);
+    a2 = boxp(a2, zb, d.hs);
+    extra = boxp(extra, zb, d.hs);
     const int z = blockIdx.y;
-    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, 0, z);
-    const size_t st = d.nxp;
+    const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, 0, z);
+    const uint32_t st = d.nxp;
     C v[NF][E];
 #pragma unroll
     for (int f = 0; f < NF; f++) tile_load<T, N>(v[f], f == 0 ? a0 : (f == 1 ? a1 : a2), base, st, t, ok);
@@ -580,9 +586,15 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_ch(
     const double *__restrict__ kzv, const cplx<T> *__restrict__ tw, K0 k0)
 {
     FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
+    conv_io = boxp(conv_io, zb, d.hs);
+    dzc_out = boxp(dzc_out, zb, d.hs);
+    c_now = boxp(c_now, zb, d.hs);
+    c_next = boxp(c_next, zb, d.hs);
+    f_now = boxp(f_now, zb, d.hs);
+    hist = boxp(hist, zb, d.hs);
     const int y = blockIdx.y;
-    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
-    const size_t st = (size_t)d.nxp * d.ny;
+    const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
+    const uint32_t st = (uint32_t)d.nxp * d.ny;
     C v[E], cn[E], fn[E], hh[E];
     if (kp.use_conv) tile_load<T, N>(v, conv_io, base, st, t, ok);
     tile_load<T, N>(cn, c_now, base, st, t, ok);
@@ -599,6 +611,12 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_ch(
 #pragma unroll
     for (int q = 0; q < E; q++) {
         const int z = t + q * P;
+        // (always true: the per-point branch keeps one point's FP64
+        // temporaries live at a time)
+        if (z >= N) {
+            v[q] = w[q] = C{T(0), T(0)};
+            continue;
+        }
         const double kzz = kzv[z];
         double k2 = 0.0;
         k2 += kxx * kxx;
@@ -650,7 +668,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_ch(
             v[q] = C{nc.x * ig, nc.y * ig};
         }
         if (ok) {
-            const size_t i = base + st * z;
+            const uint32_t i = base + st * z;
             c_next[i] = nc;
             hist[i] = nh;
         }
@@ -673,9 +691,13 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_init(
                                                                      const cplx<T> *__restrict__ tw)
 {
     FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
+    c_io = boxp(c_io, zb, d.hs);
+    f_io = boxp(f_io, zb, d.hs);
+    c_k = boxp(c_k, zb, d.hs);
+    f_k = boxp(f_k, zb, d.hs);
     const int y = blockIdx.y;
-    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
-    const size_t st = (size_t)d.nxp * d.ny;
+    const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
+    const uint32_t st = (uint32_t)d.nxp * d.ny;
     C fv[E], v[E], w[E];
     tile_load<T, N>(fv, f_io, base, st, t, ok);
     tile_load<T, N>(v, c_io, base, st, t, ok);
@@ -709,9 +731,12 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_mu(cp
                                                                    const cplx<T> *__restrict__ tw)
 {
     FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
+    f_io = boxp(f_io, zb, d.hs);
+    f_k = boxp(f_k, zb, d.hs);
+    c_k = boxp(c_k, zb, d.hs);
     const int y = blockIdx.y;
-    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
-    const size_t st = (size_t)d.nxp * d.ny;
+    const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
+    const uint32_t st = (uint32_t)d.nxp * d.ny;
     C v[E], cc[E];
     tile_load<T, N>(v, f_io, base, st, t, ok);
     tile_load<T, N>(cc, c_k, base, st, t, ok);
@@ -721,6 +746,10 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_mu(cp
 #pragma unroll
     for (int q = 0; q < E; q++) {
         const int z = t + q * P;
+        if (z >= N) {  // never: one point's FP64 temporaries live at a time
+            v[q] = C{T(0), T(0)};
+            continue;
+        }
         const double kzz = kzv[z];
         double k2 = 0.0;
         k2 += kxx * kxx;
@@ -752,9 +781,12 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_stoke
                                                                        const cplx<T> *__restrict__ tw)
 {
     FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
+    a0 = boxp(a0, zb, d.hs);
+    a1 = boxp(a1, zb, d.hs);
+    a2 = boxp(a2, zb, d.hs);
     const int y = blockIdx.y;
-    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
-    const size_t st = (size_t)d.nxp * d.ny;
+    const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
+    const uint32_t st = (uint32_t)d.nxp * d.ny;
     C v[3][E];
     tile_load<T, N>(v[0], a0, base, st, t, ok);
     tile_load<T, N>(v[1], a1, base, st, t, ok);
@@ -820,9 +852,10 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_z(cplx<T>
                                                               const cplx<T> *__restrict__ tw)
 {
     FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
+    a = boxp(a, zb, d.hs);
     const int y = blockIdx.y;
-    const size_t base = zb * d.hs + hidx(d, ok ? kx : 0, y, 0);
-    const size_t st = (size_t)d.nxp * d.ny;
+    const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
+    const uint32_t st = (uint32_t)d.nxp * d.ny;
     C v[E];
     tile_load<T, N>(v, a, base, st, t, ok);
     fft_dist<T, N, SIGN, TL>(v, L, t, tw);

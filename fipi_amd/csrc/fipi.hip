@@ -252,8 +252,8 @@ void scan(fipi_ctx *ctx, const int *cnt, int *start, int n, int *bsum, const dou
 {
     const int nb = n > 0 ? (n + SCAN_ITEMS - 1) / SCAN_ITEMS : 1;
     hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, ctx->stream, cnt, start, n, bsum);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, ctx->stream, bsum, nb, start, n, ymax, yidx, nparts, mark);
-    hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(256), 0, ctx->stream, start, n, (const int *)bsum);
+    hipLaunchKernelGGL(k_scan_finish, dim3(nb), dim3(256), 0, ctx->stream, start, n, (const int *)bsum, nb, ymax, yidx,
+                       nparts, mark);
 }
 
 // advance the device step counter (one thread)
@@ -393,11 +393,6 @@ struct Impl {
         const int nl = ctx->nlines;
         const double n = pp.n, E = (double)sizeof(SpreadEntry<T>);
         {
-            ProfScope ps_(ctx, "spread_fill", 8.0 * nl);
-            hipLaunchKernelGGL(k_fill_int, grid1(nl, 256), dim3(256), 0, st, ctx->line_cnt, nl, 0);
-            hipLaunchKernelGGL(k_fill_int, grid1(nl, 256), dim3(256), 0, st, ctx->line_cursor, nl, 0);
-        }
-        {
             ProfScope ps_(ctx, "spread_count", 24.0 * n + 32.0 * n);
             hipLaunchKernelGGL(k_spread_count, grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, box,
                                ctx->line_cnt, ctx->d_flags);
@@ -420,7 +415,8 @@ struct Impl {
             // per-node sums into the (now free) scatter buffer
             ProfScope ps_(ctx, "spread_nodes", 8.0 * n * E + 8.0 * n * sizeof(SpreadNode<T>) + 8.0 * nl);
             hipLaunchKernelGGL((k_spread_nodes<T>), grid1(nl, 4), dim3(256), 0, st, nl, ctx->line_start,
-                               (const SpreadEntry<T> *)ctx->spr, (SpreadNode<T> *)ctx->spr_tmp, ctx->node_cnt);
+                               (const SpreadEntry<T> *)ctx->spr, (SpreadNode<T> *)ctx->spr_tmp, ctx->node_cnt,
+                               ctx->line_cnt, ctx->line_cursor);
         }
     }
 
@@ -429,12 +425,6 @@ struct Impl {
     {
         hipStream_t st = ctx->stream;
         const double n = pp.n, nc = ctx->ncell;
-        {
-            ProfScope ps_(ctx, "cells_fill", 8.0 * nc);
-            hipLaunchKernelGGL(k_fill_int, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->cell_cnt, ctx->ncell, 0);
-            hipLaunchKernelGGL(k_fill_int, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->cell_cursor, ctx->ncell,
-                               0);
-        }
         if (interp) {
             ProfScope ps_(ctx, "interp_fpi_count", n * (24.0 + 32.0 * sizeof(T) + 16.0 + 56.0 + 4.0));
             hipLaunchKernelGGL((k_interp_fpi_count<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, ps, R(ctx->c),
@@ -454,7 +444,8 @@ struct Impl {
         }
         {
             ProfScope ps_(ctx, "cells_sort", 4.0 * nc + 8.0 * n);
-            hipLaunchKernelGGL(k_cell_sort, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->ncell, start, ids);
+            hipLaunchKernelGGL(k_cell_sort, grid1(ctx->ncell, 256), dim3(256), 0, st, ctx->ncell, start, ids, ps,
+                               ctx->cell_cnt, ctx->cell_cursor);
         }
     }
 
@@ -474,7 +465,7 @@ struct Impl {
         const double rmp = ramp((double)(ctx->t + 1), ctx->p.ramp_t);
         {
             ProfScope ps_(ctx, "contact", pp.n * (24.0 + 4.0 + 24.0 + 8.0 + 24.0));
-            hipLaunchKernelGGL(k_contact, grid1(pp.n, 128), dim3(128), 0, st, pp, ctx->ps, ctx->cell_start,
+            hipLaunchKernelGGL(k_contact_par, grid1(pp.n, 8), dim3(256), 0, st, pp, ctx->ps, ctx->cell_start,
                                ctx->cell_ids, rmp, (const int *)ctx->t_dev, ctx->p.ramp_t, ctx->d_flags + 1);
         }
         {
@@ -814,6 +805,11 @@ int setup_buffers(fipi_ctx *ctx)
     ALLOC(ctx->line_cursor, sizeof(int) * (ctx->nlines + 1));
     ALLOC(ctx->node_cnt, sizeof(int) * (ctx->nlines + 1));
     HIPCHK(ctx, hipMemset(ctx->line_start, 0, sizeof(int) * (ctx->nlines + 1)));
+    // counters start at zero; each build's last kernel clears what it used
+    HIPCHK(ctx, hipMemset(ctx->cell_cnt, 0, sizeof(int) * (ctx->ncell + 1)));
+    HIPCHK(ctx, hipMemset(ctx->cell_cursor, 0, sizeof(int) * (ctx->ncell + 1)));
+    HIPCHK(ctx, hipMemset(ctx->line_cnt, 0, sizeof(int) * (ctx->nlines + 1)));
+    HIPCHK(ctx, hipMemset(ctx->line_cursor, 0, sizeof(int) * (ctx->nlines + 1)));
     const size_t n = ctx->n > 0 ? ctx->n : 1;
     ALLOC(ctx->cell_ids, sizeof(int) * n);
     ALLOC(ctx->spr, sizeof(SpreadEntry<T>) * 8 * n);
@@ -833,6 +829,7 @@ int setup_buffers(fipi_ctx *ctx)
         ALLOC(ps.fpp[a], sizeof(double) * n);
         ALLOC(ps.fpi[a], sizeof(double) * n);
         ALLOC(ps.dcdx[a], sizeof(double) * n);
+        ALLOC(ps.sp[a], sizeof(double) * n);
     }
     ALLOC(ps.rad, sizeof(double) * n);
     ALLOC(ps.cth, sizeof(double) * n);
@@ -914,6 +911,7 @@ struct TmpParts {
             if ((rc = alloc((void **)&ps.p[a], b))) return rc;
             if ((rc = alloc((void **)&ps.fpp[a], b))) return rc;
             if ((rc = alloc((void **)&ps.fpi[a], b))) return rc;
+            if ((rc = alloc((void **)&ps.sp[a], b))) return rc;
         }
         if ((rc = alloc((void **)&ps.tau_i, b))) return rc;
         if ((rc = alloc((void **)&ps.cell, sizeof(int) * (n > 0 ? n : 1)))) return rc;
@@ -1391,8 +1389,8 @@ int fipi_contact_forces(fipi_ctx *ctx, const double *d_pos, int32_t n, int32_t t
     const double thr = ctx->p.ramp_t;
     const double rmp = t < 0 ? 0.0 : (t < thr ? t / thr : 1.0);
     if (n > 0) {
-        hipLaunchKernelGGL(k_contact, grid1(n, 128), dim3(128), 0, ctx->stream, pp, tp.ps, ctx->cell_start, ids, rmp,
-                           (const int *)nullptr, 0.0, ctx->d_flags + 1);
+        hipLaunchKernelGGL(k_contact_par, grid1(n, 8), dim3(256), 0, ctx->stream, pp, tp.ps, ctx->cell_start, ids,
+                           rmp, (const int *)nullptr, 0.0, ctx->d_flags + 1);
         hipLaunchKernelGGL(k_soa_to_aos, grid1(n, 256), dim3(256), 0, ctx->stream, tp.ps.fpp[0], tp.ps.fpp[1],
                            tp.ps.fpp[2], d_f_pp, n);
     }
@@ -1458,3 +1456,124 @@ int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32
 
 #include "ensemble.hpp"
 #include "adjoint.hpp"
+
+// ---------------------------------------------------------------------------
+// Snapshot / restart: the whole state of a context (fields, SBDF2 history
+// and ring index, mass mode, particles, step counters) as one host buffer
+// that fipi_snapshot_load restores into a context created with the same
+// parameters; the run then continues bit for bit.  The reference writes
+// ASCII VTK output without the history (field.cpp:469-518,
+// cluster.cpp:415-456), which cannot restart a run.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct SnapHeader {
+    char magic[8];
+    int32_t version, nx, ny, nz, n, nbox, precision, t, cur, conv_ready, has_ctheta, reserved;
+    int64_t bytes;
+};
+
+std::vector<std::pair<void *, size_t>> snapshot_buffers(fipi_ctx *ctx)
+{
+    const size_t nb = ctx->nbox;
+    const size_t R = ctx->G * nb * ctx->esz, H = ctx->GH * nb * 2 * ctx->esz;
+    std::vector<std::pair<void *, size_t>> v = {
+        {ctx->c, R},          {ctx->g[0], R},  {ctx->g[1], R},  {ctx->g[2], R},  {ctx->u[0], R},
+        {ctx->u[1], R},       {ctx->u[2], R},  {ctx->ck[0], H}, {ctx->ck[1], H}, {ctx->fk, H},
+        {ctx->hist, H},       {ctx->h[0], H},  {ctx->k0, sizeof(double) * K0_STRIDE * nb},
+        {ctx->conv0, sizeof(double) * (nb + 1)},
+        {ctx->conv_part, sizeof(double) * nb * (size_t)(ctx->fp64 ? xblocks<double>(ctx->d) : xblocks<float>(ctx->d))},
+        {ctx->t_dev, sizeof(int)}};
+    const size_t n = ctx->n;
+    if (n > 0) {
+        PartState &ps = ctx->ps;
+        const size_t D = sizeof(double) * n;
+        for (int a = 0; a < 3; a++) {
+            v.push_back({ps.p[a], D});
+            v.push_back({ps.po[a], D});
+            v.push_back({ps.u[a], D});
+            v.push_back({ps.uo[a], D});
+            v.push_back({ps.fpp[a], D});
+            v.push_back({ps.fpi[a], D});
+            v.push_back({ps.dcdx[a], D});
+        }
+        v.push_back({ps.rad, D});
+        v.push_back({ps.cth, D});
+        v.push_back({ps.cloc, D});
+        v.push_back({ps.tau_i, D});
+        v.push_back({ps.mark, sizeof(int)});
+        if (ps.box) v.push_back({(void *)ps.box, sizeof(int) * n});
+    }
+    return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t fipi_snapshot_size(fipi_ctx *ctx)
+{
+    if (!ctx) return set_err(ctx, FIPI_ERR_ARG, "ctx is NULL");
+    int64_t b = sizeof(SnapHeader);
+    for (auto &x : snapshot_buffers(ctx)) b += (int64_t)x.second;
+    return b;
+}
+
+int fipi_snapshot_save(fipi_ctx *ctx, void *buf, int64_t size)
+{
+    if (!ctx || !buf) return set_err(ctx, FIPI_ERR_ARG, "ctx and buf are required");
+    if (!ctx->has_state) return set_err(ctx, FIPI_ERR_STATE, "no state to save");
+    const int64_t need = fipi_snapshot_size(ctx);
+    if (size < need) return set_err(ctx, FIPI_ERR_OVERFLOW, "snapshot buffer too small");
+    SnapHeader h{};
+    std::memcpy(h.magic, "FIPISNAP", 8);
+    h.version = 1;
+    h.nx = ctx->p.nx;
+    h.ny = ctx->p.ny;
+    h.nz = ctx->p.nz;
+    h.n = ctx->n;
+    h.nbox = ctx->nbox;
+    h.precision = ctx->p.precision;
+    h.t = ctx->t;
+    h.cur = ctx->cur;
+    h.conv_ready = ctx->conv_ready ? 1 : 0;
+    h.has_ctheta = ctx->pp.has_ctheta;
+    h.bytes = need;
+    std::memcpy(buf, &h, sizeof(h));
+    char *o = (char *)buf + sizeof(h);
+    for (auto &x : snapshot_buffers(ctx)) {
+        HIPCHK(ctx, hipMemcpyAsync(o, x.first, x.second, hipMemcpyDeviceToHost, ctx->stream));
+        o += x.second;
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return FIPI_OK;
+}
+
+int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size)
+{
+    if (!ctx || !buf) return set_err(ctx, FIPI_ERR_ARG, "ctx and buf are required");
+    if (size < (int64_t)sizeof(SnapHeader)) return set_err(ctx, FIPI_ERR_ARG, "not a snapshot");
+    SnapHeader h;
+    std::memcpy(&h, buf, sizeof(h));
+    if (std::memcmp(h.magic, "FIPISNAP", 8) != 0 || h.version != 1) return set_err(ctx, FIPI_ERR_ARG, "not a snapshot");
+    if (h.nx != ctx->p.nx || h.ny != ctx->p.ny || h.nz != ctx->p.nz || h.n != ctx->n || h.nbox != ctx->nbox ||
+        h.precision != ctx->p.precision)
+        return set_err(ctx, FIPI_ERR_ARG, "snapshot of a different grid, particle count or precision");
+    if (h.bytes != fipi_snapshot_size(ctx) || size < h.bytes) return set_err(ctx, FIPI_ERR_ARG, "snapshot size mismatch");
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    const char *in = (const char *)buf + sizeof(h);
+    for (auto &x : snapshot_buffers(ctx)) {
+        HIPCHK(ctx, hipMemcpyAsync(x.first, in, x.second, hipMemcpyHostToDevice, ctx->stream));
+        in += x.second;
+    }
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->t = h.t;
+    ctx->cur = h.cur;
+    ctx->conv_ready = h.conv_ready != 0;
+    ctx->pp.has_ctheta = h.has_ctheta;
+    ctx->has_state = true;
+    return FIPI_OK;
+}
+
+}  // extern "C"

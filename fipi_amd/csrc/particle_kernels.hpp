@@ -42,6 +42,7 @@ struct PartState {
     double *rad, *cth;
     double *cloc, *dcdx[3];
     double *tau_i;               // per-particle packing-energy contributions
+    double *sp[3];               // positions in cell-sorted order (sp[d][s] = p[d][ids[s]])
     int *cell;                   // contact cell of each particle
     int *err;                    // device error flags (bit 0: sorting range)
     int *mark;                   // it_mark (argmax y), persists across sorts
@@ -67,8 +68,7 @@ __device__ __forceinline__ int wave_incl_scan(int v)
 // ---------------------------------------------------------------------------
 // Multi-block exclusive scan: k_scan_blocks (each block of 256 threads scans
 // SCAN_ITEMS consecutive counts, 4 per thread, and writes its total),
-// k_scan_top (one block scans the block totals, writes start[n]),
-// k_scan_add (adds each block's offset).
+// k_scan_finish (adds each block's offset, writes start[n]).
 // ---------------------------------------------------------------------------
 constexpr int SCAN_ITEMS = 1024;
 
@@ -104,44 +104,41 @@ __global__ __launch_bounds__(256) void k_scan_blocks(const int *__restrict__ cnt
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-// Scan of the nb block totals in place (chunks of 1024 with a carry);
-// start[n] = grand total.  Also reduces the argmax-y partials (y, index) of
-// nparts blocks into *mark: the strictly largest y > 0, first index on ties
-// (cluster.cpp:128-138); the previous value is kept if no y > 0.  The
-// (y descending, index ascending) order is total, so the tree is exact.
-__global__ __launch_bounds__(256) void k_scan_top(int *__restrict__ bsum, int nb, int *__restrict__ start, int n,
-                                                  const double *__restrict__ ymax, const int *__restrict__ yidx,
-                                                  int nparts, int *__restrict__ mark)
+// Each block adds the sum of the block totals before it (bsum[0..b), a
+// block reduction; integer sums, so the order is immaterial); the last
+// block writes start[n] = grand total.  Block 0 also reduces the argmax-y
+// partials (y, index) of nparts blocks into *mark: the strictly largest
+// y > 0, first index on ties (cluster.cpp:128-138); the previous value is
+// kept if no y > 0.  The (y descending, index ascending) order is total,
+// so the tree is exact.
+__global__ __launch_bounds__(256) void k_scan_finish(int *__restrict__ start, int n, const int *__restrict__ bsum,
+                                                     int nb, const double *__restrict__ ymax,
+                                                     const int *__restrict__ yidx, int nparts, int *__restrict__ mark)
 {
-    __shared__ int wsum[4];
+    __shared__ int ws[256];
     __shared__ double sy[256];
     __shared__ int si[256];
-    int carry = 0;
-    for (int base = 0; base < nb; base += 1024) {
-        int v[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int i = base + threadIdx.x * 4 + q;
-            v[q] = i < nb ? bsum[i] : 0;
-        }
-        const int s = v[0] + v[1] + v[2] + v[3];
-        int tot;
-        int run = carry + block_excl_scan256(s, wsum, &tot);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int i = base + threadIdx.x * 4 + q;
-            if (i < nb) bsum[i] = run;
-            run += v[q];
-        }
-        carry += tot;
+    const int b = blockIdx.x;
+    int acc = 0;
+    for (int i = threadIdx.x; i < b; i += 256) acc += bsum[i];
+    ws[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) ws[threadIdx.x] += ws[threadIdx.x + o];
+        __syncthreads();
     }
-    if (threadIdx.x == 0) start[n] = carry;
-    if (mark == nullptr) return;
+    const int off = ws[0];
+    const int i0 = b * SCAN_ITEMS + threadIdx.x * 4;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (i0 + q < n) start[i0 + q] += off;
+    if (b == nb - 1 && threadIdx.x == 0) start[n] = off + bsum[b];
+    if (mark == nullptr || b != 0) return;
     double by = 0.0;
     int bi = -1;
-    for (int b = threadIdx.x; b < nparts; b += 256) {
-        const double oy = ymax[b];
-        const int oi = yidx[b];
+    for (int k = threadIdx.x; k < nparts; k += 256) {
+        const double oy = ymax[k];
+        const int oi = yidx[k];
         if (oi >= 0 && (bi < 0 || oy > by || (oy == by && oi < bi))) { by = oy; bi = oi; }
     }
     sy[threadIdx.x] = by;
@@ -160,15 +157,6 @@ __global__ __launch_bounds__(256) void k_scan_top(int *__restrict__ bsum, int nb
         __syncthreads();
     }
     if (threadIdx.x == 0 && si[0] >= 0) *mark = si[0];
-}
-
-__global__ __launch_bounds__(256) void k_scan_add(int *__restrict__ start, int n, const int *__restrict__ bsum)
-{
-    const int i = blockIdx.x * SCAN_ITEMS + threadIdx.x * 4;
-    const int off = bsum[blockIdx.x];
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-        if (i + q < n) start[i + q] += off;
 }
 
 // Block partial of argmax y (strictly greater, first index; y_max starts at
@@ -197,12 +185,6 @@ __device__ __forceinline__ void block_argmax_y(double y, int idx, double *ymax, 
         ymax[blockIdx.x] = sy[0];
         yidx[blockIdx.x] = si[0];
     }
-}
-
-__global__ void k_fill_int(int *__restrict__ a, int n, int v)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
 }
 
 // cluster.cpp:140-142 / 197-199
@@ -250,17 +232,30 @@ __global__ void k_cell_scatter(int n, const int *__restrict__ cell, const int *_
     ids[start[cc] + slot] = i;
 }
 
-// Ascending ids within each cell (list::append order, linked_list.cpp:30-43).
-__global__ void k_cell_sort(int ncell, const int *__restrict__ start, int *__restrict__ ids)
+// Ascending ids within each cell (list::append order, linked_list.cpp:30-43);
+// the positions are gathered in the same order (sp, may be null).
+// cnt and cursor of the cell are cleared for the next build.
+__global__ void k_cell_sort(int ncell, const int *__restrict__ start, int *__restrict__ ids, PartState S,
+                            int *__restrict__ cnt, int *__restrict__ cursor)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
+    cnt[c] = 0;
+    cursor[c] = 0;
     const int b = start[c], e = start[c + 1];
     for (int i = b + 1; i < e; i++) {
         const int v = ids[i];
         int j = i - 1;
         while (j >= b && ids[j] > v) { ids[j + 1] = ids[j]; j--; }
         ids[j + 1] = v;
+    }
+    if (S.sp[0]) {
+        for (int i = b; i < e; i++) {
+            const int a = ids[i];
+            S.sp[0][i] = S.p[0][a];
+            S.sp[1][i] = S.p[1][a];
+            S.sp[2][i] = S.p[2][a];
+        }
     }
 }
 
@@ -386,13 +381,6 @@ __device__ void contact_slow(const PartParams &P, const PartState &S, const int 
     *tau_out = tau;
 }
 
-// One thread per particle (taken in cell-sorted order for locality).
-// f_pp[a] is summed in the order the reference accumulates it: first the
-// contributions of partners b < a in ascending b (they were added when the
-// outer loop visited b), then partners b > a in 27-cell traversal order.
-// One scan of the 27 cells collects both partner lists.
-// tau_i[a] = sum over b > a of 0.5 F.d when f_pi[a] and f_pi[mark] are
-// non-zero (cluster.cpp:256-259).
 // rmp: the contact ramp; when t_dev is set it is ramp(*t_dev + 1, ramp_t)
 // (primitive.cpp:122-129) from the device step counter instead.
 __device__ __forceinline__ double ramp_dev(double t, double thr)
@@ -402,69 +390,138 @@ __device__ __forceinline__ double ramp_dev(double t, double thr)
     return 1.0;
 }
 
-__global__ __launch_bounds__(128) void k_contact(PartParams P, PartState S, const int *__restrict__ start,
-                                                 const int *__restrict__ ids, double rmp,
-                                                 const int *__restrict__ t_dev, double ramp_t,
-                                                 int *__restrict__ overflow)
+// Contact forces (cluster.cpp:174-275) with 32 lanes per particle (8
+// particles per workgroup, taken in cell-sorted order): lane c < 27 scans
+// neighbour cell c in the reference's traversal order (k outer, then j,
+// then i) over the cell-sorted positions, and an ordered prefix sum over the
+// lanes builds the two partner lists in LDS.  The first lane then sums
+// f_pp[a] in the order the reference accumulates it: the contributions of
+// partners b < a in ascending b (added when the outer loop visited b), then
+// partners b > a in traversal order, with F(a,b) = -F(b,a) bitwise.
+// tau_i[a] = sum over b > a of 0.5 F.d when f_pi[a] and f_pi[mark] are
+// non-zero (cluster.cpp:256-259).
+constexpr int CP_HITS = 4;  // partners per lane held before falling back
+
+__device__ __forceinline__ void pair_disp_pos(const PartParams &P, double ax, double ay, double az, double bx,
+                                              double by, double bz, int a, int b, double *d)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    // only the slots the sort filled (particles outside the range are left out)
-    if (s >= P.n || s >= start[P.ncell_box * P.nbox]) return;
+    d[0] = ax - bx;
+    d[1] = ay - by;
+    d[2] = az - bz;
+    const double L[3] = {P.lx, P.ly, P.lz};
+    const double th[3] = {P.dcx * (P.ncx - 2), P.dcy * (P.ncy - 2), P.dcz * (P.ncz - 2)};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        if (d[q] > th[q]) d[q] -= L[q];
+        else if (d[q] < -th[q]) d[q] += L[q];
+    }
+    if (vnorm(d[0], d[1], d[2]) == 0.0) {
+        const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double jv = jitter(lo, hi, q, P.r_p);
+            d[q] = a < b ? jv : -jv;
+        }
+    }
+}
+
+// inclusive sum over the 32 lanes of each half wave
+__device__ __forceinline__ int half_wave_incl_scan(int v)
+{
+    const int lane = threadIdx.x & 31;
+#pragma unroll
+    for (int o = 1; o < 32; o <<= 1) {
+        const int u = __shfl_up(v, o, 32);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, const int *__restrict__ start,
+                                                     const int *__restrict__ ids, double rmp,
+                                                     const int *__restrict__ t_dev, double ramp_t,
+                                                     int *__restrict__ overflow)
+{
+    __shared__ int lists[8][2][MAX_NB];
+    const int sub = threadIdx.x >> 5, c = threadIdx.x & 31;
+    const int s = blockIdx.x * 8 + sub;
+    const int nsorted = start[P.ncell_box * P.nbox];
+    const bool live = s < P.n && s < nsorted;
     if (t_dev) rmp = ramp_dev((double)(*t_dev + 1), ramp_t);
-    const int a = ids[s];
-    const int cb = part_box(S.box, a) * P.ncell_box;
-    int o, pp, q;
-    cell_of(P, S.p[0][a], S.p[1][a], S.p[2][a], &o, &pp, &q);
+    int a = 0, nl = 0, nu = 0;
+    int hl[CP_HITS], hu[CP_HITS];
+    bool ovf = false;
+    double ax = 0, ay = 0, az = 0;
+    int o = 0, pp = 0, q = 0, cb = 0;
+    if (live) {
+        a = ids[s];
+        ax = S.sp[0][s];
+        ay = S.sp[1][s];
+        az = S.sp[2][s];
+        cb = part_box(S.box, a) * P.ncell_box;
+        cell_of(P, ax, ay, az, &o, &pp, &q);
+        if (c < 27) {
+            const int k = q - 1 + c / 9, j = pp - 1 + (c / 3) % 3, i = o - 1 + c % 3;
+            const int co = cb + wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+            const int e1 = start[co + 1];
+            for (int e = start[co]; e < e1; e++) {
+                const int b = ids[e];
+                if (b == a) continue;
+                double d[3];
+                pair_disp_pos(P, ax, ay, az, S.sp[0][e], S.sp[1][e], S.sp[2][e], a, b, d);
+                if (vnorm(d[0], d[1], d[2]) < P.r_eq) {
+                    if (b < a) {
+                        if (nl < CP_HITS) hl[nl] = b;
+                        else ovf = true;
+                        nl++;
+                    } else {
+                        if (nu < CP_HITS) hu[nu] = b;
+                        else ovf = true;
+                        nu++;
+                    }
+                }
+            }
+        }
+    }
+    // ordered compaction of the lanes' partners into the half wave's lists
+    const int il = half_wave_incl_scan(nl), iu = half_wave_incl_scan(nu);
+    const int tl = __shfl(il, 31, 32), tu = __shfl(iu, 31, 32);
+    const bool any_ovf = __any(ovf) != 0;  // per wave: either half falls back
+    const bool fits = !ovf && tl <= MAX_NB && tu <= MAX_NB;
+    if (fits) {
+        for (int x = 0; x < nl && x < CP_HITS; x++) lists[sub][0][il - nl + x] = hl[x];
+        for (int x = 0; x < nu && x < CP_HITS; x++) lists[sub][1][iu - nu + x] = hu[x];
+    }
+    __syncthreads();
+    if (!live || c != 0) return;
     const int mk = *S.mark;
     bool tau_on = false;
     if (mk >= 0) {
         tau_on = vnorm(S.fpi[0][a], S.fpi[1][a], S.fpi[2][a]) != 0.0 &&
                  vnorm(S.fpi[0][mk], S.fpi[1][mk], S.fpi[2][mk]) != 0.0;
     }
-    int low[MAX_NB], up[MAX_NB];
-    int nl = 0, nu = 0;
-    bool ovf = false;
-    for (int k = q - 1; k <= q + 1; k++)
-        for (int j = pp - 1; j <= pp + 1; j++)
-            for (int i = o - 1; i <= o + 1; i++) {
-                const int co = cb + wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
-                const int e1 = start[co + 1];
-                for (int e = start[co]; e < e1; e++) {
-                    const int b = ids[e];
-                    if (b == a) continue;
-                    double d[3];
-                    pair_disp(P, S, a, b, d);
-                    if (vnorm(d[0], d[1], d[2]) < P.r_eq) {
-                        if (b < a) {
-                            if (nl < MAX_NB) low[nl++] = b;
-                            else ovf = true;
-                        } else {
-                            if (nu < MAX_NB) up[nu++] = b;
-                            else ovf = true;
-                        }
-                    }
-                }
-            }
     double fo[3], tau = 0.0;
-    if (ovf) {
+    if (any_ovf || tl > MAX_NB || tu > MAX_NB) {
         atomicOr(overflow, 1);
         contact_slow(P, S, start, ids, a, o, pp, q, cb, rmp, tau_on, fo, &tau);
     } else {
-        for (int x = 1; x < nl; x++) {
+        int *low = lists[sub][0];
+        const int *up = lists[sub][1];
+        for (int x = 1; x < tl; x++) {
             const int v = low[x];
             int y = x - 1;
             while (y >= 0 && low[y] > v) { low[y + 1] = low[y]; y--; }
             low[y + 1] = v;
         }
         double fx = 0.0, fy = 0.0, fz = 0.0;
-        for (int x = 0; x < nl; x++) {
+        for (int x = 0; x < tl; x++) {
             double f[3], d[3];
             pair_force(P, S, a, low[x], rmp, f, d);
             fx = fx + f[0];
             fy = fy + f[1];
             fz = fz + f[2];
         }
-        for (int x = 0; x < nu; x++) {
+        for (int x = 0; x < tu; x++) {
             double f[3], d[3];
             pair_force(P, S, a, up[x], rmp, f, d);
             fx = fx + f[0];
@@ -739,9 +796,11 @@ __global__ void k_spread_scatter(PartParams P, const double *__restrict__ px, co
 }
 
 // Sort each line's entries by (x, particle id) into `out`: one wave per
-// line, each lane ranks its entries by counting smaller keys.  Within a line
-// the keys are unique (a particle meets a line at two different x), so the
-// ranks are a permutation and the result is deterministic.
+// line; each chunk of 64 entries is held one per lane, and every lane ranks
+// its entry by counting smaller keys among all entries of the line (keys of
+// the chunk in hand are broadcast from their lanes).  Within a line the keys
+// are unique (a particle meets a line at two different x), so the ranks are
+// a permutation and the result is deterministic.
 template <typename T>
 __global__ __launch_bounds__(256) void k_spread_rank(int nlines, const int *__restrict__ line_start,
                                                      const SpreadEntry<T> *__restrict__ in,
@@ -751,14 +810,23 @@ __global__ __launch_bounds__(256) void k_spread_rank(int nlines, const int *__re
     const int lane = threadIdx.x & 63;
     if (line >= nlines) return;
     const int b = line_start[line], e = line_start[line + 1];
-    for (int i = b + lane; i < e; i += 64) {
-        const SpreadEntry<T> v = in[i];
+    for (int i0 = b; i0 < e; i0 += 64) {
+        const int i = i0 + lane;
+        const bool mine = i < e;
+        SpreadEntry<T> v{};
+        if (mine) v = in[i];
         int r = 0;
-        for (int j = b; j < e; j++) {
-            const int xj = in[j].x, pj = in[j].pid;
-            r += (xj < v.x || (xj == v.x && pj < v.pid)) ? 1 : 0;
+        for (int j0 = b; j0 < e; j0 += 64) {
+            // chunk j0 of the line, one key per lane
+            const int jj = j0 + lane;
+            const int kx = jj < e ? in[jj].x : 0x7fffffff, kp = jj < e ? in[jj].pid : 0x7fffffff;
+            const int m = min(64, e - j0);
+            for (int k = 0; k < m; k++) {
+                const int xj = __shfl(kx, k), pj = __shfl(kp, k);
+                r += (xj < v.x || (xj == v.x && pj < v.pid)) ? 1 : 0;
+            }
         }
-        out[b + r] = v;
+        if (mine) out[b + r] = v;
     }
 }
 
@@ -767,14 +835,20 @@ __global__ __launch_bounds__(256) void k_spread_rank(int nlines, const int *__re
 // order (the reference's accumulation order, cluster.cpp:355-407) and the
 // heads are compacted with a wave ballot + popcount.  Node k of line l is
 // stored at nodes[line_start[l] + k]; node_cnt[l] gets the count.
+// line_cnt and line_cursor of the line are cleared for the next build.
 template <typename T>
 __global__ __launch_bounds__(256) void k_spread_nodes(int nlines, const int *__restrict__ line_start,
                                                       const SpreadEntry<T> *__restrict__ e,
-                                                      SpreadNode<T> *__restrict__ nodes, int *__restrict__ node_cnt)
+                                                      SpreadNode<T> *__restrict__ nodes, int *__restrict__ node_cnt,
+                                                      int *__restrict__ line_cnt, int *__restrict__ line_cursor)
 {
     const int line = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (line >= nlines) return;
+    if (lane == 0) {
+        line_cnt[line] = 0;
+        line_cursor[line] = 0;
+    }
     const int b = line_start[line], en = line_start[line + 1];
     int base = 0;
     for (int c0 = b; c0 < en; c0 += 64) {

@@ -168,6 +168,18 @@ int fipi_interp(fipi_ctx *ctx, const void *d_grid, const double *d_pos, int32_t 
  * the contributions are summed in ascending particle order. */
 int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32_t n, void *d_grid3);
 
+/* ---- snapshot / restart ------------------------------------------------ */
+/* The whole state of a context (fields, the SBDF2 history and ring index,
+ * the FP64 mass mode, particles, step counters) as one host buffer: a
+ * header (magic "FIPISNAP", version, grid, particle count, precision, t)
+ * followed by the raw device arrays.  fipi_snapshot_load restores it into a
+ * context created with the same parameters; stepping then continues bit
+ * for bit.  The reference writes ASCII VTK output without the history
+ * (field.cpp:469-518, cluster.cpp:415-456), which cannot restart a run. */
+int64_t fipi_snapshot_size(fipi_ctx *ctx);
+int fipi_snapshot_save(fipi_ctx *ctx, void *buf, int64_t size);
+int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size);
+
 /* ---- batched ensemble of independent boxes ---------------------------- */
 /* No counterpart in the reference, which runs one box per process
  * (src/main.cpp:19-252).  Many independent periodic cubic boxes of mixed
