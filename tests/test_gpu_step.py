@@ -74,6 +74,19 @@ def test_step_config1_64(gpu):
     assert rel_l2(gp["p"], rp["p"]) < TOL
 
 
+def test_1000_steps_64(gpu):
+    """The north-star bar (1e-10 relative L2 after 1000 FP64 steps) at 64^3
+    with 800 spheres of radius 2.5 (config-1 physics), checked every 250
+    steps; the oracle takes about half a minute for this on the host."""
+    print("1000 steps 64^3", flush=True)
+    p, phi0, pos = make_case(64, 800, r_p=2.5, steps=1000, seed=7)
+    for t, g_phi, r_phi, g_u, r_u, gp, rp in _run_both(p, phi0, pos, 1000, chunks=4):
+        print(t, rel_l2(g_phi, r_phi), rel_l2(g_u, r_u), rel_l2(gp["p"], rp["p"]), flush=True)
+        assert rel_l2(g_phi, r_phi) < TOL, (t, rel_l2(g_phi, r_phi))
+        assert rel_l2(g_u, r_u) < TOL, (t, rel_l2(g_u, r_u))
+        assert rel_l2(gp["p"], rp["p"]) < TOL, (t, rel_l2(gp["p"], rp["p"]))
+
+
 def test_reference_defaults_64(gpu):
     """The reference's own constants (head.hpp: L = 2 pi, dt = 1e-3,
     r_eq = 5 r_p), 400 particles, 5 steps."""
