@@ -36,13 +36,18 @@ struct Dims {
     size_t rs, hs;
 };
 
-constexpr int XT = 256;  // threads per x-pass workgroup (at most)
+// threads per x-pass workgroup (at most): FP64 kernels hold ~230 VGPRs,
+// so a CU runs 8 of their waves whatever the workgroup size; 2-wave
+// workgroups let 4 of them work in different phases (load / FFT / store)
+template <typename T> struct XT_of {
+    static constexpr int v = sizeof(T) == 8 ? 128 : 256;
+};
 
 // x-pass workgroup: up to LPW complex lines (2 real lines each) of P
 // threads; a launch uses blockDim.x / P lines, a divisor of ny*nz/2, so
 // every line of every workgroup exists
 template <typename T, int N> struct XShape {
-    static constexpr int LPW = XT / Line<T, N>::P;
+    static constexpr int LPW = XT_of<T>::v / Line<T, N>::P;
 };
 
 // kx columns per y/z-pass workgroup: a TB-byte run of complex values
@@ -173,14 +178,17 @@ struct ProdCF {
     }
 };
 
-// Fixed-order FP64 sum of one value per thread over the block (at most 256
-// threads); the result is valid in thread 0.  `red` holds 256 doubles.
+// Fixed-order FP64 sum of one value per thread over the block; the result
+// is valid in thread 0.  `red` holds blockDim.x rounded up to a power of two
+// doubles (at least 2).
 __device__ __forceinline__ double block_sum_f64(double v, double *red)
 {
+    int np = 2;
+    while (np < (int)blockDim.x) np <<= 1;
     red[threadIdx.x] = v;
-    for (int i = blockDim.x + threadIdx.x; i < 256; i += blockDim.x) red[i] = 0.0;
+    for (int i = blockDim.x + threadIdx.x; i < np; i += blockDim.x) red[i] = 0.0;
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
+    for (int o = np / 2; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
@@ -211,7 +219,7 @@ __global__ __launch_bounds__(256) void k_sum_strided(const double *part, int nb,
 // part (optional): FP64 sum of field 0's real values per workgroup, for
 // the k = 0 mode of the FP32-mixed path.
 template <typename T, int N, int NF, class Prod>
-__global__ __launch_bounds__(XT) void k_x_forward(Prod prod, cplx<T> *__restrict__ out0, cplx<T> *__restrict__ out1,
+__global__ __launch_bounds__(XT_of<T>::v) void k_x_forward(Prod prod, cplx<T> *__restrict__ out0, cplx<T> *__restrict__ out1,
                                                   Dims d, const cplx<T> *__restrict__ tw, double *__restrict__ part)
 {
     FIPI_XPASS_SETUP(T, N)
@@ -238,7 +246,7 @@ __global__ __launch_bounds__(XT) void k_x_forward(Prod prod, cplx<T> *__restrict
 
 // Plain inverse x transform (C2R) of NF fields into real arrays.
 template <typename T, int N, int NF>
-__global__ __launch_bounds__(XT) void k_x_inverse(const cplx<T> *__restrict__ in0, const cplx<T> *__restrict__ in1,
+__global__ __launch_bounds__(XT_of<T>::v) void k_x_inverse(const cplx<T> *__restrict__ in0, const cplx<T> *__restrict__ in1,
                                                   const cplx<T> *__restrict__ in2, T *__restrict__ o0,
                                                   T *__restrict__ o1, T *__restrict__ o2, Dims d,
                                                   const cplx<T> *__restrict__ tw)
@@ -270,7 +278,7 @@ __global__ __launch_bounds__(XT) void k_x_inverse(const cplx<T> *__restrict__ in
 // g holds grad c^{n+1}, the gradient the next step's convection uses.
 // part (optional): FP64 block sums of conv, as k_x_forward.
 template <typename T, int N>
-__global__ __launch_bounds__(XT) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *__restrict__ h1,
+__global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *__restrict__ h1,
                                                    const cplx<T> *__restrict__ h2, T *__restrict__ u0,
                                                    T *__restrict__ u1, T *__restrict__ u2,
                                                    const T *__restrict__ g0, const T *__restrict__ g1,
@@ -322,7 +330,7 @@ __global__ __launch_bounds__(XT) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *_
 // P5: inverse x of (c_hat, dy c_hat, dz c_hat) -> c, grad c (real), then
 // f = (lambda/eps^2)(c^3 - c) and its forward x transform (src/main.cpp:153).
 template <typename T, int N>
-__global__ __launch_bounds__(XT) void k_x_cgrad_f(const cplx<T> *__restrict__ ch, const cplx<T> *__restrict__ dyc,
+__global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_f(const cplx<T> *__restrict__ ch, const cplx<T> *__restrict__ dyc,
                                                   const cplx<T> *__restrict__ dzc, T *__restrict__ c,
                                                   T *__restrict__ gx, T *__restrict__ gy, T *__restrict__ gz,
                                                   cplx<T> *__restrict__ fout, T fscale, Dims d,
@@ -393,7 +401,7 @@ struct SpreadNode {
 // the sums is the reference's ((mu grad c + spread) + buoyancy).  Buoyancy
 // enters the z component only: g = (0, 0, gravity) (field.cpp:122-133).
 template <typename T, int N>
-__global__ __launch_bounds__(XT) void k_x_force(cplx<T> *__restrict__ mu_io /* in: mu_hat (X); out: F_0 (X) */,
+__global__ __launch_bounds__(XT_of<T>::v) void k_x_force(cplx<T> *__restrict__ mu_io /* in: mu_hat (X); out: F_0 (X) */,
                                                 cplx<T> *__restrict__ f1, cplx<T> *__restrict__ f2,
                                                 const T *__restrict__ c, const T *__restrict__ gx,
                                                 const T *__restrict__ gy, const T *__restrict__ gz,

@@ -214,7 +214,7 @@ struct ProfScope {
 inline dim3 grid1(size_t n, int b) { return dim3((unsigned)(n == 0 ? 1 : (n + b - 1) / b)); }
 
 // x-pass lines per workgroup: the largest divisor of ny*nz/2 (complex
-// lines, 2 real lines each) that fits in XT threads
+// lines, 2 real lines each) that fits in the x-pass workgroup (XT_of)
 template <typename T, int N>
 int xlines(const Dims &d)
 {
