@@ -216,25 +216,89 @@ def step_bytes(p, n_particles):
     return 20 * R + 18 * H + n_particles * bp
 
 
-def cpu_baseline(p, phi0, pos, cos_theta, steps):
-    """The oracle (C restatement of the reference step, 25 complex 3-D FFTs
-    per step like the reference) timed on this host's cores."""
+def host_cpu():
+    """CPU model name and the host cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
+def oracle_rate(p, phi0, pos, cos_theta, steps, threads):
+    """steps/s of the oracle after one untimed first-order step."""
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
     sim = oracle.Sim(p, phi0, pos, None, cos_theta, n_threads=threads)
-    sim.step(1)  # first-order start step, untimed
+    sim.step(1)
     t0 = time.perf_counter()
     sim.step(steps)
     dt = time.perf_counter() - t0
     del sim
+    return steps / dt, dt
+
+
+def cpu_baseline(p, phi0, pos, cos_theta, steps):
+    """The oracle (C restatement of the reference step, 25 complex 3-D FFTs
+    per step like the reference) timed on this host's cores, with all of
+    them (at most 16) and with the reference's 4 FFTW threads."""
+    model, ncpu = host_cpu()
+    threads = min(ncpu, 16)
+    rate, dt = oracle_rate(p, phi0, pos, cos_theta, steps, threads)
+    rate4, dt4 = oracle_rate(p, phi0, pos, cos_theta, steps, 4)
     return {
-        "value": steps / dt,
+        "value": rate,
         "unit": "steps/s",
         "cores": threads,
         "kind": "port",
         "sample": f"{steps} reference-structured FP64 steps (25 complex 3-D FFTs each, "
                   f"OpenMP) of the same {p.nx}^3 + {pos.shape[0]} workload after 1 warm-up step, "
                   f"{dt:.1f} s wall",
+        "value_4_threads": rate4,
+        "host_cpu": model,
+        "host_cores_available": ncpu,
+        "fft": "the oracle's Stockham FFT (FFTW is not installed)",
+    }
+
+
+def config1_run(steps=200, seed=7):
+    """The reference's CPU-runnable case at full length: 64^3, spheres of
+    radius 2.5 dx (800: the requested 2,000 exceed random packing), FP64,
+    200 steps, on the GPU and in the oracle (all host cores), with the
+    parity of the two after the last step."""
+    import fipi_amd
+    model, ncpu = host_cpu()
+    n, npart = 64, 800
+    p = fipi_amd.config_params(n, npart, 2.5, steps)
+    phi0 = fipi_amd.init_phi((n, n, n), 0.0, 0.05, seed)
+    pos = fipi_amd.init_rsa(npart, 2.5, 0.5, (p.lx, p.ly, p.lz), seed + 1000)
+    sim = fipi_amd.Simulation(p)
+    sim.set_state(phi0, pos)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    gpu_dt = time.perf_counter() - t0
+    g_phi, g_pos = sim.phi(), sim.particles()["p"]
+    sim.close()
+    import oracle
+    threads = min(ncpu, 16)
+    ref = oracle.Sim(p, phi0, pos, n_threads=threads)
+    t0 = time.perf_counter()
+    ref.step(steps)
+    cpu_dt = time.perf_counter() - t0
+    r_phi, r_pos = ref.phi(), ref.particles()["p"]
+    del ref
+    return {
+        "workload": f"{n}^3 + {pos.shape[0]} spheres r=2.5dx, FP64, {steps} steps from t = 0",
+        "gpu_steps_per_s": steps / gpu_dt,
+        "cpu_steps_per_s": steps / cpu_dt,
+        "cpu_cores": threads,
+        "gpu_vs_cpu": round(cpu_dt / gpu_dt, 1),
+        "phi_rel_l2_vs_oracle": float(np.linalg.norm(g_phi - r_phi) / np.linalg.norm(r_phi)),
+        "pos_rel_l2_vs_oracle": float(np.linalg.norm(g_pos - r_pos) / np.linalg.norm(r_pos)),
     }
 
 
@@ -246,7 +310,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--particles", type=int, default=100000)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--single", action="store_true", help="skip the other precision")
@@ -256,6 +320,7 @@ def main():
     ap.add_argument("--ensemble-loop-steps", type=int, default=10)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-config1", action="store_true")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -387,6 +452,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, phi0, pos, cos_theta, args.cpu_steps)
+    cfg1 = None if (args.no_config1 or args.no_cpu_baseline) else config1_run()
 
     out = {
         "metric": METRIC,
@@ -427,6 +493,7 @@ def main():
         "diagnostics": {k: diag[k] for k in ("t", "mean_phi", "free_energy", "interface_area")},
         "ensemble": ens,
         "adjoint": adj,
+        "config1": cfg1,
     }
     print(json.dumps(out), flush=True)
 
