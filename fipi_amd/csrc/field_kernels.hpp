@@ -392,22 +392,62 @@ struct SpreadNode {
     T v[3];
 };
 
-// P9: inverse x of mu_hat -> mu (real); force F = mu grad c + particle
-// spread + buoyancy (src/main.cpp:149-172); forward x transform of F.
-// nodes[line_start[l] + k], k < node_cnt[l], are the per-node spread sums of
-// real line l (unique x per line), each already summed in ascending
-// particle order (cluster.cpp:355-407); every thread adds whole nodes, so
-// there is one add per node and no ordering between threads.  The order of
-// the sums is the reference's ((mu grad c + spread) + buoyancy).  Buoyancy
-// enters the z component only: g = (0, 0, gravity) (field.cpp:122-133).
+// Adds the spread sums of real lines rl, rl+1 to component f of a packed
+// line (thread t holds elements t + q*P of both lines).  nodes[line_start[l]
+// + k], k < node_cnt[l], are the per-node spread sums of real line l
+// (unique x per line), each already summed in ascending particle order
+// (cluster.cpp:355-407); every thread adds whole nodes, so there is one add
+// per node and no ordering between threads.
 template <typename T, int N>
-__global__ __launch_bounds__(XT_of<T>::v) void k_x_force(cplx<T> *__restrict__ mu_io /* in: mu_hat (X); out: F_0 (X) */,
-                                                cplx<T> *__restrict__ f1, cplx<T> *__restrict__ f2,
-                                                const T *__restrict__ c, const T *__restrict__ gx,
-                                                const T *__restrict__ gy, const T *__restrict__ gz,
-                                                const int *__restrict__ line_start, const int *__restrict__ node_cnt,
-                                                const SpreadNode<T> *__restrict__ nodes, int use_spread, T rho1,
-                                                T rho2, T gravity, Dims d, const cplx<T> *__restrict__ tw)
+__device__ __forceinline__ void add_spread(cplx<T> *v, cplx<T> *L, int t, int rl, int f,
+                                           const int *__restrict__ line_start, const int *__restrict__ node_cnt,
+                                           const SpreadNode<T> *__restrict__ nodes)
+{
+    constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
+    T *Lr = reinterpret_cast<T *>(L);  // natural-order buffer: line rl then rl+1
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int x = t + q * P;
+        Lr[x] = v[q].x;
+        Lr[N + x] = v[q].y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int b = line_start[rl + h], cnt = node_cnt[rl + h];
+        for (int k = t; k < cnt; k += P) {
+            const SpreadNode<T> nd = nodes[b + k];
+            Lr[h * N + nd.x] += nd.v[f];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < E; q++) {
+        const int x = t + q * P;
+        v[q] = cplx<T>{Lr[x], Lr[N + x]};
+    }
+    __syncthreads();
+}
+
+// P5 + P9: inverse x of (c_hat, dx c_hat, dy c_hat, dz c_hat, lambda k^2
+// c_hat) -> c and grad c (real, stored); f = (lambda/eps^2)(c^3 - c)
+// (src/main.cpp:81,153) and its forward x transform (the next step's CH
+// update takes it from there); mu = f + lambda (-lap c) (src/main.cpp:157);
+// the force F = mu grad c + particle spread + buoyancy
+// (src/main.cpp:149-172) and its forward x transform.  The order of the
+// sums is the reference's ((mu grad c + spread) + buoyancy); buoyancy
+// enters the z component only: g = (0, 0, gravity) (field.cpp:122-133).
+// In place: F_x over c_hat, F_y over dz c_hat, F_z over lambda k^2 c_hat.
+// Each of these arrays is consumed by a transform (whose LDS barriers
+// every thread of the workgroup passes) before it is overwritten.
+template <typename T, int N>
+__global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
+    cplx<T> *ch /* in: c_hat (X); out: F_x (X) */, const cplx<T> *__restrict__ dyc,
+    cplx<T> *dzc /* in: dz c_hat (X); out: F_y (X) */, cplx<T> *lap /* in: lambda k^2 c_hat (X); out: F_z (X) */,
+    cplx<T> *__restrict__ fout, T *c, T *__restrict__ gx, T *__restrict__ gy, T *__restrict__ gz,
+    const int *__restrict__ line_start, const int *__restrict__ node_cnt, const SpreadNode<T> *__restrict__ nodes,
+    int use_spread, T fscale, const T *__restrict__ W, T rho1, T rho2, T gravity, Dims d,
+    const double *__restrict__ kxv, const cplx<T> *__restrict__ tw)
 {
     FIPI_XPASS_SETUP(T, N)
     const size_t rA = zb * d.hs + (size_t)rl * d.nxp, rB = rA + d.nxp;
@@ -415,59 +455,61 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_force(cplx<T> *__restrict__ m
     // spread CSR over the lines of all boxes: box zb's lines follow box 0's
     line_start = boxp(line_start, zb, (size_t)(d.ny * d.nz));
     node_cnt = boxp(node_cnt, zb, (size_t)(d.ny * d.nz));
-    C muv[E];
-    T ga[3][E], gb[3][E], ca[E], cb[E];
-    load_c2r<T, N, false>(muv, mu_io, rA, rB, t, nullptr);
+    // mu <- lambda (-lap c), then + f(c); one array in flight per transform
+    C mu[E];
+    load_c2r<T, N, false>(mu, lap, rA, rB, t, nullptr);
+    fft_dist<T, N, +1>(mu, L, t, tw);
+    {
+        C cv[E], fv[E];
+        load_c2r<T, N, false>(cv, ch, rA, rB, t, nullptr);
+        fft_dist<T, N, +1>(cv, L, t, tw);
 #pragma unroll
-    for (int q = 0; q < E; q++) {
-        const int x = t + q * P;
-        ga[0][q] = gx[xA + x]; gb[0][q] = gx[xB + x];
-        ga[1][q] = gy[xA + x]; gb[1][q] = gy[xB + x];
-        ga[2][q] = gz[xA + x]; gb[2][q] = gz[xB + x];
-        ca[q] = c[xA + x]; cb[q] = c[xB + x];
-    }
-    fft_dist<T, N, +1>(muv, L, t, tw);
-#pragma unroll
-    for (int f = 0; f < 3; f++) {
-        C v[E];
-#pragma unroll
-        for (int q = 0; q < E; q++) v[q] = C{muv[q].x * ga[f][q], muv[q].y * gb[f][q]};
-        if (use_spread) {
-            T *Lr = reinterpret_cast<T *>(L);  // natural-order buffer: line rl then rl+1
-#pragma unroll
-            for (int q = 0; q < E; q++) {
-                const int x = t + q * P;
-                Lr[x] = v[q].x;
-                Lr[N + x] = v[q].y;
+        for (int q = 0; q < E; q++) {
+            const int x = t + q * P;
+            const T a = cv[q].x, b = cv[q].y;
+            c[xA + x] = a;
+            c[xB + x] = b;
+            fv[q] = C{(a * a * a - a) * fscale, (b * b * b - b) * fscale};
+            if (W) {
+                fv[q].x += W[xA + x];
+                fv[q].y += W[xB + x];
             }
-            __syncthreads();
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int b = line_start[rl + h], cnt = node_cnt[rl + h];
-                for (int k = t; k < cnt; k += P) {
-                    const SpreadNode<T> nd = nodes[b + k];
-                    Lr[h * N + nd.x] += nd.v[f];
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < E; q++) {
-                const int x = t + q * P;
-                v[q] = C{Lr[x], Lr[N + x]};
-            }
-            __syncthreads();
+            mu[q] = C{fv[q].x + mu[q].x, fv[q].y + mu[q].y};
         }
+        fft_dist<T, N, -1>(fv, L, t, tw);
+        store_r2c<T, N>(fv, L, fout, rA, rB, t);
+    }
+    // components in the order x, z, y: each output array is consumed first
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int f = k == 0 ? 0 : (k == 1 ? 2 : 1);
+        T *g = f == 0 ? gx : (f == 1 ? gy : gz);
+        C v[E];
+        if (f == 0) load_c2r<T, N, true>(v, ch, rA, rB, t, kxv);
+        else load_c2r<T, N, false>(v, f == 1 ? dyc : dzc, rA, rB, t, nullptr);
+        fft_dist<T, N, +1>(v, L, t, tw);
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            const int x = t + q * P;
+            g[xA + x] = v[q].x;
+            g[xB + x] = v[q].y;
+            v[q] = C{mu[q].x * v[q].x, mu[q].y * v[q].y};
+        }
+        if (use_spread) add_spread<T, N>(v, L, t, rl, f, line_start, node_cnt, nodes);
         if (f == 2) {
             // + ((1+c)/2*rho1 + (1-c)/2*rho2) * gravity, src/main.cpp:172
 #pragma unroll
             for (int q = 0; q < E; q++) {
-                const T ba = (ca[q] + T(1)) / T(2) * rho1 + (ca[q] * T(-1) + T(1)) / T(2) * rho2;
-                const T bb = (cb[q] + T(1)) / T(2) * rho1 + (cb[q] * T(-1) + T(1)) / T(2) * rho2;
+                // c as this thread stored it above (not kept in registers)
+                const int x = t + q * P;
+                const T ca = c[xA + x], cb = c[xB + x];
+                const T ba = (ca + T(1)) / T(2) * rho1 + (ca * T(-1) + T(1)) / T(2) * rho2;
+                const T bb = (cb + T(1)) / T(2) * rho1 + (cb * T(-1) + T(1)) / T(2) * rho2;
                 v[q] = C{v[q].x + ba * gravity, v[q].y + bb * gravity};
             }
         }
         fft_dist<T, N, -1>(v, L, t, tw);
-        store_r2c<T, N>(v, L, f == 0 ? mu_io : (f == 1 ? f1 : f2), rA, rB, t);
+        store_r2c<T, N>(v, L, f == 0 ? ch : (f == 1 ? dzc : lap), rA, rB, t);
     }
 }
 
@@ -487,15 +529,34 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_force(cplx<T> *__restrict__ m
     C *L = lds + lane;                                                \
     (void)E; (void)P;
 
-// 32-bit element offsets within one box (the box offset is applied to the
-// pointer, which stays uniform): half the address registers of 64-bit ones
+// Element offsets within one box (the box offset is applied to the
+// pointer, which stays uniform), turned into 32-bit byte offsets: a load or
+// store then takes the pointer in scalar registers and one 32-bit vector
+// offset, not a 64-bit vector address.  One box's array is below 4 GiB for
+// every supported edge length.
+template <typename C_>
+__device__ __forceinline__ const C_ *at_b32(const C_ *a, uint32_t i)
+{
+    return reinterpret_cast<const C_ *>(reinterpret_cast<const char *>(a) + (uint32_t)(i * (uint32_t)sizeof(C_)));
+}
+template <typename C_>
+__device__ __forceinline__ C_ *at_b32(C_ *a, uint32_t i)
+{
+    return reinterpret_cast<C_ *>(reinterpret_cast<char *>(a) + (uint32_t)(i * (uint32_t)sizeof(C_)));
+}
+
+// Lanes past the last column (ok false) load column 0 of their rows (every
+// caller clamps base), so the loads need no branch, and zero the values.
 template <typename T, int N>
 __device__ __forceinline__ void tile_load(cplx<T> *v, const cplx<T> *__restrict__ a, uint32_t base, uint32_t st,
                                           int t, bool ok)
 {
     constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
 #pragma unroll
-    for (int q = 0; q < E; q++) v[q] = ok ? a[base + st * (t + q * P)] : cplx<T>{0, 0};
+    for (int q = 0; q < E; q++) {
+        const cplx<T> x = *at_b32(a, base + st * (t + q * P));
+        v[q] = ok ? x : cplx<T>{0, 0};
+    }
 }
 
 template <typename T, int N>
@@ -505,7 +566,7 @@ __device__ __forceinline__ void tile_store(const cplx<T> *v, cplx<T> *__restrict
     constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
     if (ok) {
 #pragma unroll
-        for (int q = 0; q < E; q++) a[base + st * (t + q * P)] = v[q];
+        for (int q = 0; q < E; q++) *at_b32(a, base + st * (t + q * P)) = v[q];
     }
 }
 
@@ -582,20 +643,26 @@ struct K0 {
 // P3: CH update.  conv (XY) -> conv_hat; with c_hat^n, f_hat^n and the
 // history term hist = -c^{n-1} + 2dtM k^2 f^{n-1} + 2dt conv^{n-1}:
 //   c^{n+1} = (4c^n - 4dtM k^2 f^n - 4dt conv^n + hist) / (3 + 2dtM lambda k^4)
-// (src/main.cpp:117-127) or on the first step
+// (src/main.cpp:117-127) or on the first step (FIRST, a template argument
+// so that each instance carries one formula)
 //   c^1 = (c^0 - dtM k^2 f^0 - dt conv^0) / (1 + dtM lambda k^4)  (:106-113).
-// Outputs c^{n+1} (K), the new history (K), and c^{n+1}/G, i kz c^{n+1}/G
-// inverse-z transformed (XY) for the gradient and real-space c.
-template <typename T, int N, int TB>
+// Outputs c^{n+1} (K), the new history (K), and, inverse-z transformed
+// (XY): c^{n+1}/G, i kz c^{n+1}/G (gradient and real-space c) and, when
+// lap_out is set, lambda k^2 c^{n+1}/G, the gradient-energy part of the
+// chemical potential mu = f - lambda lap c (src/main.cpp:157), which the
+// force pass completes in real space.
+template <typename T, int N, int TB, bool FIRST>
 __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_ch(
     cplx<T> *__restrict__ conv_io /* in conv XY, out c XY */, cplx<T> *__restrict__ dzc_out,
-    const cplx<T> *__restrict__ c_now, cplx<T> *__restrict__ c_next, const cplx<T> *__restrict__ f_now,
-    cplx<T> *__restrict__ hist, KParams kp, Dims d, const double *__restrict__ kxv, const double *__restrict__ kyv,
-    const double *__restrict__ kzv, const cplx<T> *__restrict__ tw, K0 k0)
+    cplx<T> *__restrict__ lap_out, const cplx<T> *__restrict__ c_now, cplx<T> *__restrict__ c_next,
+    const cplx<T> *__restrict__ f_now, cplx<T> *__restrict__ hist, KParams kp, Dims d,
+    const double *__restrict__ kxv, const double *__restrict__ kyv, const double *__restrict__ kzv,
+    const cplx<T> *__restrict__ tw, K0 k0)
 {
     FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
     conv_io = boxp(conv_io, zb, d.hs);
     dzc_out = boxp(dzc_out, zb, d.hs);
+    lap_out = boxp(lap_out, zb, d.hs);
     c_now = boxp(c_now, zb, d.hs);
     c_next = boxp(c_next, zb, d.hs);
     f_now = boxp(f_now, zb, d.hs);
@@ -637,7 +704,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_ch(
         const double cnx = cn[q].x, cny = cn[q].y, fnx = fn[q].x, fny = fn[q].y, cvx = v[q].x, cvy = v[q].y;
         // one reciprocal of the (real) divisor per point
         C nc;
-        if (kp.first) {
+        if constexpr (FIRST) {
             const double m1 = k2 * (kp.dt * kp.mobility);
             const double rdiv = recip_nr(k2 * k2 * (kp.dt * kp.mobility * kp.lambda) + 1.0);
             double r = cnx - m1 * fnx, im = cny - m1 * fny;
@@ -665,7 +732,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_ch(
             const double c0 = kb[k0.cur], h0 = kb[2];
             const double cv0 = kp.use_conv ? k0.conv0[zb] : 0.0;
             double n0;
-            if (kp.first) n0 = c0 - cv0 * kp.dt;
+            if constexpr (FIRST) n0 = c0 - cv0 * kp.dt;
             else n0 = (c0 * 4.0 - cv0 * (4.0 * kp.dt) + h0) / 3.0;
             kb[k0.cur ^ 1] = n0;
             kb[2] = -c0 + cv0 * (2.0 * kp.dt);
@@ -677,16 +744,31 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_ch(
         }
         if (ok) {
             const uint32_t i = base + st * z;
-            c_next[i] = nc;
-            hist[i] = nh;
+            *at_b32(c_next, i) = nc;
+            *at_b32(hist, i) = nh;
         }
         const T kz = z == N / 2 ? T(0) : (T)kzz;
         w[q] = C{-kz * v[q].y, kz * v[q].x};
     }
-    fft_dist<T, N, +1, TL>(v, L, t, tw);
-    tile_store<T, N>(v, conv_io, base, st, t, ok);
     fft_dist<T, N, +1, TL>(w, L, t, tw);
     tile_store<T, N>(w, dzc_out, base, st, t, ok);
+    if (lap_out != nullptr) {
+        // lambda k^2 c/G from the rounded c/G: FP64 product, one rounding
+#pragma unroll
+        for (int q = 0; q < E; q++) {
+            const double kzz = kzv[t + q * P];
+            double k2 = 0.0;
+            k2 += kxx * kxx;
+            k2 += kyy * kyy;
+            k2 += kzz * kzz;
+            const double ml = k2 * kp.lambda;
+            w[q] = C{(T)(ml * (double)v[q].x), (T)(ml * (double)v[q].y)};
+        }
+        fft_dist<T, N, +1, TL>(w, L, t, tw);
+        tile_store<T, N>(w, lap_out, base, st, t, ok);
+    }
+    fft_dist<T, N, +1, TL>(v, L, t, tw);
+    tile_store<T, N>(v, conv_io, base, st, t, ok);
 }
 
 // Initial hub: (c XY, f XY) -> c_hat^0, f_hat^0 (K) and c/G, i kz c/G (XY).
@@ -727,51 +809,6 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_init(
     tile_store<T, N>(w, f_io, base, st, t, ok);
 }
 
-// P7: f (XY) -> f_hat^{n+1} (K); mu_hat = lambda k^2 c_hat + f_hat
-// (src/main.cpp:157) -> mu/G (XY, in place of f).
-template <typename T, int N, int TB>
-__global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_mu(cplx<T> *__restrict__ f_io,
-                                                                   cplx<T> *__restrict__ f_k,
-                                                                   const cplx<T> *__restrict__ c_k, KParams kp,
-                                                                   Dims d, const double *__restrict__ kxv,
-                                                                   const double *__restrict__ kyv,
-                                                                   const double *__restrict__ kzv,
-                                                                   const cplx<T> *__restrict__ tw)
-{
-    FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
-    f_io = boxp(f_io, zb, d.hs);
-    f_k = boxp(f_k, zb, d.hs);
-    c_k = boxp(c_k, zb, d.hs);
-    const int y = blockIdx.y;
-    const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
-    const uint32_t st = (uint32_t)d.nxp * d.ny;
-    C v[E], cc[E];
-    tile_load<T, N>(v, f_io, base, st, t, ok);
-    tile_load<T, N>(cc, c_k, base, st, t, ok);
-    fft_dist<T, N, -1, TL>(v, L, t, tw);
-    tile_store<T, N>(v, f_k, base, st, t, ok);
-    const double kxx = kxv[ok ? kx : 0], kyy = kyv[y];
-#pragma unroll
-    for (int q = 0; q < E; q++) {
-        const int z = t + q * P;
-        if (z >= N) {  // never: one point's FP64 temporaries live at a time
-            v[q] = C{T(0), T(0)};
-            continue;
-        }
-        const double kzz = kzv[z];
-        double k2 = 0.0;
-        k2 += kxx * kxx;
-        k2 += kyy * kyy;
-        k2 += kzz * kzz;
-        // FP64 arithmetic on the stored values, one rounding at the end
-        const double ml = k2 * kp.lambda;
-        v[q] = C{(T)((ml * (double)cc[q].x + (double)v[q].x) * kp.inv_g),
-                 (T)((ml * (double)cc[q].y + (double)v[q].y) * kp.inv_g)};
-    }
-    fft_dist<T, N, +1, TL>(v, L, t, tw);
-    tile_store<T, N>(v, f_io, base, st, t, ok);
-}
-
 // P11: Stokes, u_hat = (F - k (k.F)/k^2)/(mu k^2), 0 at k = 0
 // (src/main.cpp:177-178, division by zero gives zero, field.cpp:1010-1019).
 // The reference keeps the real part of the inverse complex transform; its
@@ -779,10 +816,14 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_mu(cp
 // Nyquist index (k_d(N/2) = +pi/dx on both sides).  The real part equals the
 // inverse of the Hermitian part, which drops the k_d k_e cross terms
 // between Nyquist and non-Nyquist components; that is what is formed here.
+// fz (optional): f (XY) -> f_hat (K) in place, the next CH update's input,
+// after the projection (its loads wait behind the projection's transforms:
+// fz is not restrict-qualified).
 template <typename T, int N, int TB>
 __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_stokes(cplx<T> *__restrict__ a0,
                                                                        cplx<T> *__restrict__ a1,
-                                                                       cplx<T> *__restrict__ a2, KParams kp, Dims d,
+                                                                       cplx<T> *__restrict__ a2, cplx<T> *fz,
+                                                                       KParams kp, Dims d,
                                                                        const double *__restrict__ kxv,
                                                                        const double *__restrict__ kyv,
                                                                        const double *__restrict__ kzv,
@@ -792,6 +833,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_stoke
     a0 = boxp(a0, zb, d.hs);
     a1 = boxp(a1, zb, d.hs);
     a2 = boxp(a2, zb, d.hs);
+    fz = boxp(fz, zb, d.hs);
     const int y = blockIdx.y;
     const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
     const uint32_t st = (uint32_t)d.nxp * d.ny;
@@ -852,6 +894,13 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_stoke
     tile_store<T, N>(v[1], a1, base, st, t, ok);
     fft_dist<T, N, +1, TL>(v[2], L, t, tw);
     tile_store<T, N>(v[2], a2, base, st, t, ok);
+    if (fz != nullptr) {
+        __builtin_amdgcn_sched_barrier(0);
+        C f[E];
+        tile_load<T, N>(f, fz, base, st, t, ok);
+        fft_dist<T, N, -1, TL>(f, L, t, tw);
+        tile_store<T, N>(f, fz, base, st, t, ok);
+    }
 }
 
 // Plain forward / inverse z transform of one field in place (unit entries).

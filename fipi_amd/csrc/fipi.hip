@@ -87,7 +87,7 @@ struct fipi_ctx {
     size_t esz = 8;  // bytes per real element
     // field state
     void *c = nullptr, *g[3] = {}, *u[3] = {};
-    void *h[3] = {};
+    void *h[4] = {};
     void *ck[2] = {}, *fk = nullptr, *hist = nullptr;
     int cur = 0;
     // h[0] already holds the next step's conv (X) and conv_part its FP64
@@ -98,6 +98,17 @@ struct fipi_ctx {
     // stages; their contact forces are spread every step unchanged)
     void *W = nullptr;
     bool fix_particles = false;
+    // Within fipi_step_async the particle stage of one step, and the spread
+    // of its forces for the next step, run on pstream beside the next
+    // step's first passes, which touch no particle state.  The stage reads
+    // c and grad c, so that next step writes them into the spare set
+    // (c_alt, g_alt; single-box contexts with particles) and swaps.  The
+    // main stream waits on ev_join before the force pass that reads the
+    // spread (spread_ready: it is built).
+    hipStream_t pstream = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    bool p_pending = false, spread_ready = false;
+    void *c_alt = nullptr, *g_alt[3] = {};
     double *kx = nullptr, *ky = nullptr, *kz = nullptr;
     void *twx = nullptr, *twy = nullptr, *twz = nullptr;
     // particles
@@ -353,35 +364,36 @@ struct Impl {
         });
     }
 
-    // c^0 -> c_hat^0, f_hat^0, grad c^0 (see set_state)
+    // c^0 -> c_hat^0, f_hat^0 (K), grad c^0 (see set_state)
     static void init_fields(fipi_ctx *ctx)
     {
         const Dims d = ctx->d;
+        C *h0 = H(ctx->h[0]), *h1 = H(ctx->h[1]), *h2 = H(ctx->h[2]);
         ProdCF<T> pr{R(ctx->c), fscale(ctx), (const T *)ctx->W};
-        x_forward<2>(ctx, pr, H(ctx->h[0]), H(ctx->h[1]));
-        y_pass<-1, 2, false>(ctx, H(ctx->h[0]), H(ctx->h[1]), nullptr, nullptr);
+        x_forward<2>(ctx, pr, h0, h1);
+        y_pass<-1, 2, false>(ctx, h0, h1, nullptr, nullptr);
         const KParams kp = kparams(ctx);
         with_n(d.nz, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
             with_tb(ctx->tune.hub_tb, [&](auto TBC) {
                 constexpr int TB = decltype(TBC)::value;
-                hipLaunchKernelGGL((k_hub_init<T, N, TB>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, ctx->stream,
-                                   H(ctx->h[0]), H(ctx->h[1]), H(ctx->ck[ctx->cur]), H(ctx->fk), kp, d, ctx->kz,
-                                   (const C *)ctx->twz);
+                hipLaunchKernelGGL((k_hub_init<T, N, TB>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, ctx->stream, h0, h1,
+                                   H(ctx->ck[ctx->cur]), H(ctx->fk), kp, d, ctx->kz, (const C *)ctx->twz);
             });
         });
-        y_pass<+1, 2, true>(ctx, H(ctx->h[0]), H(ctx->h[1]), nullptr, H(ctx->h[2]));
-        cgrad_f(ctx);
+        y_pass<+1, 2, true>(ctx, h0, h1, nullptr, h2);
+        cgrad_f(ctx, h2, h0);  // f (X) into h0 is not used
     }
 
-    static void cgrad_f(fipi_ctx *ctx)
+    // c, grad c from c_hat (h0), dy c (dyc), dz c (h1); f (X) into fout
+    static void cgrad_f(fipi_ctx *ctx, C *dyc, C *fout)
     {
         const Dims d = ctx->d;
         with_n(d.nx, [&](auto NC) {
             constexpr int N = decltype(NC)::value;
-            hipLaunchKernelGGL((k_x_cgrad_f<T, N>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, ctx->stream, H(ctx->h[0]), H(ctx->h[2]),
-                               H(ctx->h[1]), R(ctx->c), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), H(ctx->h[0]),
-                               fscale(ctx), d, ctx->kx, (const C *)ctx->twx, (const T *)ctx->W);
+            hipLaunchKernelGGL((k_x_cgrad_f<T, N>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, ctx->stream,
+                               H(ctx->h[0]), dyc, H(ctx->h[1]), R(ctx->c), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]),
+                               fout, fscale(ctx), d, ctx->kx, (const C *)ctx->twx, (const T *)ctx->W);
         });
     }
 
@@ -449,6 +461,42 @@ struct Impl {
         }
     }
 
+    // the main stream waits for the particle stage on pstream
+    static void join_particles(fipi_ctx *ctx)
+    {
+        if (!ctx->p_pending) return;
+        (void)hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+        ctx->p_pending = false;
+    }
+
+    // spread of the particle forces at the current positions (the next
+    // force pass adds them)
+    static void spread_now(fipi_ctx *ctx)
+    {
+        const PartState &ps = ctx->ps;
+        spread_build(ctx, ctx->pp, ps.p[0], ps.p[1], ps.p[2], ps.fpp[0], ps.fpp[1], ps.fpp[2], ps.box);
+    }
+
+    // both Heun stages, the step counter and the next step's spread on
+    // pstream, after the work queued so far on the main stream
+    static void particle_stages_beside(fipi_ctx *ctx)
+    {
+        hipStream_t main = ctx->stream;
+        (void)hipEventRecord(ctx->ev_fork, main);
+        (void)hipStreamWaitEvent(ctx->pstream, ctx->ev_fork, 0);
+        ctx->stream = ctx->pstream;  // the stage's launches go to pstream
+        particle_stage(ctx, 1);
+        particle_stage(ctx, 2);
+        hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, ctx->pstream, ctx->t_dev);
+        if (ctx->p.switch_ns != 0) {
+            spread_now(ctx);
+            ctx->spread_ready = true;
+        }
+        (void)hipEventRecord(ctx->ev_join, ctx->pstream);
+        ctx->stream = main;
+        ctx->p_pending = true;
+    }
+
     static double ramp(double t, double thr)
     {
         // primitive.cpp:122-129
@@ -474,13 +522,14 @@ struct Impl {
         }
     }
 
-    static void step(fipi_ctx *ctx)
+    // overlap_next: this step's particle stage may run beside the next step
+    static void step(fipi_ctx *ctx, bool overlap_next = false)
     {
         const Dims d = ctx->d;
         hipStream_t st = ctx->stream;
         const bool ns = ctx->p.switch_ns != 0;
         const bool parts = ctx->n > 0;
-        C *h0 = H(ctx->h[0]), *h1 = H(ctx->h[1]), *h2 = H(ctx->h[2]);
+        C *h0 = H(ctx->h[0]), *h1 = H(ctx->h[1]), *h2 = H(ctx->h[2]), *h3 = H(ctx->h[3]), *fk = H(ctx->fk);
         const double Rb = (double)ctx->G * sizeof(T), Hb = (double)ctx->GH * 2 * sizeof(T);
         const bool mixed = !std::is_same<T, double>::value;
         if (ns) {
@@ -503,75 +552,86 @@ struct Impl {
         const KParams kp = kparams(ctx);
         const int nxt = ctx->cur ^ 1;
         {
-            ProfScope ps_(ctx, "z_hub_ch", (ns ? 8 : 7) * Hb);
+            // reads conv, f_hat^n, c^n, history; writes c^{n+1}, history,
+            // c/G, dz c/G and (with Stokes) lambda k^2 c/G
+            ProfScope ps_(ctx, "z_hub_ch", (ns ? 9 : 7) * Hb);
             with_n(d.nz, [&](auto NC) {
                 constexpr int N = decltype(NC)::value;
                 with_tb(ctx->tune.hub_tb, [&](auto TBC) {
                     constexpr int TB = decltype(TBC)::value;
-                    hipLaunchKernelGGL((k_hub_ch<T, N, TB>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, st, h0, h1,
-                                       (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]), (const C *)ctx->fk,
-                                       H(ctx->hist), kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz,
-                                       K0{mixed ? ctx->k0 : nullptr, ctx->conv0, ctx->cur});
+                    const K0 k0{mixed ? ctx->k0 : nullptr, ctx->conv0, ctx->cur};
+                    if (kp.first)
+                        hipLaunchKernelGGL((k_hub_ch<T, N, TB, true>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, st, h0,
+                                           h1, ns ? h2 : nullptr, (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]),
+                                           (const C *)fk, H(ctx->hist), kp, d, ctx->kx, ctx->ky, ctx->kz,
+                                           (const C *)ctx->twz, k0);
+                    else
+                        hipLaunchKernelGGL((k_hub_ch<T, N, TB, false>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, st, h0,
+                                           h1, ns ? h2 : nullptr, (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]),
+                                           (const C *)fk, H(ctx->hist), kp, d, ctx->kx, ctx->ky, ctx->kz,
+                                           (const C *)ctx->twz, k0);
                 });
             });
         }
         ctx->cur = nxt;
         {
             ProfScope ps_(ctx, "y_inv_grad", 5 * Hb);
-            y_pass<+1, 2, true>(ctx, h0, h1, nullptr, h2);
-        }
-        {
-            ProfScope ps_(ctx, "x_inv_cgrad_f", 4 * Hb + 4 * Rb);
-            cgrad_f(ctx);
-        }
-        {
-            ProfScope ps_(ctx, "y_fwd_1", 2 * Hb);
-            y_pass<-1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
-        }
-        {
-            ProfScope ps_(ctx, "z_hub_mu", 4 * Hb);
-            with_n(d.nz, [&](auto NC) {
-                constexpr int N = decltype(NC)::value;
-                with_tb(ctx->tune.hub_tb, [&](auto TBC) {
-                    constexpr int TB = decltype(TBC)::value;
-                    hipLaunchKernelGGL((k_hub_mu<T, N, TB>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, st, h0,
-                                       H(ctx->fk), (const C *)ctx->ck[ctx->cur], kp, d, ctx->kx, ctx->ky, ctx->kz,
-                                       (const C *)ctx->twz);
-                });
-            });
+            y_pass<+1, 2, true>(ctx, h0, h1, nullptr, h3);
         }
         if (ns) {
+            ProfScope ps_(ctx, "y_inv_1", 2 * Hb);
+            y_pass<+1, 1, false>(ctx, h2, nullptr, nullptr, nullptr);
+        }
+        if (ctx->p_pending) {
+            // the previous step's particle stage still reads c and grad c
+            std::swap(ctx->c, ctx->c_alt);
+            for (int a = 0; a < 3; a++) std::swap(ctx->g[a], ctx->g_alt[a]);
+        }
+        if (!ns) {
             {
-                ProfScope ps_(ctx, "y_inv_1", 2 * Hb);
-                y_pass<+1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
-            }
-            if (parts) {
-                const PartState &ps = ctx->ps;
-                spread_build(ctx, ctx->pp, ps.p[0], ps.p[1], ps.p[2], ps.fpp[0], ps.fpp[1], ps.fpp[2], ps.box);
+                ProfScope ps_(ctx, "x_inv_cgrad_f", 4 * Hb + 4 * Rb);
+                cgrad_f(ctx, h3, fk);
             }
             {
-                ProfScope ps_(ctx, "x_inv_force_fwd", 4 * Hb + 4 * Rb + (parts ? 8.0 * ctx->n * sizeof(SpreadEntry<T>) : 0.0));
+                ProfScope ps_(ctx, "y_fwd_f", 2 * Hb);
+                y_pass<-1, 1, false>(ctx, fk, nullptr, nullptr, nullptr);
+            }
+            ProfScope ps_(ctx, "z_fwd_f", 2 * Hb);
+            z_pass<-1>(ctx, fk);
+        }
+        if (ns) {
+            join_particles(ctx);
+            if (parts && !ctx->spread_ready) spread_now(ctx);
+            ctx->spread_ready = false;
+            {
+                ProfScope ps_(ctx, "x_cgrad_force", 8 * Hb + 4 * Rb + (parts ? 8.0 * ctx->n * sizeof(SpreadNode<T>) : 0.0));
                 with_n(d.nx, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_x_force<T, N>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, st, h0, h1, h2, (const T *)R(ctx->c),
-                                       (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
-                                       (const int *)ctx->line_start, (const int *)ctx->node_cnt,
-                                       (const SpreadNode<T> *)ctx->spr_tmp, parts ? 1 : 0, (T)ctx->p.rho1,
-                                       (T)ctx->p.rho2, (T)ctx->p.gravity, d, (const C *)ctx->twx);
+                    hipLaunchKernelGGL((k_x_cgrad_force<T, N>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, st,
+                                       h0, (const C *)h3, h1, h2, fk, R(ctx->c), R(ctx->g[0]), R(ctx->g[1]),
+                                       R(ctx->g[2]), (const int *)ctx->line_start, (const int *)ctx->node_cnt,
+                                       (const SpreadNode<T> *)ctx->spr_tmp, parts ? 1 : 0, fscale(ctx),
+                                       (const T *)ctx->W, (T)ctx->p.rho1, (T)ctx->p.rho2, (T)ctx->p.gravity, d,
+                                       ctx->kx, (const C *)ctx->twx);
                 });
+            }
+            {
+                ProfScope ps_(ctx, "y_fwd_f", 2 * Hb);
+                y_pass<-1, 1, false>(ctx, fk, nullptr, nullptr, nullptr);
             }
             {
                 ProfScope ps_(ctx, "y_fwd_3", 6 * Hb);
                 y_pass<-1, 3, false>(ctx, h0, h1, h2, nullptr);
             }
             {
-                ProfScope ps_(ctx, "z_hub_stokes", 6 * Hb);
+                // and f (XY) -> f_hat (K) for the next CH update
+                ProfScope ps_(ctx, "z_hub_stokes", 8 * Hb);
                 with_n(d.nz, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
                     with_tb(ctx->tune.stokes_tb, [&](auto TBC) {
                         constexpr int TB = decltype(TBC)::value;
                         hipLaunchKernelGGL((k_hub_stokes<T, N, TB>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, st, h0,
-                                           h1, h2, kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
+                                           h1, h2, fk, kp, d, ctx->kx, ctx->ky, ctx->kz, (const C *)ctx->twz);
                     });
                 });
             }
@@ -595,11 +655,16 @@ struct Impl {
                 ctx->conv_ready = true;
             }
         }
-        if (parts && !ctx->fix_particles) {
-            particle_stage(ctx, 1);
-            particle_stage(ctx, 2);
+        join_particles(ctx);
+        if (parts && !ctx->fix_particles && overlap_next) {
+            particle_stages_beside(ctx);
+        } else {
+            if (parts && !ctx->fix_particles) {
+                particle_stage(ctx, 1);
+                particle_stage(ctx, 2);
+            }
+            hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, ctx->t_dev);
         }
-        hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, st, ctx->t_dev);
         ctx->t++;
     }
 
@@ -739,11 +804,16 @@ int setup_buffers(fipi_ctx *ctx)
 #define ALLOC(ptr, bytes) \
     if ((rc = dalloc(ctx, (void **)&(ptr), (bytes))) != FIPI_OK) return rc
     ALLOC(ctx->c, G * sizeof(T));
+    if (ctx->n > 0 && nb == 1) {
+        ALLOC(ctx->c_alt, G * sizeof(T));
+        for (int a = 0; a < 3; a++) ALLOC(ctx->g_alt[a], G * sizeof(T));
+    }
     for (int a = 0; a < 3; a++) {
         ALLOC(ctx->g[a], G * sizeof(T));
         ALLOC(ctx->u[a], G * sizeof(T));
         ALLOC(ctx->h[a], GH * 2 * sizeof(T));
     }
+    ALLOC(ctx->h[3], GH * 2 * sizeof(T));
     ALLOC(ctx->ck[0], GH * 2 * sizeof(T));
     ALLOC(ctx->ck[1], GH * 2 * sizeof(T));
     ALLOC(ctx->fk, GH * 2 * sizeof(T));
@@ -978,6 +1048,9 @@ static int create_ctx(const fipi_params *params, int nbox, int n_total, fipi_ctx
     ctx->n = params->switch_particle ? n_total : 0;
     hipError_t e = hipGetDevice(&ctx->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->pstream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
         rc = set_err(nullptr, FIPI_ERR_HIP, std::string("HIP init: ") + hipGetErrorString(e));
         delete ctx;
@@ -1003,10 +1076,14 @@ int fipi_destroy(fipi_ctx *ctx)
 {
     if (!ctx) return FIPI_OK;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->pstream) (void)hipStreamSynchronize(ctx->pstream);
     for (void *p : ctx->allocs) (void)hipFree(p);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
     if (ctx->h_flags) (void)hipHostFree(ctx->h_flags);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->pstream) (void)hipStreamDestroy(ctx->pstream);
     delete ctx;
     return FIPI_OK;
 }
@@ -1241,9 +1318,14 @@ int fipi_step_async(fipi_ctx *ctx, int32_t nsteps)
     if (!ctx) return set_err(ctx, FIPI_ERR_ARG, "ctx is NULL");
     if (!ctx->has_state) return set_err(ctx, FIPI_ERR_STATE, "fipi_set_state must be called first");
     if (nsteps < 0) return set_err(ctx, FIPI_ERR_ARG, "nsteps < 0");
+    // each step's particle stage runs beside the next step of this call
+    // (never under per-launch timing, whose events are on the main stream);
+    // the last step's runs in order, so all work ends on the main stream
+    const bool overlap = ctx->c_alt != nullptr && !ctx->prof_on && !ctx->fix_particles;
     for (int s = 0; s < nsteps; s++) {
-        if (ctx->fp64) Impl<double>::step(ctx);
-        else Impl<float>::step(ctx);
+        const bool next = overlap && s + 1 < nsteps;
+        if (ctx->fp64) Impl<double>::step(ctx, next);
+        else Impl<float>::step(ctx, next);
     }
     HIPCHK(ctx, hipGetLastError());
     return FIPI_OK;
@@ -1467,6 +1549,8 @@ int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32
 // ---------------------------------------------------------------------------
 namespace {
 
+constexpr int32_t SNAP_VERSION = 1;
+
 struct SnapHeader {
     char magic[8];
     int32_t version, nx, ny, nz, n, nbox, precision, t, cur, conv_ready, has_ctheta, reserved;
@@ -1527,7 +1611,7 @@ int fipi_snapshot_save(fipi_ctx *ctx, void *buf, int64_t size)
     if (size < need) return set_err(ctx, FIPI_ERR_OVERFLOW, "snapshot buffer too small");
     SnapHeader h{};
     std::memcpy(h.magic, "FIPISNAP", 8);
-    h.version = 1;
+    h.version = SNAP_VERSION;
     h.nx = ctx->p.nx;
     h.ny = ctx->p.ny;
     h.nz = ctx->p.nz;
@@ -1555,7 +1639,8 @@ int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size)
     if (size < (int64_t)sizeof(SnapHeader)) return set_err(ctx, FIPI_ERR_ARG, "not a snapshot");
     SnapHeader h;
     std::memcpy(&h, buf, sizeof(h));
-    if (std::memcmp(h.magic, "FIPISNAP", 8) != 0 || h.version != 1) return set_err(ctx, FIPI_ERR_ARG, "not a snapshot");
+    if (std::memcmp(h.magic, "FIPISNAP", 8) != 0 || h.version != SNAP_VERSION)
+        return set_err(ctx, FIPI_ERR_ARG, "not a snapshot of this format version");
     if (h.nx != ctx->p.nx || h.ny != ctx->p.ny || h.nz != ctx->p.nz || h.n != ctx->n || h.nbox != ctx->nbox ||
         h.precision != ctx->p.precision)
         return set_err(ctx, FIPI_ERR_ARG, "snapshot of a different grid, particle count or precision");
