@@ -321,6 +321,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--tune-steps", type=int, default=3,
+                    help="steps per tile-width trial before the run (0: defaults)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="run one device-to-device copy of this size first (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -352,6 +354,7 @@ def main():
         q = p.replace(precision=precision)
         sim = fipi_amd.Simulation(q)
         sim.set_state(phi0, pos, None, cos_theta)
+        tuning = sim.tune(args.tune_steps) if args.tune_steps > 0 else sim.tuning()
         sim.step(args.warmup)
         stream = torch.cuda.ExternalStream(sim.stream_ptr)
         e0 = torch.cuda.Event(enable_timing=True)
@@ -366,6 +369,7 @@ def main():
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         res = {
+            "tuning": tuning,
             "wall": wall,
             "ev_ms": e0.elapsed_time(e1),
             "phi": sim.phi(),
@@ -490,6 +494,7 @@ def main():
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "kernels": kernel_table(stats),
         "kernels_other_precision": kernel_table(summarize(other_run.get("stats", []))) if other_run else None,
+        "tile_widths": {"headline": main_run["tuning"], "other_precision": other_run["tuning"] if other_run else None},
         "diagnostics": {k: diag[k] for k in ("t", "mean_phi", "free_energy", "interface_area")},
         "ensemble": ens,
         "adjoint": adj,

@@ -36,7 +36,7 @@ ABI_SYMBOLS = [
     "fipi_step", "fipi_step_async", "fipi_sync", "fipi_profile_steps",
     "fipi_fft_r2c", "fipi_fft_c2r", "fipi_cells_build", "fipi_contact_pairs",
     "fipi_contact_forces", "fipi_interp", "fipi_spread",
-    "fipi_snapshot_size", "fipi_snapshot_save", "fipi_snapshot_load",
+    "fipi_snapshot_size", "fipi_snapshot_save", "fipi_snapshot_load", "fipi_tune", "fipi_tuning",
     "fipi_ensemble_create", "fipi_ensemble_destroy", "fipi_ensemble_last_error", "fipi_ensemble_set_box",
     "fipi_ensemble_tune", "fipi_ensemble_step", "fipi_ensemble_step_async", "fipi_ensemble_sync",
     "fipi_ensemble_stream", "fipi_ensemble_get_phi", "fipi_ensemble_get_particles",
@@ -137,6 +137,8 @@ def lib():
     L.fipi_snapshot_size.restype = i64
     L.fipi_snapshot_save.argtypes = [vp, vp, i64]
     L.fipi_snapshot_load.argtypes = [vp, vp, i64]
+    L.fipi_tune.argtypes = [vp, C.c_int32]
+    L.fipi_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.fipi_ensemble_create.argtypes = [C.POINTER(CParams), C.c_int32, C.POINTER(CBoxDesc), C.POINTER(vp)]
     L.fipi_ensemble_destroy.argtypes = [vp]
     L.fipi_ensemble_last_error.argtypes = [vp]
@@ -312,6 +314,17 @@ class Simulation:
         """Continue from a snapshot of a context with the same parameters."""
         buf = C.create_string_buffer(snap, len(snap))
         self._c(lib().fipi_snapshot_load(self._h, buf, len(snap)))
+
+    def tune(self, steps: int = 3) -> dict:
+        """Pick the fastest tile widths over `steps` steps each; the state is
+        left as it was.  Returns the widths in bytes."""
+        self._c(lib().fipi_tune(self._h, steps))
+        return self.tuning()
+
+    def tuning(self) -> dict:
+        y, h, s = C.c_int32(), C.c_int32(), C.c_int32()
+        self._c(lib().fipi_tuning(self._h, C.byref(y), C.byref(h), C.byref(s)))
+        return {"y_tile_bytes": y.value, "hub_tile_bytes": h.value, "stokes_tile_bytes": s.value}
 
     # ---- unit entry points on device memory (torch tensors) --------------
     # torch only provides device memory here; the work runs in libfipi.

@@ -1661,4 +1661,79 @@ int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size)
     return FIPI_OK;
 }
 
+int fipi_tune(fipi_ctx *ctx, int32_t steps)
+{
+    if (!ctx || steps < 1) return set_err(ctx, FIPI_ERR_ARG, "invalid argument");
+    if (!ctx->has_state) return set_err(ctx, FIPI_ERR_STATE, "fipi_set_state must be called first");
+    struct DevBuf {
+        void *p = nullptr;
+        hipEvent_t a = nullptr, b = nullptr;
+        ~DevBuf()
+        {
+            if (p) (void)hipFree(p);
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } keep;
+    size_t total = 0;
+    for (auto &x : snapshot_buffers(ctx)) total += x.second;
+    HIPCHK(ctx, hipMalloc(&keep.p, total));
+    HIPCHK(ctx, hipEventCreate(&keep.a));
+    HIPCHK(ctx, hipEventCreate(&keep.b));
+    {
+        char *o = (char *)keep.p;
+        for (auto &x : snapshot_buffers(ctx)) {
+            HIPCHK(ctx, hipMemcpyAsync(o, x.first, x.second, hipMemcpyDeviceToDevice, ctx->stream));
+            o += x.second;
+        }
+    }
+    const int t0 = ctx->t, cur0 = ctx->cur;
+    const bool conv0 = ctx->conv_ready;
+    fipi_ctx::Tune best = ctx->tune;
+    float best_ms = 1e30f;
+    for (int y : {64, 128})
+        for (int h : {64, 128})
+            for (int st : {64, 128}) {
+                ctx->tune.y_tb = y;
+                ctx->tune.hub_tb = h;
+                ctx->tune.stokes_tb = st;
+                int rc = fipi_step_async(ctx, 1);  // warm-up (first use of these kernels)
+                if (rc) return rc;
+                HIPCHK(ctx, hipEventRecord(keep.a, ctx->stream));
+                if ((rc = fipi_step_async(ctx, steps))) return rc;
+                HIPCHK(ctx, hipEventRecord(keep.b, ctx->stream));
+                HIPCHK(ctx, hipEventSynchronize(keep.b));
+                float ms = 0.f;
+                HIPCHK(ctx, hipEventElapsedTime(&ms, keep.a, keep.b));
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = ctx->tune;
+                }
+            }
+    int rc = check_flags(ctx);
+    ctx->tune = best;
+    // back to the state before the trials (into the arrays in use now: the
+    // spare c / grad c set may have been swapped in)
+    const char *in = (const char *)keep.p;
+    for (auto &x : snapshot_buffers(ctx)) {
+        HIPCHK(ctx, hipMemcpyAsync(x.first, in, x.second, hipMemcpyDeviceToDevice, ctx->stream));
+        in += x.second;
+    }
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->t = t0;
+    ctx->cur = cur0;
+    ctx->conv_ready = conv0;
+    return rc;
+}
+
+int fipi_tuning(const fipi_ctx *ctx, int32_t *y_tb, int32_t *hub_tb, int32_t *stokes_tb)
+{
+    if (!ctx || !y_tb || !hub_tb || !stokes_tb) return FIPI_ERR_ARG;
+    *y_tb = ctx->tune.y_tb;
+    *hub_tb = ctx->tune.hub_tb;
+    *stokes_tb = ctx->tune.stokes_tb;
+    return FIPI_OK;
+}
+
 }  // extern "C"

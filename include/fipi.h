@@ -180,6 +180,16 @@ int64_t fipi_snapshot_size(fipi_ctx *ctx);
 int fipi_snapshot_save(fipi_ctx *ctx, void *buf, int64_t size);
 int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size);
 
+/* ---- tile-width tuning ----------------------------------------------- */
+/* No counterpart in the reference (FFTW plans, field.cpp:24-139, are made
+ * once with fixed options).  Times `steps` steps with each tile width (64 or
+ * 128 bytes) of the y passes, the CH hub and the Stokes hub, keeps the
+ * fastest, and restores the state from before the trials (kept in device
+ * memory meanwhile; about the size of a snapshot).  Results do not depend
+ * on the tile widths, only the speed does.  fipi_tuning reads them. */
+int fipi_tune(fipi_ctx *ctx, int32_t steps);
+int fipi_tuning(const fipi_ctx *ctx, int32_t *y_tb, int32_t *hub_tb, int32_t *stokes_tb);
+
 /* ---- batched ensemble of independent boxes ---------------------------- */
 /* No counterpart in the reference, which runs one box per process
  * (src/main.cpp:19-252).  Many independent periodic cubic boxes of mixed

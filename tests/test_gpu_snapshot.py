@@ -46,3 +46,49 @@ def test_snapshot_rejects_other_grid(gpu):
     with pytest.raises(fipi_amd.FipiError):
         b.restore(b"not a snapshot at all")
     b.close()
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32_MIXED])
+def test_tune_keeps_state(gpu, prec):
+    """fipi_tune times every tile-width choice and restores the state: the
+    run continues as if it had not been called (results do not depend on
+    the tile widths)."""
+    print(f"tune prec={prec}", flush=True)
+    p, phi0, pos = make_case(32, 120, steps=40, seed=10, precision=prec)
+    a = fipi_amd.Simulation(p)
+    a.set_state(phi0, pos)
+    a.step(3)
+    widths = a.tune(2)
+    assert set(widths.values()) <= {64, 128}
+    assert a.tuning() == widths
+    assert a.t == 3
+    a.step(6)
+    b = fipi_amd.Simulation(p)
+    b.set_state(phi0, pos)
+    b.step(3)
+    b.step(6)
+    assert np.array_equal(a.phi(), b.phi())
+    assert np.array_equal(a.particles()["p"], b.particles()["p"])
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("prec", [FP64, FP32_MIXED])
+def test_one_call_matches_single_steps(gpu, prec):
+    """Within one fipi_step call each step's particle stage runs on a second
+    stream beside the next step's field passes; step-by-step calls run it in
+    order.  Both give the same state bit for bit."""
+    print(f"overlap prec={prec}", flush=True)
+    p, phi0, pos = make_case(32, 150, steps=40, seed=12, precision=prec)
+    a = fipi_amd.Simulation(p)
+    a.set_state(phi0, pos)
+    a.step(12)
+    b = fipi_amd.Simulation(p)
+    b.set_state(phi0, pos)
+    for _ in range(12):
+        b.step(1)
+    assert np.array_equal(a.phi(), b.phi())
+    assert np.array_equal(a.grad_phi(), b.grad_phi())
+    assert np.array_equal(a.particles()["p"], b.particles()["p"])
+    a.close()
+    b.close()
