@@ -1212,7 +1212,6 @@ int fipi_get_phi(fipi_ctx *ctx, double *phi)
 int fipi_get_velocity(fipi_ctx *ctx, double *u)
 {
     if (!ctx || !u) return set_err(ctx, FIPI_ERR_ARG, "ctx and u are required");
-    ctx->conv_ready = false;
     for (int a = 0; a < 3; a++) {
         int rc = copy_out_real(ctx, u + (size_t)a * ctx->G, ctx->u[a], ctx->G);
         if (rc) return rc;
