@@ -442,7 +442,7 @@ def main():
             "after_steps": args.warmup + args.steps,
             "phi_rel_l2": float(np.linalg.norm(mix["phi"] - f64["phi"]) / np.linalg.norm(f64["phi"])),
             "interface_area_rel_err": float(abs(amx / a64 - 1.0)),
-            "max_particle_disp_err": float(np.abs(dp).max()),
+            "max_particle_disp_err": float(np.abs(dp).max()) if dp.size else 0.0,
             "mean_phi_abs_err": float(abs(mix["diag"]["mean_phi"] - f64["diag"]["mean_phi"])),
         }
 

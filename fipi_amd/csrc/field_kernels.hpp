@@ -444,7 +444,7 @@ template <typename T, int N>
 __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
     cplx<T> *ch /* in: c_hat (X); out: F_x (X) */, const cplx<T> *__restrict__ dyc,
     cplx<T> *dzc /* in: dz c_hat (X); out: F_y (X) */, cplx<T> *lap /* in: lambda k^2 c_hat (X); out: F_z (X) */,
-    cplx<T> *__restrict__ fout, T *c, T *__restrict__ gx, T *__restrict__ gy, T *__restrict__ gz,
+    cplx<T> *__restrict__ fout, T *__restrict__ c, T *__restrict__ gx, T *__restrict__ gy, T *__restrict__ gz,
     const int *__restrict__ line_start, const int *__restrict__ node_cnt, const SpreadNode<T> *__restrict__ nodes,
     int use_spread, T fscale, const T *__restrict__ W, T rho1, T rho2, T gravity, Dims d,
     const double *__restrict__ kxv, const cplx<T> *__restrict__ tw)
@@ -455,12 +455,27 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
     // spread CSR over the lines of all boxes: box zb's lines follow box 0's
     line_start = boxp(line_start, zb, (size_t)(d.ny * d.nz));
     node_cnt = boxp(node_cnt, zb, (size_t)(d.ny * d.nz));
+    // Thread-private LDS: the buoyancy of this thread's points and (FP32)
+    // the packed i kx c_hat input of the x component, each written and read
+    // by the same thread (no barrier).  They replace second reads from
+    // global memory; FP64 keeps the second read of c_hat (the extra LDS
+    // would cost occupancy).
+    constexpr bool DX_LDS = sizeof(T) == 4;
+    __shared__ T bst[LPW * 2 * N];
+    __shared__ C dst[DX_LDS ? LPW * N : 1];
+    T *bq = bst + lp * 2 * N;
     // mu <- lambda (-lap c), then + f(c); one array in flight per transform
     C mu[E];
     load_c2r<T, N, false>(mu, lap, rA, rB, t, nullptr);
     fft_dist<T, N, +1>(mu, L, t, tw);
     {
         C cv[E], fv[E];
+        if constexpr (DX_LDS) {
+            C dv[E];
+            load_c2r<T, N, true>(dv, ch, rA, rB, t, kxv);
+#pragma unroll
+            for (int q = 0; q < E; q++) dst[lp * N + t + q * P] = dv[q];
+        }
         load_c2r<T, N, false>(cv, ch, rA, rB, t, nullptr);
         fft_dist<T, N, +1>(cv, L, t, tw);
 #pragma unroll
@@ -469,6 +484,9 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
             const T a = cv[q].x, b = cv[q].y;
             c[xA + x] = a;
             c[xB + x] = b;
+            // ((1+c)/2*rho1 + (1-c)/2*rho2), src/main.cpp:172
+            bq[x] = (a + T(1)) / T(2) * rho1 + (a * T(-1) + T(1)) / T(2) * rho2;
+            bq[N + x] = (b + T(1)) / T(2) * rho1 + (b * T(-1) + T(1)) / T(2) * rho2;
             fv[q] = C{(a * a * a - a) * fscale, (b * b * b - b) * fscale};
             if (W) {
                 fv[q].x += W[xA + x];
@@ -485,7 +503,14 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
         const int f = k == 0 ? 0 : (k == 1 ? 2 : 1);
         T *g = f == 0 ? gx : (f == 1 ? gy : gz);
         C v[E];
-        if (f == 0) load_c2r<T, N, true>(v, ch, rA, rB, t, kxv);
+        if (f == 0) {
+            if constexpr (DX_LDS) {
+#pragma unroll
+                for (int q = 0; q < E; q++) v[q] = dst[lp * N + t + q * P];
+            } else {
+                load_c2r<T, N, true>(v, ch, rA, rB, t, kxv);
+            }
+        }
         else load_c2r<T, N, false>(v, f == 1 ? dyc : dzc, rA, rB, t, nullptr);
         fft_dist<T, N, +1>(v, L, t, tw);
 #pragma unroll
@@ -497,15 +522,11 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
         }
         if (use_spread) add_spread<T, N>(v, L, t, rl, f, line_start, node_cnt, nodes);
         if (f == 2) {
-            // + ((1+c)/2*rho1 + (1-c)/2*rho2) * gravity, src/main.cpp:172
+            // + buoyancy * gravity, src/main.cpp:172
 #pragma unroll
             for (int q = 0; q < E; q++) {
-                // c as this thread stored it above (not kept in registers)
                 const int x = t + q * P;
-                const T ca = c[xA + x], cb = c[xB + x];
-                const T ba = (ca + T(1)) / T(2) * rho1 + (ca * T(-1) + T(1)) / T(2) * rho2;
-                const T bb = (cb + T(1)) / T(2) * rho1 + (cb * T(-1) + T(1)) / T(2) * rho2;
-                v[q] = C{v[q].x + ba * gravity, v[q].y + bb * gravity};
+                v[q] = C{v[q].x + bq[x] * gravity, v[q].y + bq[N + x] * gravity};
             }
         }
         fft_dist<T, N, -1>(v, L, t, tw);
