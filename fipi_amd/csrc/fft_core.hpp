@@ -194,7 +194,8 @@ template <int R, int SIGN, typename C> __device__ __forceinline__ void dft(C *v)
 // divide E the N/M butterflies of the middle stage are spread over the P
 // threads unevenly (some threads take one more).  Lines of 256 use E = 8
 // by default (fewer registers for kernels that keep three fields in
-// flight; E = 16 would need one LDS exchange instead of two).
+// flight); a kernel may choose another E (the FP32 y passes use 16: one LDS
+// exchange instead of two, twice the bytes in flight per thread).
 // ---------------------------------------------------------------------------
 template <int N, int E> struct PlanE;
 template <> struct PlanE<16, 4> { static constexpr int M = 0; };
@@ -220,8 +221,9 @@ template <typename T, int N> struct PlanE_for {
 template <> struct PlanE_for<double, 256> { static constexpr int E = FIPI_E256_F64; };
 template <> struct PlanE_for<float, 256> { static constexpr int E = FIPI_E256_F32; };
 
-template <typename T, int N> struct Line {
-    static constexpr int E = PlanE_for<T, N>::E;   // values per thread
+// EV: values per thread of a kernel that does not use the default
+template <typename T, int N, int EV = PlanE_for<T, N>::E> struct Line {
+    static constexpr int E = EV;                   // values per thread
     static constexpr int M = PlanE<N, E>::M;       // middle radix (0: none)
     static constexpr int P = N / E;                // threads per line
     static constexpr int LL = N + N / E;           // padded LDS slots per line
@@ -238,10 +240,10 @@ __device__ __forceinline__ C tw_at(const C *__restrict__ tw, int idx)
     return w;
 }
 
-template <typename T, int N, int SIGN, int S>
+template <typename T, int N, int SIGN, int S, int EV>
 __device__ __forceinline__ void mid_stage(cplx<T> *lds, int t, const cplx<T> *__restrict__ tw)
 {
-    constexpr int E = Line<T, N>::E, P = Line<T, N>::P, RM = Line<T, N>::M, NS = E;
+    constexpr int E = EV, P = Line<T, N, EV>::P, RM = Line<T, N, EV>::M, NS = E;
     constexpr int B = N / RM;               // butterflies of the stage
     constexpr int NB = (B + P - 1) / P;     // per thread (the last round may be partial)
     using C = cplx<T>;
@@ -275,15 +277,15 @@ __device__ __forceinline__ void mid_stage(cplx<T> *lds, int t, const cplx<T> *__
 // In-place transform of one line held in D by threads t < P (every thread of
 // the workgroup must call it: it contains barriers).  lds: slot 0 of this
 // line; slots are S values apart.
-template <typename T, int N, int SIGN, int S = 1>
+template <typename T, int N, int SIGN, int S = 1, int EV = PlanE_for<T, N>::E>
 __device__ __forceinline__ void fft_dist(cplx<T> *v, cplx<T> *lds, int t, const cplx<T> *__restrict__ tw)
 {
-    constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
+    constexpr int E = EV, P = Line<T, N, EV>::P;
     dft<E, SIGN>(v);
 #pragma unroll
     for (int q = 0; q < E; q++) lds[lpad<E>(t * E + q) * S] = v[q];
     __syncthreads();
-    if constexpr (Line<T, N>::M > 0) mid_stage<T, N, SIGN, S>(lds, t, tw);
+    if constexpr (Line<T, N, EV>::M > 0) mid_stage<T, N, SIGN, S, EV>(lds, t, tw);
 #pragma unroll
     for (int q = 0; q < E; q++) v[q] = lds[lpad<E>(t + q * P) * S];
 #pragma unroll

@@ -538,8 +538,8 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
 // y and z passes: a tile of TL neighbouring kx columns; lane = column,
 // t = position in the line (distribution D along y or z).
 // ---------------------------------------------------------------------------
-#define FIPI_TILE_SETUP(T, N, TLV)                                   \
-    constexpr int E = Line<T, N>::E, P = Line<T, N>::P, LL = Line<T, N>::LL;   \
+#define FIPI_TILE_SETUP_E(T, N, TLV, EV)                             \
+    constexpr int E = Line<T, N, EV>::E, P = Line<T, N, EV>::P, LL = Line<T, N, EV>::LL; \
     constexpr int TL = TLV;                                           \
     using C = cplx<T>;                                                \
     __shared__ C lds[TL * LL];                                        \
@@ -549,6 +549,14 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
     const size_t zb = blockIdx.z;                                     \
     C *L = lds + lane;                                                \
     (void)E; (void)P;
+#define FIPI_TILE_SETUP(T, N, TLV) FIPI_TILE_SETUP_E(T, N, TLV, (PlanE_for<T, N>::E))
+
+// values per thread of the y passes: 256-point FP32 lines use 16 (one LDS
+// exchange, twice the bytes in flight per thread; these passes hold at most
+// three fields), everything else the default of its size
+template <typename T, int N> struct YE {
+    static constexpr int v = (sizeof(T) == 4 && N == 256) ? 16 : PlanE_for<T, N>::E;
+};
 
 // Element offsets within one box (the box offset is applied to the
 // pointer, which stays uniform), turned into 32-bit byte offsets: a load or
@@ -568,11 +576,11 @@ __device__ __forceinline__ C_ *at_b32(C_ *a, uint32_t i)
 
 // Lanes past the last column (ok false) load column 0 of their rows (every
 // caller clamps base), so the loads need no branch, and zero the values.
-template <typename T, int N>
+template <typename T, int N, int EV = PlanE_for<T, N>::E>
 __device__ __forceinline__ void tile_load(cplx<T> *v, const cplx<T> *__restrict__ a, uint32_t base, uint32_t st,
                                           int t, bool ok)
 {
-    constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
+    constexpr int E = EV, P = Line<T, N, EV>::P;
 #pragma unroll
     for (int q = 0; q < E; q++) {
         const cplx<T> x = *at_b32(a, base + st * (t + q * P));
@@ -580,11 +588,11 @@ __device__ __forceinline__ void tile_load(cplx<T> *v, const cplx<T> *__restrict_
     }
 }
 
-template <typename T, int N>
+template <typename T, int N, int EV = PlanE_for<T, N>::E>
 __device__ __forceinline__ void tile_store(const cplx<T> *v, cplx<T> *__restrict__ a, uint32_t base, uint32_t st,
                                            int t, bool ok)
 {
-    constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
+    constexpr int E = EV, P = Line<T, N, EV>::P;
     if (ok) {
 #pragma unroll
         for (int q = 0; q < E; q++) *at_b32(a, base + st * (t + q * P)) = v[q];
@@ -594,12 +602,13 @@ __device__ __forceinline__ void tile_store(const cplx<T> *v, cplx<T> *__restrict
 // y pass: transforms along y of NF fields in place.  GRAD (inverse only):
 // field 0 also yields i*ky*field0 into `extra` (zero on ky = NY/2).
 template <typename T, int N, int SIGN, int NF, bool GRAD, int TB>
-__global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_y(cplx<T> *__restrict__ a0, cplx<T> *__restrict__ a1,
+__global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N, YE<T, N>::v>::P)) void k_y(cplx<T> *__restrict__ a0, cplx<T> *__restrict__ a1,
                                                               cplx<T> *__restrict__ a2, cplx<T> *__restrict__ extra,
                                                               Dims d, const double *__restrict__ kyv,
                                                               const cplx<T> *__restrict__ tw)
 {
-    FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
+    constexpr int EY = YE<T, N>::v;
+    FIPI_TILE_SETUP_E(T, N, (TileW<T, TB>::L), EY)
     a0 = boxp(a0, zb, d.hs);
     a1 = boxp(a1, zb, d.hs);
     a2 = boxp(a2, zb, d.hs);
@@ -609,7 +618,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_y(cplx<T>
     const uint32_t st = d.nxp;
     C v[NF][E];
 #pragma unroll
-    for (int f = 0; f < NF; f++) tile_load<T, N>(v[f], f == 0 ? a0 : (f == 1 ? a1 : a2), base, st, t, ok);
+    for (int f = 0; f < NF; f++) tile_load<T, N, EY>(v[f], f == 0 ? a0 : (f == 1 ? a1 : a2), base, st, t, ok);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         C *a = f == 0 ? a0 : (f == 1 ? a1 : a2);
@@ -621,11 +630,11 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_y(cplx<T>
                 const T kv = y == N / 2 ? T(0) : (T)kyv[y];
                 w[q] = C{-kv * v[0][q].y, kv * v[0][q].x};
             }
-            fft_dist<T, N, SIGN, TL>(w, L, t, tw);
-            tile_store<T, N>(w, extra, base, st, t, ok);
+            fft_dist<T, N, SIGN, TL, EY>(w, L, t, tw);
+            tile_store<T, N, EY>(w, extra, base, st, t, ok);
         }
-        fft_dist<T, N, SIGN, TL>(v[f], L, t, tw);
-        tile_store<T, N>(v[f], a, base, st, t, ok);
+        fft_dist<T, N, SIGN, TL, EY>(v[f], L, t, tw);
+        tile_store<T, N, EY>(v[f], a, base, st, t, ok);
     }
 }
 

@@ -331,7 +331,7 @@ struct Impl {
                 constexpr int TB = decltype(TBC)::value, TL = TileW<T, TB>::L;
                 hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD, TB>),
                                    dim3((unsigned)((d.nxh + TL - 1) / TL), d.nz, ctx->nbox),
-                                   dim3((TL * Line<T, N>::P)), 0, ctx->stream, a0, a1, a2, extra, d, ctx->ky,
+                                   dim3((TL * Line<T, N, YE<T, N>::v>::P)), 0, ctx->stream, a0, a1, a2, extra, d, ctx->ky,
                                    (const C *)ctx->twy);
             });
         });
