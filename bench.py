@@ -98,7 +98,7 @@ def ensemble_run(nboxes, steps, loop_sample, loop_steps):
     ens.sync()
     wall = time.perf_counter() - t0
     m0 = [ens.diagnostics(b)["mean_phi"] for b in range(min(nboxes, 8))]
-    tuning = {str(n): dict(zip(("y_tile_bytes", "hub_tile_bytes", "stokes_tile_bytes"), ens.tuning(n)))
+    tuning = {str(n): ens.tuning(n)
               for n in EDGE_SET}
     ens.close()
     del ens
@@ -494,7 +494,7 @@ def main():
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "kernels": kernel_table(stats),
         "kernels_other_precision": kernel_table(summarize(other_run.get("stats", []))) if other_run else None,
-        "tile_widths": {"headline": main_run["tuning"], "other_precision": other_run["tuning"] if other_run else None},
+        "tuning": {"headline": main_run["tuning"], "other_precision": other_run["tuning"] if other_run else None},
         "diagnostics": {k: diag[k] for k in ("t", "mean_phi", "free_energy", "interface_area")},
         "ensemble": ens,
         "adjoint": adj,

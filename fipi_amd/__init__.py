@@ -36,11 +36,13 @@ ABI_SYMBOLS = [
     "fipi_step", "fipi_step_async", "fipi_sync", "fipi_profile_steps",
     "fipi_fft_r2c", "fipi_fft_c2r", "fipi_cells_build", "fipi_contact_pairs",
     "fipi_contact_forces", "fipi_interp", "fipi_spread",
-    "fipi_snapshot_size", "fipi_snapshot_save", "fipi_snapshot_load", "fipi_tune", "fipi_tuning",
+    "fipi_snapshot_size", "fipi_snapshot_save", "fipi_snapshot_load", "fipi_tune", "fipi_get_tuning",
+    "fipi_set_tuning",
     "fipi_ensemble_create", "fipi_ensemble_destroy", "fipi_ensemble_last_error", "fipi_ensemble_set_box",
     "fipi_ensemble_tune", "fipi_ensemble_step", "fipi_ensemble_step_async", "fipi_ensemble_sync",
     "fipi_ensemble_stream", "fipi_ensemble_get_phi", "fipi_ensemble_get_particles",
-    "fipi_ensemble_diagnostics", "fipi_ensemble_tuning",
+    "fipi_ensemble_diagnostics", "fipi_ensemble_get_tuning",
+    "fipi_ensemble_set_tuning",
     "fipi_adjoint_create", "fipi_adjoint_destroy", "fipi_adjoint_last_error", "fipi_adjoint_set_target",
     "fipi_adjoint_forward", "fipi_adjoint_gradient",
     "fipi_init_phi", "fipi_init_rsa", "fipi_init_uniform",
@@ -94,6 +96,20 @@ def _to_cparams(p: Params) -> CParams:
 _lib = None
 
 
+class CTuning(C.Structure):
+    """fipi_tuning_t: tile widths in bytes and the y-pass FFT plan (values
+    per thread; 0 = default when set)."""
+    _fields_ = [("y_tile_bytes", C.c_int32), ("hub_tile_bytes", C.c_int32),
+                ("stokes_tile_bytes", C.c_int32), ("y_values_per_thread", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+    @classmethod
+    def from_dict(cls, d: dict) -> "CTuning":
+        return cls(*(int(d.get(k, 0)) for k, _ in cls._fields_))
+
+
 def lib():
     """Load libfipi.so (built in-tree by fipi_amd.build)."""
     global _lib
@@ -138,7 +154,8 @@ def lib():
     L.fipi_snapshot_save.argtypes = [vp, vp, i64]
     L.fipi_snapshot_load.argtypes = [vp, vp, i64]
     L.fipi_tune.argtypes = [vp, C.c_int32]
-    L.fipi_tuning.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.fipi_get_tuning.argtypes = [vp, C.POINTER(CTuning)]
+    L.fipi_set_tuning.argtypes = [vp, C.POINTER(CTuning)]
     L.fipi_ensemble_create.argtypes = [C.POINTER(CParams), C.c_int32, C.POINTER(CBoxDesc), C.POINTER(vp)]
     L.fipi_ensemble_destroy.argtypes = [vp]
     L.fipi_ensemble_last_error.argtypes = [vp]
@@ -153,7 +170,8 @@ def lib():
     L.fipi_ensemble_get_phi.argtypes = [vp, C.c_int32, dp]
     L.fipi_ensemble_get_particles.argtypes = [vp, C.c_int32, dp]
     L.fipi_ensemble_diagnostics.argtypes = [vp, C.c_int32, C.POINTER(CDiag)]
-    L.fipi_ensemble_tuning.argtypes = [vp, C.c_int32, ip, ip, ip]
+    L.fipi_ensemble_get_tuning.argtypes = [vp, C.c_int32, C.POINTER(CTuning)]
+    L.fipi_ensemble_set_tuning.argtypes = [vp, C.c_int32, C.POINTER(CTuning)]
     L.fipi_adjoint_create.argtypes = [C.POINTER(CParams), dp, dp, C.c_double, C.c_int32, C.c_int32, C.POINTER(vp)]
     L.fipi_adjoint_destroy.argtypes = [vp]
     L.fipi_adjoint_last_error.argtypes = [vp]
@@ -316,15 +334,18 @@ class Simulation:
         self._c(lib().fipi_snapshot_load(self._h, buf, len(snap)))
 
     def tune(self, steps: int = 3) -> dict:
-        """Pick the fastest tile widths over `steps` steps each; the state is
-        left as it was.  Returns the widths in bytes."""
+        """Pick the fastest tile widths and y-pass FFT plan over `steps`
+        steps each; the state is left as it was.  Returns the choice."""
         self._c(lib().fipi_tune(self._h, steps))
         return self.tuning()
 
     def tuning(self) -> dict:
-        y, h, s = C.c_int32(), C.c_int32(), C.c_int32()
-        self._c(lib().fipi_tuning(self._h, C.byref(y), C.byref(h), C.byref(s)))
-        return {"y_tile_bytes": y.value, "hub_tile_bytes": h.value, "stokes_tile_bytes": s.value}
+        t = CTuning()
+        self._c(lib().fipi_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, tuning: dict):
+        self._c(lib().fipi_set_tuning(self._h, C.byref(CTuning.from_dict(tuning))))
 
     # ---- unit entry points on device memory (torch tensors) --------------
     # torch only provides device memory here; the work runs in libfipi.
@@ -465,10 +486,14 @@ class Ensemble:
     def tune(self, steps=3):
         self._c(lib().fipi_ensemble_tune(self._h, steps))
 
-    def tuning(self, n):
-        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
-        self._c(lib().fipi_ensemble_tuning(self._h, n, C.byref(a), C.byref(b), C.byref(c)))
-        return a.value, b.value, c.value
+    def tuning(self, n) -> dict:
+        """Choices of the slab of edge n (all zero when there is none)."""
+        t = CTuning()
+        self._c(lib().fipi_ensemble_get_tuning(self._h, n, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, n, tuning: dict):
+        self._c(lib().fipi_ensemble_set_tuning(self._h, n, C.byref(CTuning.from_dict(tuning))))
 
     def step(self, nsteps=1):
         self._c(lib().fipi_ensemble_step(self._h, nsteps))

@@ -265,9 +265,14 @@ int fipi_ensemble_tune(fipi_ensemble *e, int32_t steps)
         fipi_ctx *ctx = s.ctx;
         fipi_ctx::Tune best = ctx->tune;
         float best_ms = 1e30f;
+        int ydef, yalt;
+        y_plans_of(ctx, &ydef, &yalt);
+        for (int ye : {ydef, yalt})
         for (int y : {64, 128})
             for (int h : {64, 128})
                 for (int st : {64, 128}) {
+                    if (ye == 0) continue;
+                    ctx->tune.y_e = ye == ydef ? 0 : ye;
                     ctx->tune.y_tb = y;
                     ctx->tune.hub_tb = h;
                     ctx->tune.stokes_tb = st;
@@ -295,17 +300,27 @@ int fipi_ensemble_tune(fipi_ensemble *e, int32_t steps)
     return ens_commit(e);  // back to the initial state
 }
 
-int fipi_ensemble_tuning(const fipi_ensemble *e, int32_t n, int32_t *y_tb, int32_t *hub_tb, int32_t *stokes_tb)
+int fipi_ensemble_get_tuning(const fipi_ensemble *e, int32_t n, fipi_tuning_t *out)
 {
-    if (!e || !y_tb || !hub_tb || !stokes_tb) return FIPI_ERR_ARG;
-    *y_tb = *hub_tb = *stokes_tb = 0;
+    if (!e || !out) return ens_err(nullptr, FIPI_ERR_ARG, "ens and out are required");
+    *out = fipi_tuning_t{0, 0, 0, 0};
     for (const auto &s : e->slabs)
-        if (s.n == n) {
-            *y_tb = s.ctx->tune.y_tb;
-            *hub_tb = s.ctx->tune.hub_tb;
-            *stokes_tb = s.ctx->tune.stokes_tb;
-        }
+        if (s.n == n) tuning_out(s.ctx, out);
     return FIPI_OK;
+}
+
+int fipi_ensemble_set_tuning(fipi_ensemble *e, int32_t n, const fipi_tuning_t *in)
+{
+    if (!e || !in) return ens_err(e, FIPI_ERR_ARG, "ens and in are required");
+    for (auto &s : e->slabs)
+        if (s.n == n) {
+            ENSCHK(e, hipStreamSynchronize(e->stream));
+            const int rc = tuning_in(s.ctx, in);
+            if (rc) return ens_from_ctx(e, s.ctx, rc);
+            ens_drop_graphs(e);  // the captured launches used the old choice
+            return FIPI_OK;
+        }
+    return ens_err(e, FIPI_ERR_ARG, "no box of that edge length");
 }
 
 int fipi_ensemble_step_async(fipi_ensemble *e, int32_t nsteps)

@@ -189,8 +189,8 @@ template <int R, int SIGN, typename C> __device__ __forceinline__ void dft(C *v)
 // put their S = 128-byte-wide columns side by side, so a wave's accesses
 // are runs of S consecutive values).
 //
-// Plans (E.M.E): 256 = 8.4.8 or 16.16, 128 = 8.2.8, 96 = 4.6.4,
-// 80 = 4.5.4, 64 = 8.8, 48 = 4.3.4, 32 = 4.2.4, 16 = 4.4.  When M does not
+// Plans (E.M.E): 256 = 8.4.8 or 16.16, 128 = 8.2.8 or 4.8.4, 96 = 4.6.4,
+// 80 = 4.5.4, 64 = 8.8 or 4.4.4, 48 = 4.3.4, 32 = 4.2.4, 16 = 4.4.  When M does not
 // divide E the N/M butterflies of the middle stage are spread over the P
 // threads unevenly (some threads take one more).  Lines of 256 use E = 8
 // by default (fewer registers for kernels that keep three fields in
@@ -202,9 +202,11 @@ template <> struct PlanE<16, 4> { static constexpr int M = 0; };
 template <> struct PlanE<32, 4> { static constexpr int M = 2; };
 template <> struct PlanE<48, 4> { static constexpr int M = 3; };
 template <> struct PlanE<64, 8> { static constexpr int M = 0; };
+template <> struct PlanE<64, 4> { static constexpr int M = 4; };
 template <> struct PlanE<80, 4> { static constexpr int M = 5; };
 template <> struct PlanE<96, 4> { static constexpr int M = 6; };
 template <> struct PlanE<128, 8> { static constexpr int M = 2; };
+template <> struct PlanE<128, 4> { static constexpr int M = 8; };
 template <> struct PlanE<256, 8> { static constexpr int M = 4; };
 template <> struct PlanE<256, 16> { static constexpr int M = 0; };
 

@@ -551,11 +551,14 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
     (void)E; (void)P;
 #define FIPI_TILE_SETUP(T, N, TLV) FIPI_TILE_SETUP_E(T, N, TLV, (PlanE_for<T, N>::E))
 
-// values per thread of the y passes: 256-point FP32 lines use 16 (one LDS
-// exchange, twice the bytes in flight per thread; these passes hold at most
-// three fields), everything else the default of its size
+// Values per thread of the y passes (their FFT plan).  Default: 256-point
+// FP32 lines use 16 (one LDS exchange, twice the bytes in flight per
+// thread; these passes hold at most three fields), everything else the
+// default of its size.  Alt: the other plan of the size (0: none); a tuner
+// picks between the two.
 template <typename T, int N> struct YE {
     static constexpr int v = (sizeof(T) == 4 && N == 256) ? 16 : PlanE_for<T, N>::E;
+    static constexpr int alt = N == 256 ? (v == 16 ? 8 : 16) : ((N == 64 || N == 128) ? 4 : 0);
 };
 
 // Element offsets within one box (the box offset is applied to the
@@ -601,13 +604,12 @@ __device__ __forceinline__ void tile_store(const cplx<T> *v, cplx<T> *__restrict
 
 // y pass: transforms along y of NF fields in place.  GRAD (inverse only):
 // field 0 also yields i*ky*field0 into `extra` (zero on ky = NY/2).
-template <typename T, int N, int SIGN, int NF, bool GRAD, int TB>
-__global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N, YE<T, N>::v>::P)) void k_y(cplx<T> *__restrict__ a0, cplx<T> *__restrict__ a1,
+template <typename T, int N, int SIGN, int NF, bool GRAD, int TB, int EY>
+__global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N, EY>::P)) void k_y(cplx<T> *__restrict__ a0, cplx<T> *__restrict__ a1,
                                                               cplx<T> *__restrict__ a2, cplx<T> *__restrict__ extra,
                                                               Dims d, const double *__restrict__ kyv,
                                                               const cplx<T> *__restrict__ tw)
 {
-    constexpr int EY = YE<T, N>::v;
     FIPI_TILE_SETUP_E(T, N, (TileW<T, TB>::L), EY)
     a0 = boxp(a0, zb, d.hs);
     a1 = boxp(a1, zb, d.hs);

@@ -79,8 +79,10 @@ struct fipi_ctx {
     // tile widths (bytes) of the y passes, the CH/mu/init hubs and the
     // Stokes hub; results do not depend on them (every line sees the same
     // arithmetic), only the speed does
+    // and the values per thread (FFT plan) of the y passes, 0 = the default
+    // of the size; that choice changes rounding, not the transform
     struct Tune {
-        int y_tb = 128, hub_tb = 128, stokes_tb = 128;
+        int y_tb = 128, hub_tb = 128, stokes_tb = 128, y_e = 0;
     } tune;
     Dims d{};
     size_t G = 0, GH = 0;
@@ -329,11 +331,32 @@ struct Impl {
             constexpr int N = decltype(NC)::value;
             with_tb(ctx->tune.y_tb, [&](auto TBC) {
                 constexpr int TB = decltype(TBC)::value, TL = TileW<T, TB>::L;
-                hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD, TB>),
-                                   dim3((unsigned)((d.nxh + TL - 1) / TL), d.nz, ctx->nbox),
-                                   dim3((TL * Line<T, N, YE<T, N>::v>::P)), 0, ctx->stream, a0, a1, a2, extra, d, ctx->ky,
-                                   (const C *)ctx->twy);
+                auto launch = [&](auto EC) {
+                    constexpr int EY = decltype(EC)::value;
+                    hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD, TB, EY>),
+                                       dim3((unsigned)((d.nxh + TL - 1) / TL), d.nz, ctx->nbox),
+                                       dim3((TL * Line<T, N, EY>::P)), 0, ctx->stream, a0, a1, a2, extra, d,
+                                       ctx->ky, (const C *)ctx->twy);
+                };
+                if constexpr (YE<T, N>::alt != 0) {
+                    if (ctx->tune.y_e == YE<T, N>::alt) {
+                        launch(std::integral_constant<int, YE<T, N>::alt>{});
+                        return;
+                    }
+                }
+                launch(std::integral_constant<int, YE<T, N>::v>{});
             });
+        });
+    }
+
+    // the y-pass plans of this context's ny: {default, alternative or 0}
+    static void y_plans(const fipi_ctx *ctx, int *def, int *alt)
+    {
+        *def = *alt = 0;
+        with_n(ctx->d.ny, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            *def = YE<T, N>::v;
+            *alt = YE<T, N>::alt;
         });
     }
 
@@ -1535,6 +1558,44 @@ int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32
 
 }  // extern "C"
 
+namespace {
+
+// tuning choices of a context (fipi_get_tuning / fipi_set_tuning)
+void y_plans_of(const fipi_ctx *ctx, int *def, int *alt)
+{
+    if (ctx->fp64) Impl<double>::y_plans(ctx, def, alt);
+    else Impl<float>::y_plans(ctx, def, alt);
+}
+
+void tuning_out(const fipi_ctx *ctx, fipi_tuning_t *out)
+{
+    int def, alt;
+    y_plans_of(ctx, &def, &alt);
+    out->y_tile_bytes = ctx->tune.y_tb;
+    out->hub_tile_bytes = ctx->tune.hub_tb;
+    out->stokes_tile_bytes = ctx->tune.stokes_tb;
+    out->y_values_per_thread = ctx->tune.y_e != 0 ? ctx->tune.y_e : def;
+}
+
+int tuning_in(fipi_ctx *ctx, const fipi_tuning_t *in)
+{
+    auto tb_ok = [](int v) { return v == 64 || v == 128; };
+    if (!tb_ok(in->y_tile_bytes) || !tb_ok(in->hub_tile_bytes) || !tb_ok(in->stokes_tile_bytes))
+        return set_err(ctx, FIPI_ERR_ARG, "tile widths are 64 or 128 bytes");
+    int def, alt;
+    y_plans_of(ctx, &def, &alt);
+    const int e = in->y_values_per_thread;
+    if (e != 0 && e != def && (alt == 0 || e != alt))
+        return set_err(ctx, FIPI_ERR_ARG, "no y-pass plan with that many values per thread for this grid");
+    ctx->tune.y_tb = in->y_tile_bytes;
+    ctx->tune.hub_tb = in->hub_tile_bytes;
+    ctx->tune.stokes_tb = in->stokes_tile_bytes;
+    ctx->tune.y_e = (e == 0 || e == def) ? 0 : e;
+    return FIPI_OK;
+}
+
+}  // namespace
+
 #include "ensemble.hpp"
 #include "adjoint.hpp"
 
@@ -1690,9 +1751,14 @@ int fipi_tune(fipi_ctx *ctx, int32_t steps)
     const bool conv0 = ctx->conv_ready;
     fipi_ctx::Tune best = ctx->tune;
     float best_ms = 1e30f;
+    int ydef, yalt;
+    y_plans_of(ctx, &ydef, &yalt);
+    for (int ye : {ydef, yalt})
     for (int y : {64, 128})
         for (int h : {64, 128})
             for (int st : {64, 128}) {
+                if (ye == 0) continue;
+                ctx->tune.y_e = ye == ydef ? 0 : ye;
                 ctx->tune.y_tb = y;
                 ctx->tune.hub_tb = h;
                 ctx->tune.stokes_tb = st;
@@ -1726,13 +1792,17 @@ int fipi_tune(fipi_ctx *ctx, int32_t steps)
     return rc;
 }
 
-int fipi_tuning(const fipi_ctx *ctx, int32_t *y_tb, int32_t *hub_tb, int32_t *stokes_tb)
+int fipi_get_tuning(const fipi_ctx *ctx, fipi_tuning_t *out)
 {
-    if (!ctx || !y_tb || !hub_tb || !stokes_tb) return FIPI_ERR_ARG;
-    *y_tb = ctx->tune.y_tb;
-    *hub_tb = ctx->tune.hub_tb;
-    *stokes_tb = ctx->tune.stokes_tb;
+    if (!ctx || !out) return set_err(nullptr, FIPI_ERR_ARG, "ctx and out are required");
+    tuning_out(ctx, out);
     return FIPI_OK;
+}
+
+int fipi_set_tuning(fipi_ctx *ctx, const fipi_tuning_t *in)
+{
+    if (!ctx || !in) return set_err(ctx, FIPI_ERR_ARG, "ctx and in are required");
+    return tuning_in(ctx, in);
 }
 
 }  // extern "C"

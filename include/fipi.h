@@ -180,15 +180,24 @@ int64_t fipi_snapshot_size(fipi_ctx *ctx);
 int fipi_snapshot_save(fipi_ctx *ctx, void *buf, int64_t size);
 int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size);
 
-/* ---- tile-width tuning ----------------------------------------------- */
+/* ---- tuning: tile widths and FFT plans --------------------------------- */
 /* No counterpart in the reference (FFTW plans, field.cpp:24-139, are made
- * once with fixed options).  Times `steps` steps with each tile width (64 or
- * 128 bytes) of the y passes, the CH hub and the Stokes hub, keeps the
- * fastest, and restores the state from before the trials (kept in device
- * memory meanwhile; about the size of a snapshot).  Results do not depend
- * on the tile widths, only the speed does.  fipi_tuning reads them. */
+ * once with FFTW_ESTIMATE).  The launch choices of a context: tile widths
+ * (64 or 128 bytes) of the y passes, the CH hub and the Stokes hub, and the
+ * FFT plan of the y passes as values per thread (first/last radix: 256
+ * points 8.4.8 or 16.16, 128 points 8.2.8 or 4.8.4, 64 points 8.8 or 4.4.4;
+ * other sizes have one plan; 0 in fipi_set_tuning = the default).  Tile
+ * widths do not change results; the plan changes rounding only.
+ * fipi_tune times `steps` steps with every combination, keeps the fastest
+ * and restores the state from before the trials (kept in device memory
+ * meanwhile; about the size of a snapshot). */
+typedef struct fipi_tuning_t {
+    int32_t y_tile_bytes, hub_tile_bytes, stokes_tile_bytes;
+    int32_t y_values_per_thread;
+} fipi_tuning_t;
 int fipi_tune(fipi_ctx *ctx, int32_t steps);
-int fipi_tuning(const fipi_ctx *ctx, int32_t *y_tb, int32_t *hub_tb, int32_t *stokes_tb);
+int fipi_get_tuning(const fipi_ctx *ctx, fipi_tuning_t *out);
+int fipi_set_tuning(fipi_ctx *ctx, const fipi_tuning_t *in);
 
 /* ---- batched ensemble of independent boxes ---------------------------- */
 /* No counterpart in the reference, which runs one box per process
@@ -214,9 +223,10 @@ const char *fipi_ensemble_last_error(const fipi_ensemble *ens);
  * first step; setting a box again restarts the whole ensemble at t = 0. */
 int fipi_ensemble_set_box(fipi_ensemble *ens, int32_t box, const double *phi, const double *pos,
                           const double *radius, const double *cos_theta);
-/* Pick the y-pass, hub and Stokes-hub tile widths of each edge length by
- * timing `steps` steps per candidate, then restart from the initial state.
- * Results do not depend on the choice. */
+/* Pick the tile widths and the y-pass FFT plan (fipi_tuning_t) of each edge
+ * length by timing `steps` steps per candidate, then restart from the
+ * initial state.  Tile widths do not change results; the plan changes
+ * rounding only. */
 int fipi_ensemble_tune(fipi_ensemble *ens, int32_t steps);
 /* Advance every box nsteps steps (one graph replay per step); _async
  * leaves the error check to fipi_ensemble_sync. */
@@ -228,7 +238,10 @@ int fipi_ensemble_get_phi(fipi_ensemble *ens, int32_t box, double *phi);
 int fipi_ensemble_get_particles(fipi_ensemble *ens, int32_t box, double *pos);
 int fipi_ensemble_diagnostics(fipi_ensemble *ens, int32_t box, fipi_diag *out);
 /* Tile widths chosen for edge n (bytes; 0 when no box has that edge). */
-int fipi_ensemble_tuning(const fipi_ensemble *ens, int32_t n, int32_t *y_tb, int32_t *hub_tb, int32_t *stokes_tb);
+/* the choices of the slab of edge n (all zero when there is none); setting
+ * them drops the captured step graphs (recaptured at the next step) */
+int fipi_ensemble_get_tuning(const fipi_ensemble *ens, int32_t n, fipi_tuning_t *out);
+int fipi_ensemble_set_tuning(fipi_ensemble *ens, int32_t n, const fipi_tuning_t *in);
 
 /* ---- adjoint of the step for calibration ------------------------------- */
 /* No counterpart in the reference.  Fits mobility, lambda and per-particle

@@ -75,23 +75,46 @@ def test_boxes_match_single_box_path(gpu, prec):
     ens.close()
 
 
-def test_tuning_does_not_change_results(gpu):
+def test_tuning(gpu):
+    """Tile widths leave the results bit for bit; the tuner's y-pass FFT
+    plan may change rounding, and its whole choice reproduces bit for bit
+    when set on a fresh ensemble."""
     print("ensemble tuning", flush=True)
     common, boxes = _case(FP32_MIXED, seed=2, edges=[32, 64, 48], counts=[30, 100, 0])
     a = _ensemble(common, boxes)
     a.step(4)
     pa = [a.phi(b) for b in range(3)]
     a.close()
+    w = _ensemble(common, boxes)
+    for n in (32, 48, 64):
+        w.set_tuning(n, {"y_tile_bytes": 64, "hub_tile_bytes": 64, "stokes_tile_bytes": 128,
+                         "y_values_per_thread": 0})
+    w.step(4)
+    for b in range(3):
+        assert np.array_equal(w.phi(b), pa[b]), b
+    w.close()
     e = _ensemble(common, boxes)
     e.tune(2)
+    chosen = {}
     for n in (32, 48, 64):
-        y, h, s = e.tuning(n)
-        assert y in (64, 128) and h in (64, 128) and s in (64, 128)
-    assert e.tuning(96) == (0, 0, 0)
+        t = e.tuning(n)
+        assert t["y_tile_bytes"] in (64, 128) and t["hub_tile_bytes"] in (64, 128)
+        assert t["stokes_tile_bytes"] in (64, 128)
+        assert t["y_values_per_thread"] in ((8, 4) if n == 64 else (4,))
+        chosen[n] = t
+    assert e.tuning(96) == {"y_tile_bytes": 0, "hub_tile_bytes": 0, "stokes_tile_bytes": 0,
+                            "y_values_per_thread": 0}
     e.step(4)
-    for b in range(3):
-        assert np.array_equal(e.phi(b), pa[b]), b
+    pe = [e.phi(b) for b in range(3)]
     e.close()
+    r = _ensemble(common, boxes)
+    for n, t in chosen.items():
+        r.set_tuning(n, t)
+    r.step(4)
+    for b in range(3):
+        assert np.array_equal(r.phi(b), pe[b]), b
+        assert np.linalg.norm(pe[b] - pa[b]) <= 1e-5 * np.linalg.norm(pa[b]), b
+    r.close()
 
 
 def test_mass_per_box(gpu):

@@ -53,3 +53,41 @@ def test_single_modes_fp64(gpu):
         ref = np.fft.rfftn(f)
         assert np.abs(Xc - ref).max() < 1e-11 * n ** 3
     sim.close()
+
+
+@pytest.mark.parametrize("ny", [64, 128, 256])
+@pytest.mark.parametrize("prec", [FP64, FP32_MIXED])
+def test_y_plans_vs_numpy(gpu, ny, prec):
+    """Both FFT plans of the y passes that a tuner may pick (256: 8.4.8 and
+    16.16; 128: 8.2.8 and 4.8.4; 64: 8.8 and 4.4.4) against numpy, on a
+    grid that is long in y."""
+    import torch
+    print(f"y plans ny={ny} prec={prec}", flush=True)
+    nx, nz = 32, 16
+    p = config_params(nx, 0, 2.0, 10, precision=prec).replace(ny=ny, nz=nz, ly=float(ny), lz=float(nz))
+    sim = fipi_amd.Simulation(p)
+    default = sim.tuning()["y_values_per_thread"]
+    plans = {8, 16} if ny == 256 else {8, 4}
+    assert default in plans
+    rng = np.random.default_rng(ny + prec)
+    x = rng.standard_normal((nz, ny, nx))
+    dt = torch.float64 if prec == FP64 else torch.float32
+    xr = x.astype(np.float32).astype(np.float64) if prec else x
+    ref = np.fft.rfftn(xr)
+    tol = 1e-14 if prec == FP64 else 2e-6
+    for e in sorted(plans):
+        t = sim.tuning()
+        t["y_values_per_thread"] = e
+        sim.set_tuning(t)
+        assert sim.tuning()["y_values_per_thread"] == e
+        X = sim.fft_r2c(torch.tensor(x, dtype=dt, device=gpu))
+        Xc = X.cpu().numpy().astype(np.float64)
+        err = np.linalg.norm(Xc[..., 0] + 1j * Xc[..., 1] - ref) / np.linalg.norm(ref)
+        assert err < tol, (e, err)
+        y = sim.fft_c2r(X).cpu().numpy().astype(np.float64)
+        assert np.linalg.norm(y - xr) / np.linalg.norm(xr) < tol, e
+    with pytest.raises(fipi_amd.FipiError):
+        t = sim.tuning()
+        t["y_values_per_thread"] = 5
+        sim.set_tuning(t)
+    sim.close()

@@ -50,21 +50,22 @@ def test_snapshot_rejects_other_grid(gpu):
 
 @pytest.mark.parametrize("prec", [FP64, FP32_MIXED])
 def test_tune_keeps_state(gpu, prec):
-    """fipi_tune times every tile-width choice and restores the state: the
-    run continues as if it had not been called (results do not depend on
-    the tile widths)."""
+    """fipi_tune times every choice and restores the state: the run
+    continues exactly as a run that was given the same choice."""
     print(f"tune prec={prec}", flush=True)
     p, phi0, pos = make_case(32, 120, steps=40, seed=10, precision=prec)
     a = fipi_amd.Simulation(p)
     a.set_state(phi0, pos)
     a.step(3)
-    widths = a.tune(2)
-    assert set(widths.values()) <= {64, 128}
-    assert a.tuning() == widths
+    chosen = a.tune(2)
+    assert {chosen[k] for k in ("y_tile_bytes", "hub_tile_bytes", "stokes_tile_bytes")} <= {64, 128}
+    assert chosen["y_values_per_thread"] == 4  # 32-point lines have one plan, 4.2.4
+    assert a.tuning() == chosen
     assert a.t == 3
     a.step(6)
     b = fipi_amd.Simulation(p)
     b.set_state(phi0, pos)
+    b.set_tuning(chosen)
     b.step(3)
     b.step(6)
     assert np.array_equal(a.phi(), b.phi())
