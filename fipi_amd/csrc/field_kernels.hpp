@@ -270,16 +270,14 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inverse(const cplx<T> *__rest
     }
 }
 
-// P13 + next step's P1 (FP32-mixed path; the FP64 path keeps the two
-// passes, whose register footprint allows twice the occupancy): inverse x
-// of u_hat (X) -> u (real), then
+// P13 + next step's P1: inverse x of u_hat (X) -> u (real), then
 // conv = u . grad c (src/main.cpp:91, field.cpp:392-400, same sum order as
 // ProdConv) and its forward x transform into h0 (X), in place of u_hat_x.
 // g holds grad c^{n+1}, the gradient the next step's convection uses.
 // part (optional): FP64 block sums of conv, as k_x_forward.
 template <typename T, int N>
-__global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *__restrict__ h1,
-                                                   const cplx<T> *__restrict__ h2, T *__restrict__ u0,
+__global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *h1, const cplx<T> *h2,
+                                                   T *__restrict__ u0,
                                                    T *__restrict__ u1, T *__restrict__ u2,
                                                    const T *__restrict__ g0, const T *__restrict__ g1,
                                                    const T *__restrict__ g2, Dims d, const cplx<T> *__restrict__ tw,
@@ -288,10 +286,13 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const c
     FIPI_XPASS_SETUP(T, N)
     const size_t rA = zb * d.hs + (size_t)rl * d.nxp, rB = rA + d.nxp;
     const size_t xA = zb * d.rs + (size_t)rl * d.nx, xB = xA + d.nx;
-    C v[3][E];
-    load_c2r<T, N, false>(v[0], h0, rA, rB, t, nullptr);
-    load_c2r<T, N, false>(v[1], h1, rA, rB, t, nullptr);
-    load_c2r<T, N, false>(v[2], h2, rA, rB, t, nullptr);
+    // FP32: all three components in flight from the start; FP64: the next
+    // component in flight while one is transformed (three would not fit in
+    // the registers of two waves per SIMD)
+    constexpr int NV = sizeof(T) == 4 ? 3 : 2;
+    C v[NV][E];
+#pragma unroll
+    for (int f = 0; f < NV; f++) load_c2r<T, N, false>(v[f], f == 0 ? (const C *)h0 : (f == 1 ? h1 : h2), rA, rB, t, nullptr);
     C cv[E];
 #pragma unroll
     for (int q = 0; q < E; q++) cv[q] = C{T(0), T(0)};
@@ -299,6 +300,8 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const c
     for (int f = 0; f < 3; f++) {
         T *u = f == 0 ? u0 : (f == 1 ? u1 : u2);
         const T *g = f == 0 ? g0 : (f == 1 ? g1 : g2);
+        C *w = v[NV == 3 ? f : (f & 1)];
+        if (NV == 2 && f == 1) load_c2r<T, N, false>(v[0], h2, rA, rB, t, nullptr);  // component 2 in flight
         T ga[E], gb[E];
 #pragma unroll
         for (int q = 0; q < E; q++) {
@@ -306,13 +309,13 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const c
             ga[q] = g[xA + x];
             gb[q] = g[xB + x];
         }
-        fft_dist<T, N, +1>(v[f], L, t, tw);
+        fft_dist<T, N, +1>(w, L, t, tw);
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int x = t + q * P;
-            u[xA + x] = v[f][q].x;
-            u[xB + x] = v[f][q].y;
-            cv[q] = C{cv[q].x + v[f][q].x * ga[q], cv[q].y + v[f][q].y * gb[q]};
+            u[xA + x] = w[q].x;
+            u[xB + x] = w[q].y;
+            cv[q] = C{cv[q].x + w[q].x * ga[q], cv[q].y + w[q].y * gb[q]};
         }
     }
     if (part != nullptr) {

@@ -662,10 +662,7 @@ struct Impl {
                 ProfScope ps_(ctx, "y_inv_3", 6 * Hb);
                 y_pass<+1, 3, false>(ctx, h0, h1, h2, nullptr);
             }
-            if constexpr (std::is_same<T, double>::value) {
-                ProfScope ps_(ctx, "x_inv_u", 3 * Hb + 3 * Rb);
-                x_inverse<3>(ctx, h0, h1, h2, R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]));
-            } else {
+            {
                 // u^{n+1} and, for the next step, conv = u^{n+1}.grad c^{n+1} (X) in h[0]
                 ProfScope ps_(ctx, "x_inv_u_conv", 4 * Hb + 6 * Rb);
                 with_n(d.nx, [&](auto NC) {
