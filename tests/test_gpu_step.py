@@ -53,6 +53,32 @@ def test_step_with_particles_32(gpu):
         assert rel_l2(gp["f_pi"], rp["f_pi"]) < 1e-8, (t, rel_l2(gp["f_pi"], rp["f_pi"]))
 
 
+@pytest.mark.parametrize("prec", [fipi_amd.FP64, fipi_amd.FP32_MIXED])
+def test_particles_without_stokes_32(gpu, prec):
+    """switch_ns = 0 with particles: no force pass, so within one call the
+    particle stage of each step reads c and grad c from the set the next
+    step does not write (the spare set).  FP64 against the oracle; in both
+    precisions one call against single steps, bitwise."""
+    print(f"particles ns=0 prec={prec}", flush=True)
+    p, phi0, pos = make_case(32, 150, steps=40, seed=6, ns=0, precision=prec)
+    if prec == fipi_amd.FP64:
+        for t, g_phi, r_phi, _, _, gp, rp in _run_both(p, phi0, pos, 12, chunks=2):
+            assert rel_l2(g_phi, r_phi) < TOL, (t, rel_l2(g_phi, r_phi))
+            assert rel_l2(gp["p"], rp["p"]) < TOL, (t, rel_l2(gp["p"], rp["p"]))
+            assert rel_l2(gp["f_pi"], rp["f_pi"]) < 1e-8, (t, rel_l2(gp["f_pi"], rp["f_pi"]))
+    a = fipi_amd.Simulation(p)
+    a.set_state(phi0, pos)
+    a.step(12)
+    b = fipi_amd.Simulation(p)
+    b.set_state(phi0, pos)
+    for _ in range(12):
+        b.step(1)
+    assert np.array_equal(a.phi(), b.phi())
+    assert np.array_equal(a.particles()["p"], b.particles()["p"])
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("n,npart", [(48, 120), (80, 300), (96, 400)])
 def test_step_odd_radix(gpu, n, npart):
     """Edges with radix-3/5/6 transforms (the ensemble's 48, 80, 96)."""
