@@ -11,11 +11,13 @@
 // Pass structure (each pass reads and writes every array once):
 //   x pass   two real lines packed as one complex line (a + i b), one
 //            complex transform, split into the two half spectra (R2C), or
-//            the reverse (C2R).  256 threads per workgroup.
-//   y pass   a 128-byte tile of neighbouring kx columns x all y at one z.
-//   z hub    a 128-byte tile of kx columns x all z at one y: forward z
-//            transform, the fused k-space update, inverse z transform, all
-//            in registers.
+//            the reverse (C2R).  128 (FP64) or 256 (FP32) threads per
+//            workgroup.
+//   y pass   a 64- or 128-byte tile of neighbouring kx columns x all y at
+//            one z.
+//   z hub    a 64- or 128-byte tile of kx columns x all z at one y: forward
+//            z transform, the fused k-space update, inverse z transform,
+//            all in registers.
 // Every line is held in the distribution D of fft_core.hpp (thread t of
 // the line holds elements t + q*P), before and after each transform.
 // Wave numbers are read from per-axis tables computed on the host with the

@@ -1,20 +1,22 @@
 // libfipi: host side of the MI355X FIPI time step behind the C ABI of
-// include/fipi.h.  One context = one periodic box on the current HIP device
-// with one stream; all state lives in device memory.
+// include/fipi.h.  One context = one periodic box (or a slab of equal
+// boxes) on the current HIP device; all state lives in device memory.
 //
 // Per step (reference src/main.cpp:75-222), with R = real grid, H = half
 // spectrum scratch, K = persistent spectra:
-//   x fwd   conv = u.grad c          R(6)  -> H0              (main.cpp:91-92)
-//   y fwd                            H0    -> H0
-//   z hub   CH SBDF2 update          H0,K  -> K(c,hist), H0,H1 (main.cpp:95-140)
-//   y inv   c, dy c, dz c            H0,H1 -> H0,H2,H1
-//   x inv   c, grad c; f(c) fwd      H     -> R(4), H0        (main.cpp:153,160-162)
-//   y fwd                            H0    -> H0
-//   z hub   f_hat, mu_hat            H0,K  -> K(f), H0         (main.cpp:154-158)
-//   y inv                            H0    -> H0
-//   x inv   mu; F = mu grad c + spread + buoyancy; fwd        (main.cpp:164-175)
-//   y fwd, z hub Stokes, y inv, x inv -> u                     (main.cpp:177-179)
-//   two Heun particle stages                                   (main.cpp:198-217)
+//   (x fwd  conv = u.grad c          R(6)  -> H0, first step after set_state)
+//   y fwd   conv                     H0    -> H0
+//   z hub   CH SBDF2 update          H0,K  -> K(c,hist), H0,H1,H2  (main.cpp:95-140)
+//           (c/G, dz c/G, lambda k^2 c/G)
+//   y inv   c, dz c -> + dy c; lambda k^2 c                         (main.cpp:160-162)
+//   x pass  c, grad c (R); f(c) fwd -> fk; mu = f + lambda(-lap c);
+//           F = mu grad c + spread + buoyancy fwd -> H0..H2   (main.cpp:153-175)
+//   y fwd   f, F
+//   z hub   Stokes projection of F; f -> f_hat (K) for the next step (main.cpp:177-178)
+//   y inv   u
+//   x pass  u (R); conv = u.grad c fwd -> H0 for the next step  (main.cpp:179, 91-92)
+//   two Heun particle stages + the next step's spread, on a second stream
+//   beside the next step's first passes within one call  (main.cpp:198-217)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
