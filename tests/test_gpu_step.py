@@ -138,3 +138,32 @@ def test_determinism(gpu):
         sim.close()
     for a, b in zip(outs[0], outs[1]):
         assert np.array_equal(a, b)
+
+
+def test_headline_size_256(gpu):
+    """The headline workload itself: 256^3 with 100,000 spheres of radius
+    2 dx (gap 0.5 dx) and contact angles U(60, 120) deg, FP64, 3 steps
+    against the oracle (the first-order start and two SBDF2 steps; both
+    Heun stages and the spread each step)."""
+    import os
+    print("headline 256^3 + 100k, 3 steps", flush=True)
+    n, npart = 256, 100000
+    p = fipi_amd.config_params(n, npart, 2.0, 1000)
+    phi0 = fipi_amd.init_phi((n, n, n), 0.0, 0.05, 1)
+    pos = fipi_amd.init_rsa(npart, 2.0, 0.5, (p.lx, p.ly, p.lz), 2)
+    assert pos.shape[0] == npart
+    cth = np.cos(np.deg2rad(fipi_amd.init_uniform(npart, 60.0, 120.0, 3)))
+    sim = fipi_amd.Simulation(p)
+    sim.set_state(phi0, pos, None, cth)
+    sim.step(3)
+    g_phi, g_u, g_p = sim.phi(), sim.velocity(), sim.particles()
+    sim.close()
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    ref = oracle.Sim(p, phi0, pos, None, cth, n_threads=threads)
+    ref.step(3)
+    r_p = ref.particles()
+    assert rel_l2(g_phi, ref.phi()) < TOL, rel_l2(g_phi, ref.phi())
+    assert rel_l2(g_u, ref.u()) < TOL, rel_l2(g_u, ref.u())
+    assert rel_l2(g_p["p"], r_p["p"]) < TOL, rel_l2(g_p["p"], r_p["p"])
+    assert rel_l2(g_p["f_pi"], r_p["f_pi"]) < 1e-8, rel_l2(g_p["f_pi"], r_p["f_pi"])
+
