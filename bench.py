@@ -321,6 +321,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-config1", action="store_true")
+    ap.add_argument("--no-field-only", action="store_true")
     ap.add_argument("--tune-steps", type=int, default=3,
                     help="steps per tile-width trial before the run (0: defaults)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -385,6 +386,29 @@ def main():
 
     main_run = timed_run(prec)
     other_run = timed_run(other) if not args.single else None
+
+    def field_only_rate(precision, steps=300):
+        """Steps/s of the same grid and phi0 without spheres (the field
+        passes alone), after the same warm-up."""
+        q = p.replace(precision=precision, n_particles=0, switch_particle=0)
+        sim = fipi_amd.Simulation(q)
+        sim.set_state(phi0)
+        sim.step(args.warmup)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step_async(steps)
+        sim.sync()
+        dt = time.perf_counter() - t0
+        sim.close()
+        return steps / dt
+
+    field_only = None
+    if not args.single and not args.no_field_only and args.particles > 0:
+        f64 = field_only_rate(fipi_amd.FP64)
+        f32 = field_only_rate(fipi_amd.FP32_MIXED)
+        field_only = {"workload": f"{args.n}^3, same phi0, no spheres, 300 steps",
+                      "fp64_steps_per_s": round(f64, 3), "fp32_mixed_steps_per_s": round(f32, 3),
+                      "mixed_speedup": round(f32 / f64, 3)}
 
     wall = main_run["wall"]
     ms_per_step = wall * 1e3 / args.steps
@@ -481,6 +505,7 @@ def main():
         },
         "event_ms_per_step": round(main_run["ev_ms"] / args.steps, 4),
         "precision_comparison": precision_cmp,
+        "field_only": field_only,
         "grid_point_steps_per_s": value * p.G,
         "step_roofline": {
             "bytes_per_step": bstep,
