@@ -258,46 +258,52 @@ int fipi_ensemble_tune(fipi_ensemble *e, int32_t steps)
     if (!e || steps < 1) return ens_err(e, FIPI_ERR_ARG, "invalid argument");
     int rc = ens_commit(e);
     if (rc) return rc;
-    hipEvent_t a, b;
-    ENSCHK(e, hipEventCreate(&a));
-    ENSCHK(e, hipEventCreate(&b));
-    for (auto &s : e->slabs) {
-        fipi_ctx *ctx = s.ctx;
-        fipi_ctx::Tune best = ctx->tune;
-        float best_ms = 1e30f;
-        int ydef, yalt;
-        y_plans_of(ctx, &ydef, &yalt);
-        for (int ye : {ydef, yalt})
-        for (int y : {64, 128})
-            for (int h : {64, 128})
-                for (int st : {64, 128}) {
-                    if (ye == 0) continue;
-                    ctx->tune.y_e = ye == ydef ? 0 : ye;
-                    ctx->tune.y_tb = y;
-                    ctx->tune.hub_tb = h;
-                    ctx->tune.stokes_tb = st;
-                    ens_step_slab(ctx);  // warm-up (first use of these kernels)
-                    ENSCHK(e, hipEventRecord(a, ctx->stream));
-                    for (int k = 0; k < steps; k++) ens_step_slab(ctx);
-                    ENSCHK(e, hipEventRecord(b, ctx->stream));
-                    ENSCHK(e, hipEventSynchronize(b));
-                    float ms = 0.f;
-                    ENSCHK(e, hipEventElapsedTime(&ms, a, b));
-                    if (ms < best_ms) {
-                        best_ms = ms;
-                        best = ctx->tune;
-                    }
-                }
-        ctx->tune = best;
-        if ((rc = check_flags(ctx))) {
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-            return ens_from_ctx(e, ctx, rc);
+    struct Events {
+        hipEvent_t a = nullptr, b = nullptr;
+        ~Events()
+        {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
         }
-    }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    return ens_commit(e);  // back to the initial state
+    } ev;
+    ENSCHK(e, hipEventCreate(&ev.a));
+    ENSCHK(e, hipEventCreate(&ev.b));
+    // per slab, time every combination; the first error ends the trials
+    auto trials = [&]() -> int {
+        for (auto &s : e->slabs) {
+            fipi_ctx *ctx = s.ctx;
+            fipi_ctx::Tune best = ctx->tune;
+            float best_ms = 1e30f;
+            int ydef, yalt;
+            y_plans_of(ctx, &ydef, &yalt);
+            std::vector<int> plans = {0};  // 0: the default plan of the size
+            if (yalt != 0) plans.push_back(yalt);
+            for (int ye : plans)
+                for (int y : {64, 128})
+                    for (int h : {64, 128})
+                        for (int st : {64, 128}) {
+                            ctx->tune = fipi_ctx::Tune{y, h, st, ye};
+                            ens_step_slab(ctx);  // warm-up (first use of these kernels)
+                            ENSCHK(e, hipEventRecord(ev.a, ctx->stream));
+                            for (int k = 0; k < steps; k++) ens_step_slab(ctx);
+                            ENSCHK(e, hipEventRecord(ev.b, ctx->stream));
+                            ENSCHK(e, hipEventSynchronize(ev.b));
+                            float ms = 0.f;
+                            ENSCHK(e, hipEventElapsedTime(&ms, ev.a, ev.b));
+                            if (ms < best_ms) {
+                                best_ms = ms;
+                                best = ctx->tune;
+                            }
+                        }
+            ctx->tune = best;
+            const int frc = check_flags(ctx);
+            if (frc) return ens_from_ctx(e, ctx, frc);
+        }
+        return FIPI_OK;
+    };
+    rc = trials();
+    const int crc = ens_commit(e);  // back to the initial state, also after a failed trial
+    return rc ? rc : crc;
 }
 
 int fipi_ensemble_get_tuning(const fipi_ensemble *e, int32_t n, fipi_tuning_t *out)

@@ -1748,36 +1748,40 @@ int fipi_tune(fipi_ctx *ctx, int32_t steps)
     }
     const int t0 = ctx->t, cur0 = ctx->cur;
     const bool conv0 = ctx->conv_ready;
-    fipi_ctx::Tune best = ctx->tune;
+    const fipi_ctx::Tune before = ctx->tune;
+    fipi_ctx::Tune best = before;
     float best_ms = 1e30f;
     int ydef, yalt;
     y_plans_of(ctx, &ydef, &yalt);
-    for (int ye : {ydef, yalt})
-    for (int y : {64, 128})
-        for (int h : {64, 128})
-            for (int st : {64, 128}) {
-                if (ye == 0) continue;
-                ctx->tune.y_e = ye == ydef ? 0 : ye;
-                ctx->tune.y_tb = y;
-                ctx->tune.hub_tb = h;
-                ctx->tune.stokes_tb = st;
-                int rc = fipi_step_async(ctx, 1);  // warm-up (first use of these kernels)
-                if (rc) return rc;
-                HIPCHK(ctx, hipEventRecord(keep.a, ctx->stream));
-                if ((rc = fipi_step_async(ctx, steps))) return rc;
-                HIPCHK(ctx, hipEventRecord(keep.b, ctx->stream));
-                HIPCHK(ctx, hipEventSynchronize(keep.b));
-                float ms = 0.f;
-                HIPCHK(ctx, hipEventElapsedTime(&ms, keep.a, keep.b));
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = ctx->tune;
-                }
-            }
-    int rc = check_flags(ctx);
-    ctx->tune = best;
-    // back to the state before the trials (into the arrays in use now: the
-    // spare c / grad c set may have been swapped in)
+    std::vector<int> plans = {0};  // 0: the default plan of the size
+    if (yalt != 0) plans.push_back(yalt);
+    // time every combination; the first error ends the trials
+    auto trials = [&]() -> int {
+        for (int ye : plans)
+            for (int y : {64, 128})
+                for (int h : {64, 128})
+                    for (int st : {64, 128}) {
+                        ctx->tune = fipi_ctx::Tune{y, h, st, ye};
+                        int rc = fipi_step_async(ctx, 1);  // warm-up (first use of these kernels)
+                        if (rc) return rc;
+                        HIPCHK(ctx, hipEventRecord(keep.a, ctx->stream));
+                        if ((rc = fipi_step_async(ctx, steps))) return rc;
+                        HIPCHK(ctx, hipEventRecord(keep.b, ctx->stream));
+                        HIPCHK(ctx, hipEventSynchronize(keep.b));
+                        float ms = 0.f;
+                        HIPCHK(ctx, hipEventElapsedTime(&ms, keep.a, keep.b));
+                        if (ms < best_ms) {
+                            best_ms = ms;
+                            best = ctx->tune;
+                        }
+                    }
+        return check_flags(ctx);
+    };
+    const int rc = trials();
+    ctx->tune = rc == FIPI_OK ? best : before;
+    // back to the state before the trials, also after a failed one (into
+    // the arrays in use now: the spare c / grad c set may have been swapped in)
+    (void)hipStreamSynchronize(ctx->stream);
     const char *in = (const char *)keep.p;
     for (auto &x : snapshot_buffers(ctx)) {
         HIPCHK(ctx, hipMemcpyAsync(x.first, in, x.second, hipMemcpyDeviceToDevice, ctx->stream));
