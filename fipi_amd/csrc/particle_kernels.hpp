@@ -394,12 +394,13 @@ __device__ __forceinline__ double ramp_dev(double t, double thr)
 // particles per workgroup, taken in cell-sorted order): lane c < 27 scans
 // neighbour cell c in the reference's traversal order (k outer, then j,
 // then i) over the cell-sorted positions, and an ordered prefix sum over the
-// lanes builds the two partner lists in LDS.  The first lane then sums
-// f_pp[a] in the order the reference accumulates it: the contributions of
-// partners b < a in ascending b (added when the outer loop visited b), then
-// partners b > a in traversal order, with F(a,b) = -F(b,a) bitwise.
-// tau_i[a] = sum over b > a of 0.5 F.d when f_pi[a] and f_pi[mark] are
-// non-zero (cluster.cpp:256-259).
+// lanes places the two partner lists in LDS.  Each lane then writes the pair
+// forces of its own partners next to their ids, so the force evaluations run
+// in parallel.  The first lane sums f_pp[a] from LDS in the order the
+// reference accumulates it: the contributions of partners b < a in ascending
+// b (added when the outer loop visited b), then partners b > a in traversal
+// order, with F(a,b) = -F(b,a) bitwise.  tau_i[a] = sum over b > a of
+// 0.5 F.d when f_pi[a] and f_pi[mark] are non-zero (cluster.cpp:256-259).
 constexpr int CP_HITS = 4;  // partners per lane held before falling back
 
 __device__ __forceinline__ void pair_disp_pos(const PartParams &P, double ax, double ay, double az, double bx,
@@ -437,19 +438,37 @@ __device__ __forceinline__ int half_wave_incl_scan(int v)
     return v;
 }
 
+// Pair force on a from the partner at cell-sorted slot e, as pair_force()
+// computes it from the same positions; w[3] = 0.5 F.d (the torque term).
+__device__ __forceinline__ void contact_entry(const PartParams &P, const PartState &S, double ax, double ay,
+                                              double az, int a, int b, int e, double rmp, double *w)
+{
+    double d[3];
+    pair_disp_pos(P, ax, ay, az, S.sp[0][e], S.sp[1][e], S.sp[2][e], a, b, d);
+    const double nrm = vnorm(d[0], d[1], d[2]);
+    const double sc = (P.r_eq - nrm) * rmp * P.k_n;
+    const double f0 = (d[0] / nrm) * sc, f1 = (d[1] / nrm) * sc, f2 = (d[2] / nrm) * sc;
+    w[0] = f0;
+    w[1] = f1;
+    w[2] = f2;
+    w[3] = 0.5 * ((f0 * d[0] + f1 * d[1]) + f2 * d[2]);
+}
+
 __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, const int *__restrict__ start,
                                                      const int *__restrict__ ids, double rmp,
                                                      const int *__restrict__ t_dev, double ramp_t,
                                                      int *__restrict__ overflow)
 {
     __shared__ int lists[8][2][MAX_NB];
+    __shared__ double forces[8][2][MAX_NB][4];  // per partner: F(a,b) and 0.5 F.d
     const int sub = threadIdx.x >> 5, c = threadIdx.x & 31;
     const int s = blockIdx.x * 8 + sub;
     const int nsorted = start[P.ncell_box * P.nbox];
     const bool live = s < P.n && s < nsorted;
     if (t_dev) rmp = ramp_dev((double)(*t_dev + 1), ramp_t);
     int a = 0, nl = 0, nu = 0;
-    int hl[CP_HITS], hu[CP_HITS];
+    int hl[CP_HITS], hu[CP_HITS];  // partner ids
+    int sl[CP_HITS], su[CP_HITS];  // their cell-sorted slots
     bool ovf = false;
     double ax = 0, ay = 0, az = 0;
     int o = 0, pp = 0, q = 0, cb = 0;
@@ -471,11 +490,11 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
                 pair_disp_pos(P, ax, ay, az, S.sp[0][e], S.sp[1][e], S.sp[2][e], a, b, d);
                 if (vnorm(d[0], d[1], d[2]) < P.r_eq) {
                     if (b < a) {
-                        if (nl < CP_HITS) hl[nl] = b;
+                        if (nl < CP_HITS) { hl[nl] = b; sl[nl] = e; }
                         else ovf = true;
                         nl++;
                     } else {
-                        if (nu < CP_HITS) hu[nu] = b;
+                        if (nu < CP_HITS) { hu[nu] = b; su[nu] = e; }
                         else ovf = true;
                         nu++;
                     }
@@ -483,14 +502,47 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
             }
         }
     }
-    // ordered compaction of the lanes' partners into the half wave's lists
+    // ordered compaction of the lanes' partners and their forces into LDS
     const int il = half_wave_incl_scan(nl), iu = half_wave_incl_scan(nu);
     const int tl = __shfl(il, 31, 32), tu = __shfl(iu, 31, 32);
     const bool any_ovf = __any(ovf) != 0;  // per wave: either half falls back
-    const bool fits = !ovf && tl <= MAX_NB && tu <= MAX_NB;
+    const bool fits = !any_ovf && tl <= MAX_NB && tu <= MAX_NB;
     if (fits) {
-        for (int x = 0; x < nl && x < CP_HITS; x++) lists[sub][0][il - nl + x] = hl[x];
-        for (int x = 0; x < nu && x < CP_HITS; x++) lists[sub][1][iu - nu + x] = hu[x];
+#pragma unroll
+        for (int x = 0; x < CP_HITS; x++)
+            if (x < nl) {
+                lists[sub][0][il - nl + x] = hl[x];
+                contact_entry(P, S, ax, ay, az, a, hl[x], sl[x], rmp, forces[sub][0][il - nl + x]);
+            }
+#pragma unroll
+        for (int x = 0; x < CP_HITS; x++)
+            if (x < nu) {
+                lists[sub][1][iu - nu + x] = hu[x];
+                contact_entry(P, S, ax, ay, az, a, hu[x], su[x], rmp, forces[sub][1][iu - nu + x]);
+            }
+    }
+    __syncthreads();
+    // partners b < a into ascending id: lane x moves entry x to its stable
+    // rank (a partner seen through two wrapped neighbour cells appears twice,
+    // with equal forces)
+    int rk = -1;
+    double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+    if (fits && c < tl) {
+        const int v = lists[sub][0][c];
+        rk = 0;
+        for (int y = 0; y < tl; y++) {
+            const int u = lists[sub][0][y];
+            rk += (u < v || (u == v && y < c)) ? 1 : 0;
+        }
+        w0 = forces[sub][0][c][0];
+        w1 = forces[sub][0][c][1];
+        w2 = forces[sub][0][c][2];
+    }
+    __syncthreads();
+    if (rk >= 0) {
+        forces[sub][0][rk][0] = w0;
+        forces[sub][0][rk][1] = w1;
+        forces[sub][0][rk][2] = w2;
     }
     __syncthreads();
     if (!live || c != 0) return;
@@ -501,33 +553,23 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
                  vnorm(S.fpi[0][mk], S.fpi[1][mk], S.fpi[2][mk]) != 0.0;
     }
     double fo[3], tau = 0.0;
-    if (any_ovf || tl > MAX_NB || tu > MAX_NB) {
+    if (!fits) {
         atomicOr(overflow, 1);
         contact_slow(P, S, start, ids, a, o, pp, q, cb, rmp, tau_on, fo, &tau);
     } else {
-        int *low = lists[sub][0];
-        const int *up = lists[sub][1];
-        for (int x = 1; x < tl; x++) {
-            const int v = low[x];
-            int y = x - 1;
-            while (y >= 0 && low[y] > v) { low[y + 1] = low[y]; y--; }
-            low[y + 1] = v;
-        }
         double fx = 0.0, fy = 0.0, fz = 0.0;
         for (int x = 0; x < tl; x++) {
-            double f[3], d[3];
-            pair_force(P, S, a, low[x], rmp, f, d);
-            fx = fx + f[0];
-            fy = fy + f[1];
-            fz = fz + f[2];
+            const double *w = forces[sub][0][x];
+            fx = fx + w[0];
+            fy = fy + w[1];
+            fz = fz + w[2];
         }
         for (int x = 0; x < tu; x++) {
-            double f[3], d[3];
-            pair_force(P, S, a, up[x], rmp, f, d);
-            fx = fx + f[0];
-            fy = fy + f[1];
-            fz = fz + f[2];
-            if (tau_on) tau += 0.5 * ((f[0] * d[0] + f[1] * d[1]) + f[2] * d[2]);
+            const double *w = forces[sub][1][x];
+            fx = fx + w[0];
+            fy = fy + w[1];
+            fz = fz + w[2];
+            if (tau_on) tau += w[3];
         }
         fo[0] = fx;
         fo[1] = fy;
