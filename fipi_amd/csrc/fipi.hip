@@ -15,9 +15,8 @@
 //   z hub   Stokes projection of F; f -> f_hat (K) for the next step (main.cpp:177-178)
 //   y inv   u
 //   x pass  u (R); conv = u.grad c fwd -> H0 for the next step  (main.cpp:179, 91-92)
-//   two Heun particle stages + the next step's spread (main.cpp:198-217):
-//   within one call, on a second stream from the first x pass (which writes
-//   c, grad c) up to the next step's, beside the passes in between
+//   two Heun particle stages + the next step's spread, on a second stream
+//   beside the next step's first passes within one call  (main.cpp:198-217)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -103,16 +102,17 @@ struct fipi_ctx {
     // stages; their contact forces are spread every step unchanged)
     void *W = nullptr;
     bool fix_particles = false;
-    // Within fipi_step_async (single-box contexts with particles: beside_ok)
-    // the particle stage of one step, and the spread of its forces for the
-    // next step, run on pstream from the pass that writes c and grad c up to
-    // the next step's pass that writes them, beside the passes in between,
-    // which only read c and grad c and touch no particle state.  The main
-    // stream waits on ev_join before that next c / grad c pass (whose force
-    // pass also reads the spread; spread_ready: it is built).
+    // Within fipi_step_async the particle stage of one step, and the spread
+    // of its forces for the next step, run on pstream beside the next
+    // step's first passes, which touch no particle state.  The stage reads
+    // c and grad c, so that next step writes them into the spare set
+    // (c_alt, g_alt; single-box contexts with particles) and swaps.  The
+    // main stream waits on ev_join before the force pass that reads the
+    // spread (spread_ready: it is built).
     hipStream_t pstream = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-    bool beside_ok = false, p_pending = false, spread_ready = false;
+    bool p_pending = false, spread_ready = false;
+    void *c_alt = nullptr, *g_alt[3] = {};
     double *kx = nullptr, *ky = nullptr, *kz = nullptr;
     void *twx = nullptr, *twy = nullptr, *twz = nullptr;
     // particles
@@ -547,15 +547,13 @@ struct Impl {
         }
     }
 
-    // overlap_next: this step's particle stage may run beside the rest of
-    // this step and the next step's first passes
+    // overlap_next: this step's particle stage may run beside the next step
     static void step(fipi_ctx *ctx, bool overlap_next = false)
     {
         const Dims d = ctx->d;
         hipStream_t st = ctx->stream;
         const bool ns = ctx->p.switch_ns != 0;
         const bool parts = ctx->n > 0;
-        const bool beside = parts && !ctx->fix_particles && overlap_next;
         C *h0 = H(ctx->h[0]), *h1 = H(ctx->h[1]), *h2 = H(ctx->h[2]), *h3 = H(ctx->h[3]), *fk = H(ctx->fk);
         const double Rb = (double)ctx->G * sizeof(T), Hb = (double)ctx->GH * 2 * sizeof(T);
         const bool mixed = !std::is_same<T, double>::value;
@@ -609,14 +607,16 @@ struct Impl {
             ProfScope ps_(ctx, "y_inv_1", 2 * Hb);
             y_pass<+1, 1, false>(ctx, h2, nullptr, nullptr, nullptr);
         }
-        // the previous step's particle stage reads c and grad c until here
-        join_particles(ctx);
+        if (ctx->p_pending) {
+            // the previous step's particle stage still reads c and grad c
+            std::swap(ctx->c, ctx->c_alt);
+            for (int a = 0; a < 3; a++) std::swap(ctx->g[a], ctx->g_alt[a]);
+        }
         if (!ns) {
             {
                 ProfScope ps_(ctx, "x_inv_cgrad_f", 4 * Hb + 4 * Rb);
                 cgrad_f(ctx, h3, fk);
             }
-            if (beside) particle_stages_beside(ctx);
             {
                 ProfScope ps_(ctx, "y_fwd_f", 2 * Hb);
                 y_pass<-1, 1, false>(ctx, fk, nullptr, nullptr, nullptr);
@@ -625,6 +625,7 @@ struct Impl {
             z_pass<-1>(ctx, fk);
         }
         if (ns) {
+            join_particles(ctx);
             if (parts && !ctx->spread_ready) spread_now(ctx);
             ctx->spread_ready = false;
             {
@@ -639,7 +640,6 @@ struct Impl {
                                        ctx->kx, (const C *)ctx->twx);
                 });
             }
-            if (beside) particle_stages_beside(ctx);
             {
                 ProfScope ps_(ctx, "y_fwd_f", 2 * Hb);
                 y_pass<-1, 1, false>(ctx, fk, nullptr, nullptr, nullptr);
@@ -677,8 +677,10 @@ struct Impl {
                 ctx->conv_ready = true;
             }
         }
-        if (!beside) {
-            join_particles(ctx);
+        join_particles(ctx);
+        if (parts && !ctx->fix_particles && overlap_next) {
+            particle_stages_beside(ctx);
+        } else {
             if (parts && !ctx->fix_particles) {
                 particle_stage(ctx, 1);
                 particle_stage(ctx, 2);
@@ -824,7 +826,10 @@ int setup_buffers(fipi_ctx *ctx)
 #define ALLOC(ptr, bytes) \
     if ((rc = dalloc(ctx, (void **)&(ptr), (bytes))) != FIPI_OK) return rc
     ALLOC(ctx->c, G * sizeof(T));
-    ctx->beside_ok = ctx->n > 0 && nb == 1;
+    if (ctx->n > 0 && nb == 1) {
+        ALLOC(ctx->c_alt, G * sizeof(T));
+        for (int a = 0; a < 3; a++) ALLOC(ctx->g_alt[a], G * sizeof(T));
+    }
     for (int a = 0; a < 3; a++) {
         ALLOC(ctx->g[a], G * sizeof(T));
         ALLOC(ctx->u[a], G * sizeof(T));
@@ -1337,7 +1342,7 @@ int fipi_step_async(fipi_ctx *ctx, int32_t nsteps)
     // each step's particle stage runs beside the next step of this call
     // (never under per-launch timing, whose events are on the main stream);
     // the last step's runs in order, so all work ends on the main stream
-    const bool overlap = ctx->beside_ok && !ctx->prof_on && !ctx->fix_particles;
+    const bool overlap = ctx->c_alt != nullptr && !ctx->prof_on && !ctx->fix_particles;
     for (int s = 0; s < nsteps; s++) {
         const bool next = overlap && s + 1 < nsteps;
         if (ctx->fp64) Impl<double>::step(ctx, next);
