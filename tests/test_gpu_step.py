@@ -167,3 +167,31 @@ def test_headline_size_256(gpu):
     assert rel_l2(g_p["p"], r_p["p"]) < TOL, rel_l2(g_p["p"], r_p["p"])
     assert rel_l2(g_p["f_pi"], r_p["f_pi"]) < 1e-8, rel_l2(g_p["f_pi"], r_p["f_pi"])
 
+
+
+def test_headline_density_128(gpu):
+    """The headline's sphere density and contact angles at 128^3 (12,500
+    spheres of radius 2 dx, U(60, 120) deg), 200 FP64 steps against the
+    oracle (about half a minute on 16 host threads)."""
+    import os
+    print("128^3 + 12.5k, 200 steps", flush=True)
+    n, npart, steps = 128, 12500, 200
+    p = fipi_amd.config_params(n, npart, 2.0, steps)
+    phi0 = fipi_amd.init_phi((n, n, n), 0.0, 0.05, 1)
+    pos = fipi_amd.init_rsa(npart, 2.0, 0.5, (p.lx, p.ly, p.lz), 2)
+    assert pos.shape[0] == npart
+    cth = np.cos(np.deg2rad(fipi_amd.init_uniform(npart, 60.0, 120.0, 3)))
+    sim = fipi_amd.Simulation(p)
+    sim.set_state(phi0, pos, None, cth)
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    ref = oracle.Sim(p, phi0, pos, None, cth, n_threads=threads)
+    for _ in range(2):
+        sim.step(steps // 2)
+        ref.step(steps // 2)
+        g_p, r_p = sim.particles(), ref.particles()
+        e = (rel_l2(sim.phi(), ref.phi()), rel_l2(sim.velocity(), ref.u()), rel_l2(g_p["p"], r_p["p"]),
+             rel_l2(g_p["f_pi"], r_p["f_pi"]))
+        print(sim.t, e, flush=True)
+        assert e[0] < TOL and e[1] < TOL and e[2] < TOL, (sim.t, e)
+        assert e[3] < 1e-8, (sim.t, e)
+    sim.close()
