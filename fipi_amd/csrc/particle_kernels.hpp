@@ -326,10 +326,8 @@ __device__ __forceinline__ bool pair_force(const PartParams &P, const PartState 
 
 __device__ __forceinline__ int wrapc(int v, int n) { return (v + n) % n; }
 
-constexpr int MAX_NB = 32;  // contact partners per side held for ordering
-
-// Reference-order force sum by repeated scans, for particles with more than
-// MAX_NB contacts on one side (not expected for spheres; kept exact).
+// Reference-order force sum by repeated scans, for the waves whose partner
+// lists do not fit the ordered path of k_contact_par (kept exact).
 __device__ void contact_slow(const PartParams &P, const PartState &S, const int *__restrict__ start,
                              const int *__restrict__ ids, int a, int o, int pp, int q, int cb, double rmp,
                              bool tau_on, double *fo, double *tau_out)
@@ -390,18 +388,27 @@ __device__ __forceinline__ double ramp_dev(double t, double thr)
     return 1.0;
 }
 
-// Contact forces (cluster.cpp:174-275) with 32 lanes per particle (8
-// particles per workgroup, taken in cell-sorted order): lane c < 27 scans
-// neighbour cell c in the reference's traversal order (k outer, then j,
-// then i) over the cell-sorted positions, and an ordered prefix sum over the
-// lanes places the two partner lists in LDS.  Each lane then writes the pair
-// forces of its own partners next to their ids, so the force evaluations run
-// in parallel.  The first lane sums f_pp[a] from LDS in the order the
-// reference accumulates it: the contributions of partners b < a in ascending
-// b (added when the outer loop visited b), then partners b > a in traversal
-// order, with F(a,b) = -F(b,a) bitwise.  tau_i[a] = sum over b > a of
-// 0.5 F.d when f_pi[a] and f_pi[mark] are non-zero (cluster.cpp:256-259).
-constexpr int CP_HITS = 4;  // partners per lane held before falling back
+// Contact forces (cluster.cpp:174-275) with CP_LANES lanes per particle
+// (CP_PPB particles per workgroup, taken in cell-sorted order): lane c scans
+// neighbour cells c*CP_CELLS .. c*CP_CELLS+CP_CELLS-1 in the reference's
+// traversal order (k outer, then j, then i) over the cell-sorted positions,
+// so the lanes' partners, concatenated in lane order, are in traversal
+// order; an ordered prefix sum over the lanes places the two partner lists
+// in LDS.  Each lane writes the pair forces of its own partners next to
+// their ids, so the force evaluations run in parallel.  The first lane sums
+// f_pp[a] from LDS in the order the reference accumulates it: the
+// contributions of partners b < a in ascending b (added when the outer loop
+// visited b), then partners b > a in traversal order, with F(a,b) = -F(b,a)
+// bitwise.  tau_i[a] = sum over b > a of 0.5 F.d when f_pi[a] and
+// f_pi[mark] are non-zero (cluster.cpp:256-259).  A wave in which some lane
+// holds more than CP_HITS partners on one side, or some particle more than
+// CP_NB, takes the exact repeated-scan path instead.
+constexpr int CP_LANES = 8;                                    // lanes per particle
+constexpr int CP_CELLS = (27 + CP_LANES - 1) / CP_LANES;       // neighbour cells per lane
+constexpr int CP_HITS = 6;                                     // partners per lane and side
+constexpr int CP_NB = 16;                                      // partners per side in LDS
+constexpr int CP_PPB = 256 / CP_LANES;                         // particles per workgroup
+static_assert(CP_NB % CP_LANES == 0, "rank pass: whole entries per lane");
 
 __device__ __forceinline__ void pair_disp_pos(const PartParams &P, double ax, double ay, double az, double bx,
                                               double by, double bz, int a, int b, double *d)
@@ -426,13 +433,13 @@ __device__ __forceinline__ void pair_disp_pos(const PartParams &P, double ax, do
     }
 }
 
-// inclusive sum over the 32 lanes of each half wave
-__device__ __forceinline__ int half_wave_incl_scan(int v)
+// inclusive sum over the CP_LANES lanes of each particle
+__device__ __forceinline__ int seg_incl_scan(int v)
 {
-    const int lane = threadIdx.x & 31;
+    const int lane = threadIdx.x & (CP_LANES - 1);
 #pragma unroll
-    for (int o = 1; o < 32; o <<= 1) {
-        const int u = __shfl_up(v, o, 32);
+    for (int o = 1; o < CP_LANES; o <<= 1) {
+        const int u = __shfl_up(v, o, CP_LANES);
         if (lane >= o) v += u;
     }
     return v;
@@ -459,10 +466,10 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
                                                      const int *__restrict__ t_dev, double ramp_t,
                                                      int *__restrict__ overflow)
 {
-    __shared__ int lists[8][2][MAX_NB];
-    __shared__ double forces[8][2][MAX_NB][4];  // per partner: F(a,b) and 0.5 F.d
-    const int sub = threadIdx.x >> 5, c = threadIdx.x & 31;
-    const int s = blockIdx.x * 8 + sub;
+    __shared__ int lists[CP_PPB][2][CP_NB];
+    __shared__ double forces[CP_PPB][2][CP_NB][4];  // per partner: F(a,b) and 0.5 F.d
+    const int sub = threadIdx.x / CP_LANES, c = threadIdx.x % CP_LANES;
+    const int s = blockIdx.x * CP_PPB + sub;
     const int nsorted = start[P.ncell_box * P.nbox];
     const bool live = s < P.n && s < nsorted;
     if (t_dev) rmp = ramp_dev((double)(*t_dev + 1), ramp_t);
@@ -479,8 +486,11 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
         az = S.sp[2][s];
         cb = part_box(S.box, a) * P.ncell_box;
         cell_of(P, ax, ay, az, &o, &pp, &q);
-        if (c < 27) {
-            const int k = q - 1 + c / 9, j = pp - 1 + (c / 3) % 3, i = o - 1 + c % 3;
+#pragma unroll
+        for (int r = 0; r < CP_CELLS; r++) {
+            const int cc = c * CP_CELLS + r;
+            if (cc >= 27) break;
+            const int k = q - 1 + cc / 9, j = pp - 1 + (cc / 3) % 3, i = o - 1 + cc % 3;
             const int co = cb + wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
             const int e1 = start[co + 1];
             for (int e = start[co]; e < e1; e++) {
@@ -503,10 +513,10 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
         }
     }
     // ordered compaction of the lanes' partners and their forces into LDS
-    const int il = half_wave_incl_scan(nl), iu = half_wave_incl_scan(nu);
-    const int tl = __shfl(il, 31, 32), tu = __shfl(iu, 31, 32);
-    const bool any_ovf = __any(ovf) != 0;  // per wave: either half falls back
-    const bool fits = !any_ovf && tl <= MAX_NB && tu <= MAX_NB;
+    const int il = seg_incl_scan(nl), iu = seg_incl_scan(nu);
+    const int tl = __shfl(il, CP_LANES - 1, CP_LANES), tu = __shfl(iu, CP_LANES - 1, CP_LANES);
+    const bool any_ovf = __any(ovf) != 0;  // per wave: all its particles fall back
+    const bool fits = !any_ovf && tl <= CP_NB && tu <= CP_NB;
     if (fits) {
 #pragma unroll
         for (int x = 0; x < CP_HITS; x++)
@@ -522,28 +532,38 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
             }
     }
     __syncthreads();
-    // partners b < a into ascending id: lane x moves entry x to its stable
-    // rank (a partner seen through two wrapped neighbour cells appears twice,
-    // with equal forces)
-    int rk = -1;
-    double w0 = 0.0, w1 = 0.0, w2 = 0.0;
-    if (fits && c < tl) {
-        const int v = lists[sub][0][c];
-        rk = 0;
-        for (int y = 0; y < tl; y++) {
-            const int u = lists[sub][0][y];
-            rk += (u < v || (u == v && y < c)) ? 1 : 0;
+    // partners b < a into ascending id: entry x moves to its stable rank (a
+    // partner seen through two wrapped neighbour cells appears twice, with
+    // equal forces)
+    constexpr int PER = CP_NB / CP_LANES;
+    int rk[PER];
+    double w[PER][3];
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const int x = c + t * CP_LANES;
+        rk[t] = -1;
+        w[t][0] = w[t][1] = w[t][2] = 0.0;
+        if (fits && x < tl) {
+            const int v = lists[sub][0][x];
+            int r = 0;
+            for (int y = 0; y < tl; y++) {
+                const int u = lists[sub][0][y];
+                r += (u < v || (u == v && y < x)) ? 1 : 0;
+            }
+            rk[t] = r;
+            w[t][0] = forces[sub][0][x][0];
+            w[t][1] = forces[sub][0][x][1];
+            w[t][2] = forces[sub][0][x][2];
         }
-        w0 = forces[sub][0][c][0];
-        w1 = forces[sub][0][c][1];
-        w2 = forces[sub][0][c][2];
     }
     __syncthreads();
-    if (rk >= 0) {
-        forces[sub][0][rk][0] = w0;
-        forces[sub][0][rk][1] = w1;
-        forces[sub][0][rk][2] = w2;
-    }
+#pragma unroll
+    for (int t = 0; t < PER; t++)
+        if (rk[t] >= 0) {
+            forces[sub][0][rk[t]][0] = w[t][0];
+            forces[sub][0][rk[t]][1] = w[t][1];
+            forces[sub][0][rk[t]][2] = w[t][2];
+        }
     __syncthreads();
     if (!live || c != 0) return;
     const int mk = *S.mark;
@@ -559,17 +579,17 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
     } else {
         double fx = 0.0, fy = 0.0, fz = 0.0;
         for (int x = 0; x < tl; x++) {
-            const double *w = forces[sub][0][x];
-            fx = fx + w[0];
-            fy = fy + w[1];
-            fz = fz + w[2];
+            const double *f = forces[sub][0][x];
+            fx = fx + f[0];
+            fy = fy + f[1];
+            fz = fz + f[2];
         }
         for (int x = 0; x < tu; x++) {
-            const double *w = forces[sub][1][x];
-            fx = fx + w[0];
-            fy = fy + w[1];
-            fz = fz + w[2];
-            if (tau_on) tau += w[3];
+            const double *f = forces[sub][1][x];
+            fx = fx + f[0];
+            fy = fy + f[1];
+            fz = fz + f[2];
+            if (tau_on) tau += f[3];
         }
         fo[0] = fx;
         fo[1] = fy;
