@@ -445,13 +445,13 @@ struct Impl {
         }
         {
             ProfScope ps_(ctx, "spread_rank", 16.0 * n * E + 4.0 * nl);
-            hipLaunchKernelGGL((k_spread_rank<T>), grid1(nl, 4), dim3(256), 0, st, nl, ctx->line_start,
+            hipLaunchKernelGGL((k_spread_rank<T>), grid1(nl, SPREAD_LPB), dim3(256), 0, st, nl, ctx->line_start,
                                (const SpreadEntry<T> *)ctx->spr_tmp, (SpreadEntry<T> *)ctx->spr);
         }
         {
             // per-node sums into the (now free) scatter buffer
             ProfScope ps_(ctx, "spread_nodes", 8.0 * n * E + 8.0 * n * sizeof(SpreadNode<T>) + 8.0 * nl);
-            hipLaunchKernelGGL((k_spread_nodes<T>), grid1(nl, 4), dim3(256), 0, st, nl, ctx->line_start,
+            hipLaunchKernelGGL((k_spread_nodes<T>), grid1(nl, SPREAD_LPB), dim3(256), 0, st, nl, ctx->line_start,
                                (const SpreadEntry<T> *)ctx->spr, (SpreadNode<T> *)ctx->spr_tmp, ctx->node_cnt,
                                ctx->line_cnt, ctx->line_cursor);
         }

@@ -857,34 +857,41 @@ __global__ void k_spread_scatter(PartParams P, const double *__restrict__ px, co
     }
 }
 
-// Sort each line's entries by (x, particle id) into `out`: one wave per
-// line; each chunk of 64 entries is held one per lane, and every lane ranks
-// its entry by counting smaller keys among all entries of the line (keys of
-// the chunk in hand are broadcast from their lanes).  Within a line the keys
-// are unique (a particle meets a line at two different x), so the ranks are
-// a permutation and the result is deterministic.
+// Lanes per grid line in the spread's sort and node passes: a line holds
+// 8 entries per sphere that touches it, about a dozen at the headline density.
+constexpr int SPREAD_LANES = 16;
+constexpr int SPREAD_LPB = 256 / SPREAD_LANES;  // lines per workgroup
+
+// Sort each line's entries by (x, particle id) into `out`: SPREAD_LANES
+// lanes per line; each chunk of SPREAD_LANES entries is held one per lane,
+// and every lane ranks its entry by counting smaller keys among all entries
+// of the line (keys of the chunk in hand are broadcast from their lanes).
+// Within a line the keys are unique (a particle meets a line at two
+// different x), so the ranks are a permutation and the result is
+// deterministic.
 template <typename T>
 __global__ __launch_bounds__(256) void k_spread_rank(int nlines, const int *__restrict__ line_start,
                                                      const SpreadEntry<T> *__restrict__ in,
                                                      SpreadEntry<T> *__restrict__ out)
 {
-    const int line = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (line >= nlines) return;
+    constexpr int W = SPREAD_LANES;
+    const int line = blockIdx.x * SPREAD_LPB + (int)threadIdx.x / W;
+    const int lane = threadIdx.x % W;
+    if (line >= nlines) return;  // whole segments leave together
     const int b = line_start[line], e = line_start[line + 1];
-    for (int i0 = b; i0 < e; i0 += 64) {
+    for (int i0 = b; i0 < e; i0 += W) {
         const int i = i0 + lane;
         const bool mine = i < e;
         SpreadEntry<T> v{};
         if (mine) v = in[i];
         int r = 0;
-        for (int j0 = b; j0 < e; j0 += 64) {
+        for (int j0 = b; j0 < e; j0 += W) {
             // chunk j0 of the line, one key per lane
             const int jj = j0 + lane;
             const int kx = jj < e ? in[jj].x : 0x7fffffff, kp = jj < e ? in[jj].pid : 0x7fffffff;
-            const int m = min(64, e - j0);
+            const int m = min(W, e - j0);
             for (int k = 0; k < m; k++) {
-                const int xj = __shfl(kx, k), pj = __shfl(kp, k);
+                const int xj = __shfl(kx, k, W), pj = __shfl(kp, k, W);
                 r += (xj < v.x || (xj == v.x && pj < v.pid)) ? 1 : 0;
             }
         }
@@ -892,20 +899,23 @@ __global__ __launch_bounds__(256) void k_spread_rank(int nlines, const int *__re
     }
 }
 
-// Per-node sums of the sorted entries of each line: one wave per line; the
-// lane at the head of each run of equal x sums the run in ascending particle
-// order (the reference's accumulation order, cluster.cpp:355-407) and the
-// heads are compacted with a wave ballot + popcount.  Node k of line l is
-// stored at nodes[line_start[l] + k]; node_cnt[l] gets the count.
-// line_cnt and line_cursor of the line are cleared for the next build.
+// Per-node sums of the sorted entries of each line, SPREAD_LANES lanes per
+// line: the lane at the head of each run of equal x sums the run in
+// ascending particle order (the reference's accumulation order,
+// cluster.cpp:355-407) and the heads are compacted with a ballot + popcount
+// over the line's lanes.  Node k of line l is stored at
+// nodes[line_start[l] + k]; node_cnt[l] gets the count.  line_cnt and
+// line_cursor of the line are cleared for the next build.
 template <typename T>
 __global__ __launch_bounds__(256) void k_spread_nodes(int nlines, const int *__restrict__ line_start,
                                                       const SpreadEntry<T> *__restrict__ e,
                                                       SpreadNode<T> *__restrict__ nodes, int *__restrict__ node_cnt,
                                                       int *__restrict__ line_cnt, int *__restrict__ line_cursor)
 {
-    const int line = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    constexpr int W = SPREAD_LANES;
+    const int line = blockIdx.x * SPREAD_LPB + (int)threadIdx.x / W;
+    const int lane = threadIdx.x % W;
+    const int shift = (int)(threadIdx.x & 63) - lane;  // first wave lane of this line
     if (line >= nlines) return;
     if (lane == 0) {
         line_cnt[line] = 0;
@@ -913,11 +923,11 @@ __global__ __launch_bounds__(256) void k_spread_nodes(int nlines, const int *__r
     }
     const int b = line_start[line], en = line_start[line + 1];
     int base = 0;
-    for (int c0 = b; c0 < en; c0 += 64) {
+    for (int c0 = b; c0 < en; c0 += W) {
         const int i = c0 + lane;
         const bool valid = i < en;
         const bool head = valid && (i == b || e[i - 1].x != e[i].x);
-        const unsigned long long mask = __ballot(head);
+        const unsigned long long seg = (__ballot(head) >> shift) & ((1ull << W) - 1ull);
         if (head) {
             const int x = e[i].x;
             T s0 = T(0), s1 = T(0), s2 = T(0);
@@ -926,7 +936,7 @@ __global__ __launch_bounds__(256) void k_spread_nodes(int nlines, const int *__r
                 s1 += e[j].v[1];
                 s2 += e[j].v[2];
             }
-            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            const int rank = __popcll(seg & ((1ull << lane) - 1ull));
             SpreadNode<T> nd;
             nd.x = x;
             nd.v[0] = s0;
@@ -934,7 +944,7 @@ __global__ __launch_bounds__(256) void k_spread_nodes(int nlines, const int *__r
             nd.v[2] = s2;
             nodes[b + base + rank] = nd;
         }
-        base += __popcll(mask);
+        base += __popcll(seg);
     }
     if (lane == 0) node_cnt[line] = base;
 }
