@@ -2,7 +2,7 @@
 src/main.cpp:75-222) on identical seeded inputs.
 
 Bar: relative L2 error of phi, u and particle positions <= 1e-10 (the
-north-star FP64 tolerance); discrete sets (cells, contact pairs) bit-exact
+FP64 tolerance of the path); discrete sets (cells, contact pairs) bit-exact
 in test_gpu_particles.py."""
 import numpy as np
 import pytest
@@ -101,7 +101,7 @@ def test_step_config1_64(gpu):
 
 
 def test_1000_steps_64(gpu):
-    """The north-star bar (1e-10 relative L2 after 1000 FP64 steps) at 64^3
+    """The FP64 bar (1e-10 relative L2 after 1000 steps) at 64^3
     with 800 spheres of radius 2.5 (config-1 physics), checked every 250
     steps; the oracle takes about half a minute for this on the host."""
     print("1000 steps 64^3", flush=True)
