@@ -538,8 +538,8 @@ struct Impl {
         const double rmp = ramp((double)(ctx->t + 1), ctx->p.ramp_t);
         {
             ProfScope ps_(ctx, "contact", pp.n * (24.0 + 4.0 + 24.0 + 8.0 + 24.0));
-            hipLaunchKernelGGL(k_contact_par, grid1(pp.n, CP_PPB), dim3(256), 0, st, pp, ctx->ps, ctx->cell_start,
-                               ctx->cell_ids, rmp, (const int *)ctx->t_dev, ctx->p.ramp_t, ctx->d_flags + 1);
+            launch_contact(st, pp.n, pp, ctx->ps, ctx->cell_start, ctx->cell_ids, rmp, (const int *)ctx->t_dev,
+                           ctx->p.ramp_t, ctx->d_flags + 1);
         }
         {
             ProfScope ps_(ctx, "advect", pp.n * (5 * 24.0 + 8.0 + 2 * 24.0));
@@ -1492,8 +1492,8 @@ int fipi_contact_forces(fipi_ctx *ctx, const double *d_pos, int32_t n, int32_t t
     const double thr = ctx->p.ramp_t;
     const double rmp = t < 0 ? 0.0 : (t < thr ? t / thr : 1.0);
     if (n > 0) {
-        hipLaunchKernelGGL(k_contact_par, grid1(n, CP_PPB), dim3(256), 0, ctx->stream, pp, tp.ps, ctx->cell_start, ids,
-                           rmp, (const int *)nullptr, 0.0, ctx->d_flags + 1);
+        launch_contact(ctx->stream, n, pp, tp.ps, ctx->cell_start, ids, rmp, (const int *)nullptr, 0.0,
+                       ctx->d_flags + 1);
         hipLaunchKernelGGL(k_soa_to_aos, grid1(n, 256), dim3(256), 0, ctx->stream, tp.ps.fpp[0], tp.ps.fpp[1],
                            tp.ps.fpp[2], d_f_pp, n);
     }

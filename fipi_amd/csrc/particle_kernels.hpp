@@ -388,9 +388,9 @@ __device__ __forceinline__ double ramp_dev(double t, double thr)
     return 1.0;
 }
 
-// Contact forces (cluster.cpp:174-275) with CP_LANES lanes per particle
-// (CP_PPB particles per workgroup, taken in cell-sorted order): lane c scans
-// neighbour cells c*CP_CELLS .. c*CP_CELLS+CP_CELLS-1 in the reference's
+// Contact forces (cluster.cpp:174-275) with LANES lanes per particle
+// (256/LANES particles per workgroup, taken in cell-sorted order): lane c
+// scans neighbour cells c*CELLS .. c*CELLS+CELLS-1 in the reference's
 // traversal order (k outer, then j, then i) over the cell-sorted positions,
 // so the lanes' partners, concatenated in lane order, are in traversal
 // order; an ordered prefix sum over the lanes places the two partner lists
@@ -402,13 +402,18 @@ __device__ __forceinline__ double ramp_dev(double t, double thr)
 // bitwise.  tau_i[a] = sum over b > a of 0.5 F.d when f_pi[a] and
 // f_pi[mark] are non-zero (cluster.cpp:256-259).  A wave in which some lane
 // holds more than CP_HITS partners on one side, or some particle more than
-// CP_NB, takes the exact repeated-scan path instead.
-constexpr int CP_LANES = 8;                                    // lanes per particle
-constexpr int CP_CELLS = (27 + CP_LANES - 1) / CP_LANES;       // neighbour cells per lane
-constexpr int CP_HITS = 6;                                     // partners per lane and side
-constexpr int CP_NB = 16;                                      // partners per side in LDS
-constexpr int CP_PPB = 256 / CP_LANES;                         // particles per workgroup
-static_assert(CP_NB % CP_LANES == 0, "rank pass: whole entries per lane");
+// NB, takes the exact repeated-scan path instead.  Few particles (small
+// boxes) take 32 lanes, one per neighbour cell, for the shortest chain of
+// dependent loads; many take 8 lanes of 4 cells each, a quarter of the
+// waves, which leaves more of the GPU to the field passes beside them.
+constexpr int CP_HITS = 6;                   // partners per lane and side
+constexpr int CP_WIDE_MIN = 16384;           // particles from which 8 lanes are used
+template <int LANES> struct ContactShape {
+    static constexpr int CELLS = (27 + LANES - 1) / LANES;  // neighbour cells per lane
+    static constexpr int NB = LANES > 16 ? LANES : 16;      // partners per side in LDS
+    static constexpr int PPB = 256 / LANES;                 // particles per workgroup
+    static_assert(NB % LANES == 0, "rank pass: whole entries per lane");
+};
 
 __device__ __forceinline__ void pair_disp_pos(const PartParams &P, double ax, double ay, double az, double bx,
                                               double by, double bz, int a, int b, double *d)
@@ -433,13 +438,14 @@ __device__ __forceinline__ void pair_disp_pos(const PartParams &P, double ax, do
     }
 }
 
-// inclusive sum over the CP_LANES lanes of each particle
+// inclusive sum over the LANES lanes of each particle
+template <int LANES>
 __device__ __forceinline__ int seg_incl_scan(int v)
 {
-    const int lane = threadIdx.x & (CP_LANES - 1);
+    const int lane = threadIdx.x & (LANES - 1);
 #pragma unroll
-    for (int o = 1; o < CP_LANES; o <<= 1) {
-        const int u = __shfl_up(v, o, CP_LANES);
+    for (int o = 1; o < LANES; o <<= 1) {
+        const int u = __shfl_up(v, o, LANES);
         if (lane >= o) v += u;
     }
     return v;
@@ -461,15 +467,18 @@ __device__ __forceinline__ void contact_entry(const PartParams &P, const PartSta
     w[3] = 0.5 * ((f0 * d[0] + f1 * d[1]) + f2 * d[2]);
 }
 
+template <int LANES>
 __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, const int *__restrict__ start,
                                                      const int *__restrict__ ids, double rmp,
                                                      const int *__restrict__ t_dev, double ramp_t,
                                                      int *__restrict__ overflow)
 {
-    __shared__ int lists[CP_PPB][2][CP_NB];
-    __shared__ double forces[CP_PPB][2][CP_NB][4];  // per partner: F(a,b) and 0.5 F.d
-    const int sub = threadIdx.x / CP_LANES, c = threadIdx.x % CP_LANES;
-    const int s = blockIdx.x * CP_PPB + sub;
+    using CS = ContactShape<LANES>;
+    constexpr int CELLS = CS::CELLS, NB = CS::NB, PPB = CS::PPB;
+    __shared__ int lists[PPB][2][NB];
+    __shared__ double forces[PPB][2][NB][4];  // per partner: F(a,b) and 0.5 F.d
+    const int sub = threadIdx.x / LANES, c = threadIdx.x % LANES;
+    const int s = blockIdx.x * PPB + sub;
     const int nsorted = start[P.ncell_box * P.nbox];
     const bool live = s < P.n && s < nsorted;
     if (t_dev) rmp = ramp_dev((double)(*t_dev + 1), ramp_t);
@@ -487,8 +496,8 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
         cb = part_box(S.box, a) * P.ncell_box;
         cell_of(P, ax, ay, az, &o, &pp, &q);
 #pragma unroll
-        for (int r = 0; r < CP_CELLS; r++) {
-            const int cc = c * CP_CELLS + r;
+        for (int r = 0; r < CELLS; r++) {
+            const int cc = c * CELLS + r;
             if (cc >= 27) break;
             const int k = q - 1 + cc / 9, j = pp - 1 + (cc / 3) % 3, i = o - 1 + cc % 3;
             const int co = cb + wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
@@ -513,10 +522,10 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
         }
     }
     // ordered compaction of the lanes' partners and their forces into LDS
-    const int il = seg_incl_scan(nl), iu = seg_incl_scan(nu);
-    const int tl = __shfl(il, CP_LANES - 1, CP_LANES), tu = __shfl(iu, CP_LANES - 1, CP_LANES);
+    const int il = seg_incl_scan<LANES>(nl), iu = seg_incl_scan<LANES>(nu);
+    const int tl = __shfl(il, LANES - 1, LANES), tu = __shfl(iu, LANES - 1, LANES);
     const bool any_ovf = __any(ovf) != 0;  // per wave: all its particles fall back
-    const bool fits = !any_ovf && tl <= CP_NB && tu <= CP_NB;
+    const bool fits = !any_ovf && tl <= NB && tu <= NB;
     if (fits) {
 #pragma unroll
         for (int x = 0; x < CP_HITS; x++)
@@ -535,12 +544,12 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
     // partners b < a into ascending id: entry x moves to its stable rank (a
     // partner seen through two wrapped neighbour cells appears twice, with
     // equal forces)
-    constexpr int PER = CP_NB / CP_LANES;
+    constexpr int PER = NB / LANES;
     int rk[PER];
     double w[PER][3];
 #pragma unroll
     for (int t = 0; t < PER; t++) {
-        const int x = c + t * CP_LANES;
+        const int x = c + t * LANES;
         rk[t] = -1;
         w[t][0] = w[t][1] = w[t][2] = 0.0;
         if (fits && x < tl) {
@@ -599,6 +608,18 @@ __global__ __launch_bounds__(256) void k_contact_par(PartParams P, PartState S, 
     S.fpp[1][a] = fo[1];
     S.fpp[2][a] = fo[2];
     S.tau_i[a] = tau;
+}
+
+// Launch of k_contact_par for n particles (the lane count by n).
+inline void launch_contact(hipStream_t st, int n, const PartParams &P, const PartState &S, const int *start,
+                           const int *ids, double rmp, const int *t_dev, double ramp_t, int *overflow)
+{
+    const unsigned ppb = n >= CP_WIDE_MIN ? ContactShape<8>::PPB : ContactShape<32>::PPB;
+    const dim3 grid((unsigned)(n <= 0 ? 1 : (n + ppb - 1) / ppb));
+    if (n >= CP_WIDE_MIN)
+        hipLaunchKernelGGL(k_contact_par<8>, grid, dim3(256), 0, st, P, S, start, ids, rmp, t_dev, ramp_t, overflow);
+    else
+        hipLaunchKernelGGL(k_contact_par<32>, grid, dim3(256), 0, st, P, S, start, ids, rmp, t_dev, ramp_t, overflow);
 }
 
 // Contact pairs (a < b) per particle: count pass then write pass.

@@ -55,6 +55,22 @@ def test_contact_forces_bitwise(gpu):
     sim.close()
 
 
+def test_contact_forces_bitwise_many(gpu):
+    """20,000 overlapping spheres: the kernel variant for many particles (8
+    lanes of 4 neighbour cells each, from 16,384 particles on)."""
+    n = 128
+    p = config_params(n, 20000, 2.0, 100)
+    sim = fipi_amd.Simulation(p)
+    pos = fipi_amd.init_rsa(20000, 2.0, -1.0, (p.lx, p.ly, p.lz), 6)
+    assert pos.shape[0] == 20000
+    f = sim.contact_forces(pos, t=7)
+    rc, fr, _ = oracle.contact_forces(p, pos, t=7)
+    assert rc == 0
+    assert np.abs(fr).sum() > 0
+    assert np.array_equal(f, fr)
+    sim.close()
+
+
 def test_coincident_pair_jitter(gpu):
     p = config_params(32, 3, 2.0, 100)
     sim = fipi_amd.Simulation(p)
