@@ -62,3 +62,28 @@ def test_reference_params_match_head_hpp():
     assert p.r_eq == p.r_c == 5 * p.r_p and p.k_n == 0.5 * 0.07
     n, d = p.cell_grid()
     assert n == [16, 16, 16]
+
+
+def test_tuning_struct_round_trip():
+    # fipi_tuning_t: four int32 (tile bytes of the y passes, CH hub, Stokes
+    # hub, and the y-pass values per thread)
+    assert C.sizeof(fipi_amd.CTuning) == 4 * 4
+    d = {"y_tile_bytes": 64, "hub_tile_bytes": 128, "stokes_tile_bytes": 64, "y_values_per_thread": 16}
+    assert fipi_amd.CTuning.from_dict(d).as_dict() == d
+    assert fipi_amd.CTuning.from_dict({}).as_dict() == {k: 0 for k in d}
+
+
+def test_tuning_calls_reject_null_context():
+    L = fipi_amd.lib()
+    t = fipi_amd.CTuning()
+    assert L.fipi_get_tuning(None, C.byref(t)) == -1
+    assert L.fipi_set_tuning(None, C.byref(t)) == -1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
