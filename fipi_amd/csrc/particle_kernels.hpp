@@ -160,30 +160,35 @@ __global__ __launch_bounds__(256) void k_scan_finish(int *__restrict__ start, in
 }
 
 // Block partial of argmax y (strictly greater, first index; y_max starts at
-// 0.0, cluster.cpp:128-138) over the particles of this block (256 threads).
+// 0.0, cluster.cpp:128-138) over the particles of this block (256 threads):
+// a butterfly over each wave, then the four wave results in LDS.  The
+// comparison is a total order on (y, -index) with -1 for "none", so the
+// result does not depend on the order of the combines.
+__device__ __forceinline__ void argmax_take(double &by, int &bi, double oy, int oi)
+{
+    if (oi >= 0 && (bi < 0 || oy > by || (oy == by && oi < bi))) {
+        by = oy;
+        bi = oi;
+    }
+}
+
 __device__ __forceinline__ void block_argmax_y(double y, int idx, double *ymax, int *yidx)
 {
-    __shared__ double sy[256];
-    __shared__ int si[256];
-    sy[threadIdx.x] = y > 0.0 ? y : 0.0;
-    si[threadIdx.x] = y > 0.0 ? idx : -1;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            const double oy = sy[threadIdx.x + o];
-            const int oi = si[threadIdx.x + o];
-            const int bi = si[threadIdx.x];
-            const double by = sy[threadIdx.x];
-            if (oi >= 0 && (bi < 0 || oy > by || (oy == by && oi < bi))) {
-                sy[threadIdx.x] = oy;
-                si[threadIdx.x] = oi;
-            }
-        }
-        __syncthreads();
+    __shared__ double sy[4];
+    __shared__ int si[4];
+    double by = y > 0.0 ? y : 0.0;
+    int bi = y > 0.0 ? idx : -1;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) argmax_take(by, bi, __shfl_xor(by, o), __shfl_xor(bi, o));
+    if ((threadIdx.x & 63) == 0) {
+        sy[threadIdx.x >> 6] = by;
+        si[threadIdx.x >> 6] = bi;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        ymax[blockIdx.x] = sy[0];
-        yidx[blockIdx.x] = si[0];
+        for (int w = 1; w < 4; w++) argmax_take(by, bi, sy[w], si[w]);
+        ymax[blockIdx.x] = by;
+        yidx[blockIdx.x] = bi;
     }
 }
 
