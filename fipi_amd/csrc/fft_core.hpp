@@ -189,7 +189,8 @@ template <int R, int SIGN, typename C> __device__ __forceinline__ void dft(C *v)
 // put their S = 128-byte-wide columns side by side, so a wave's accesses
 // are runs of S consecutive values).
 //
-// Plans (E.M.E): 256 = 8.4.8 or 16.16, 128 = 8.2.8 or 4.8.4, 96 = 4.6.4,
+// Plans (E.M.E): 256 = 8.4.8, 16.16 or 4.4.4.4 (two middle stages),
+// 128 = 8.2.8 or 4.8.4, 96 = 4.6.4,
 // 80 = 4.5.4, 64 = 8.8 or 4.4.4, 48 = 4.3.4, 32 = 4.2.4, 16 = 4.4.  When M does not
 // divide E the N/M butterflies of the middle stage are spread over the P
 // threads unevenly (some threads take one more).  Lines of 256 use E = 8
@@ -209,6 +210,10 @@ template <> struct PlanE<128, 8> { static constexpr int M = 2; };
 template <> struct PlanE<128, 4> { static constexpr int M = 8; };
 template <> struct PlanE<256, 8> { static constexpr int M = 4; };
 template <> struct PlanE<256, 16> { static constexpr int M = 0; };
+template <> struct PlanE<256, 4> { static constexpr int M = 4; };
+// radix of a second middle stage (0: none): 256 = 4.4.4.4
+template <int N, int E> struct PlanM2 { static constexpr int v = 0; };
+template <> struct PlanM2<256, 4> { static constexpr int v = 4; };
 
 #ifndef FIPI_E256_F64
 #define FIPI_E256_F64 8
@@ -227,6 +232,7 @@ template <> struct PlanE_for<float, 256> { static constexpr int E = FIPI_E256_F3
 template <typename T, int N, int EV = PlanE_for<T, N>::E> struct Line {
     static constexpr int E = EV;                   // values per thread
     static constexpr int M = PlanE<N, E>::M;       // middle radix (0: none)
+    static constexpr int M2 = PlanM2<N, E>::v;     // second middle radix (0: none)
     static constexpr int P = N / E;                // threads per line
     static constexpr int LL = N + N / E;           // padded LDS slots per line
 };
@@ -242,10 +248,12 @@ __device__ __forceinline__ C tw_at(const C *__restrict__ tw, int idx)
     return w;
 }
 
-template <typename T, int N, int SIGN, int S, int EV>
+// middle Stockham stage of radix RM and span NS (the product of the
+// radices before it)
+template <typename T, int N, int SIGN, int S, int EV, int RM = Line<T, N, EV>::M, int NS = EV>
 __device__ __forceinline__ void mid_stage(cplx<T> *lds, int t, const cplx<T> *__restrict__ tw)
 {
-    constexpr int E = EV, P = Line<T, N, EV>::P, RM = Line<T, N, EV>::M, NS = E;
+    constexpr int E = EV, P = Line<T, N, EV>::P;
     constexpr int B = N / RM;               // butterflies of the stage
     constexpr int NB = (B + P - 1) / P;     // per thread (the last round may be partial)
     using C = cplx<T>;
@@ -288,6 +296,8 @@ __device__ __forceinline__ void fft_dist(cplx<T> *v, cplx<T> *lds, int t, const 
     for (int q = 0; q < E; q++) lds[lpad<E>(t * E + q) * S] = v[q];
     __syncthreads();
     if constexpr (Line<T, N, EV>::M > 0) mid_stage<T, N, SIGN, S, EV>(lds, t, tw);
+    if constexpr (Line<T, N, EV>::M2 > 0)
+        mid_stage<T, N, SIGN, S, EV, Line<T, N, EV>::M2, EV * Line<T, N, EV>::M>(lds, t, tw);
 #pragma unroll
     for (int q = 0; q < E; q++) v[q] = lds[lpad<E>(t + q * P) * S];
 #pragma unroll

@@ -48,8 +48,15 @@ template <typename T> struct XT_of {
 // x-pass workgroup: up to LPW complex lines (2 real lines each) of P
 // threads; a launch uses blockDim.x / P lines, a divisor of ny*nz/2, so
 // every line of every workgroup exists
-template <typename T, int N> struct XShape {
-    static constexpr int LPW = XT_of<T>::v / Line<T, N>::P;
+template <typename T, int N, int EV = PlanE_for<T, N>::E> struct XShape {
+    static constexpr int LPW = XT_of<T>::v / Line<T, N, EV>::P;
+};
+
+// Values per thread of the velocity/convection x pass: FP64 256-point rows
+// use the 4.4.4.4 plan there (measured faster than 8.4.8 for that pass and
+// slower for the force pass and the Stokes hub, which keep the default).
+template <typename T, int N> struct XE_conv {
+    static constexpr int v = (sizeof(T) == 8 && N == 256) ? 4 : PlanE_for<T, N>::E;
 };
 
 // kx columns per y/z-pass workgroup: a TB-byte run of complex values
@@ -73,11 +80,11 @@ __device__ __forceinline__ size_t hidx(const Dims &d, int kx, int y, int z)
 // The imaginary parts of the kx = 0 and kx = NX/2 columns are dropped, i.e.
 // the line is the Hermitian part, which is what the reference keeps when it
 // reads only the real part after an inverse complex transform.
-template <typename T, int N, bool DERIV>
+template <typename T, int N, bool DERIV, int EV = PlanE_for<T, N>::E>
 __device__ __forceinline__ void load_c2r(cplx<T> *v, const cplx<T> *__restrict__ in, size_t rowA, size_t rowB,
                                          int t, const double *__restrict__ kxv)
 {
-    constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
+    constexpr int E = EV, P = Line<T, N, EV>::P;
     using C = cplx<T>;
 #pragma unroll
     for (int q = 0; q < E; q++) {
@@ -98,11 +105,11 @@ __device__ __forceinline__ void load_c2r(cplx<T> *v, const cplx<T> *__restrict__
 // After a forward transform of the packed line (Z in D), write Z to the
 // line buffer in natural order, split into the two half spectra and store
 // rows rowA, rowB.
-template <typename T, int N>
+template <typename T, int N, int EV = PlanE_for<T, N>::E>
 __device__ __forceinline__ void store_r2c(const cplx<T> *v, cplx<T> *L, cplx<T> *__restrict__ out, size_t rowA,
                                           size_t rowB, int t)
 {
-    constexpr int E = Line<T, N>::E, P = Line<T, N>::P;
+    constexpr int E = EV, P = Line<T, N, EV>::P;
     using C = cplx<T>;
 #pragma unroll
     for (int q = 0; q < E; q++) L[t + q * P] = v[q];
@@ -119,9 +126,9 @@ __device__ __forceinline__ void store_r2c(const cplx<T> *v, cplx<T> *L, cplx<T> 
 
 // common x-pass geometry: complex line lp of the workgroup = real lines
 // rl, rl+1 of box blockIdx.z
-#define FIPI_XPASS_SETUP(T, N)                                               \
-    constexpr int E = Line<T, N>::E, P = Line<T, N>::P, LL = Line<T, N>::LL; \
-    constexpr int LPW = XShape<T, N>::LPW;                                   \
+#define FIPI_XPASS_SETUP_E(T, N, EV)                                                     \
+    constexpr int E = Line<T, N, EV>::E, P = Line<T, N, EV>::P, LL = Line<T, N, EV>::LL; \
+    constexpr int LPW = XShape<T, N, EV>::LPW;                                           \
     using C = cplx<T>;                                                       \
     __shared__ C lds[LPW * LL];                                              \
     const int lp = threadIdx.x / P, t = threadIdx.x % P;                     \
@@ -130,6 +137,7 @@ __device__ __forceinline__ void store_r2c(const cplx<T> *v, cplx<T> *L, cplx<T> 
     C *L = lds + lp * LL;                                                    \
     (void)E;                                                                 \
     (void)zb;
+#define FIPI_XPASS_SETUP(T, N) FIPI_XPASS_SETUP_E(T, N, (PlanE_for<T, N>::E))
 
 // pointer of box zb in a slab with per-box stride st (null stays null)
 template <typename P_>
@@ -277,7 +285,7 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inverse(const cplx<T> *__rest
 // ProdConv) and its forward x transform into h0 (X), in place of u_hat_x.
 // g holds grad c^{n+1}, the gradient the next step's convection uses.
 // part (optional): FP64 block sums of conv, as k_x_forward.
-template <typename T, int N>
+template <typename T, int N, int EV = XE_conv<T, N>::v>
 __global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *h1, const cplx<T> *h2,
                                                    T *__restrict__ u0,
                                                    T *__restrict__ u1, T *__restrict__ u2,
@@ -285,7 +293,7 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const c
                                                    const T *__restrict__ g2, Dims d, const cplx<T> *__restrict__ tw,
                                                    double *__restrict__ part)
 {
-    FIPI_XPASS_SETUP(T, N)
+    FIPI_XPASS_SETUP_E(T, N, EV)
     const size_t rA = zb * d.hs + (size_t)rl * d.nxp, rB = rA + d.nxp;
     const size_t xA = zb * d.rs + (size_t)rl * d.nx, xB = xA + d.nx;
     // FP32: all three components in flight from the start; FP64: the next
@@ -294,7 +302,7 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const c
     constexpr int NV = sizeof(T) == 4 ? 3 : 2;
     C v[NV][E];
 #pragma unroll
-    for (int f = 0; f < NV; f++) load_c2r<T, N, false>(v[f], f == 0 ? (const C *)h0 : (f == 1 ? h1 : h2), rA, rB, t, nullptr);
+    for (int f = 0; f < NV; f++) load_c2r<T, N, false, EV>(v[f], f == 0 ? (const C *)h0 : (f == 1 ? h1 : h2), rA, rB, t, nullptr);
     C cv[E];
 #pragma unroll
     for (int q = 0; q < E; q++) cv[q] = C{T(0), T(0)};
@@ -303,7 +311,7 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const c
         T *u = f == 0 ? u0 : (f == 1 ? u1 : u2);
         const T *g = f == 0 ? g0 : (f == 1 ? g1 : g2);
         C *w = v[NV == 3 ? f : (f & 1)];
-        if (NV == 2 && f == 1) load_c2r<T, N, false>(v[0], h2, rA, rB, t, nullptr);  // component 2 in flight
+        if (NV == 2 && f == 1) load_c2r<T, N, false, EV>(v[0], h2, rA, rB, t, nullptr);  // component 2 in flight
         T ga[E], gb[E];
 #pragma unroll
         for (int q = 0; q < E; q++) {
@@ -311,7 +319,7 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const c
             ga[q] = g[xA + x];
             gb[q] = g[xB + x];
         }
-        fft_dist<T, N, +1>(w, L, t, tw);
+        fft_dist<T, N, +1, 1, EV>(w, L, t, tw);
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int x = t + q * P;
@@ -328,8 +336,8 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const c
         if (threadIdx.x == 0) part[zb * gridDim.x + blockIdx.x] = s;
         __syncthreads();
     }
-    fft_dist<T, N, -1>(cv, L, t, tw);
-    store_r2c<T, N>(cv, L, h0, rA, rB, t);
+    fft_dist<T, N, -1, 1, EV>(cv, L, t, tw);
+    store_r2c<T, N, EV>(cv, L, h0, rA, rB, t);
 }
 
 // P5: inverse x of (c_hat, dy c_hat, dz c_hat) -> c, grad c (real), then

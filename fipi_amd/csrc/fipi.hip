@@ -230,25 +230,25 @@ inline dim3 grid1(size_t n, int b) { return dim3((unsigned)(n == 0 ? 1 : (n + b 
 
 // x-pass lines per workgroup: the largest divisor of ny*nz/2 (complex
 // lines, 2 real lines each) that fits in the x-pass workgroup (XT_of)
-template <typename T, int N>
+template <typename T, int N, int EV = PlanE_for<T, N>::E>
 int xlines(const Dims &d)
 {
     const int cl = d.ny * d.nz / 2;
-    int l = XShape<T, N>::LPW;
+    int l = XShape<T, N, EV>::LPW;
     while (cl % l != 0) l--;
     return l;
 }
 
 // x-pass grid (one slab of workgroups per box) and workgroup
-template <typename T, int N>
+template <typename T, int N, int EV = PlanE_for<T, N>::E>
 dim3 xgrid(const Dims &d, int nbox)
 {
-    return dim3((unsigned)(d.ny * d.nz / 2 / xlines<T, N>(d)), 1, (unsigned)nbox);
+    return dim3((unsigned)(d.ny * d.nz / 2 / xlines<T, N, EV>(d)), 1, (unsigned)nbox);
 }
-template <typename T, int N>
+template <typename T, int N, int EV = PlanE_for<T, N>::E>
 dim3 xblock(const Dims &d)
 {
-    return dim3((unsigned)(xlines<T, N>(d) * Line<T, N>::P));
+    return dim3((unsigned)(xlines<T, N, EV>(d) * Line<T, N, EV>::P));
 }
 
 template <typename T>
@@ -669,7 +669,8 @@ struct Impl {
                 ProfScope ps_(ctx, "x_inv_u_conv", 4 * Hb + 6 * Rb);
                 with_n(d.nx, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
-                    hipLaunchKernelGGL((k_x_inv_conv<T, N>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, st, h0, (const C *)h1,
+                    constexpr int XE = XE_conv<T, N>::v;
+                    hipLaunchKernelGGL((k_x_inv_conv<T, N, XE>), (xgrid<T, N, XE>(d, ctx->nbox)), (xblock<T, N, XE>(d)), 0, st, h0, (const C *)h1,
                                        (const C *)h2, R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]),
                                        (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]), d,
                                        (const C *)ctx->twx, mixed ? ctx->conv_part : nullptr);
