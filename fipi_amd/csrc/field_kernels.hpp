@@ -55,9 +55,12 @@ template <typename T, int N, int EV = PlanE_for<T, N>::E> struct XShape {
 // Values per thread of the velocity/convection x pass: FP64 256-point rows
 // use the 4.4.4.4 plan there (measured faster than 8.4.8 for that pass and
 // slower for the force pass and the Stokes hub, which keep the default).
+// FP32 keeps the default: its per-workgroup partial sums of conv (the
+// FP64 k = 0 mode) are laid out by the default x-pass grid.
 template <typename T, int N> struct XE_conv {
     static constexpr int v = (sizeof(T) == 8 && N == 256) ? 4 : PlanE_for<T, N>::E;
 };
+static_assert(XE_conv<float, 256>::v == PlanE_for<float, 256>::E, "FP32 conv partials use the default grid");
 
 // kx columns per y/z-pass workgroup: a TB-byte run of complex values
 // (TB = 64 or 128, chosen per launch site and edge length)
