@@ -574,7 +574,9 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
 // picks between the two.
 template <typename T, int N> struct YE {
     static constexpr int v = (sizeof(T) == 4 && N == 256) ? 16 : PlanE_for<T, N>::E;
-    static constexpr int alt = N == 256 ? (v == 16 ? 8 : 16) : ((N == 64 || N == 128) ? 4 : 0);
+    // FP64 256: 4.4.4.4 (16.16 would not fit the registers of two waves per SIMD)
+    static constexpr int alt =
+        N == 256 ? (v == 16 ? 8 : (sizeof(T) == 8 ? 4 : 16)) : ((N == 64 || N == 128) ? 4 : 0);
 };
 
 // Element offsets within one box (the box offset is applied to the

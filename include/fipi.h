@@ -185,7 +185,8 @@ int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size);
  * once with FFTW_ESTIMATE).  The launch choices of a context: tile widths
  * (64 or 128 bytes) of the y passes, the CH hub and the Stokes hub, and the
  * FFT plan of the y passes as values per thread (first/last radix: 256
- * points 8.4.8 or 16.16, 128 points 8.2.8 or 4.8.4, 64 points 8.8 or 4.4.4;
+ * points 8.4.8 or 4.4.4.4 in FP64, 16.16 or 8.4.8 in FP32, 128 points 8.2.8
+ * or 4.8.4, 64 points 8.8 or 4.4.4;
  * other sizes have one plan; 0 in fipi_set_tuning = the default).  Tile
  * widths do not change results; the plan changes rounding only.
  * fipi_tune times `steps` steps with every combination, keeps the fastest

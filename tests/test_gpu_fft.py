@@ -59,15 +59,15 @@ def test_single_modes_fp64(gpu):
 @pytest.mark.parametrize("prec", [FP64, FP32_MIXED])
 def test_y_plans_vs_numpy(gpu, ny, prec):
     """Both FFT plans of the y passes that a tuner may pick (256: 8.4.8 and
-    16.16; 128: 8.2.8 and 4.8.4; 64: 8.8 and 4.4.4) against numpy, on a
-    grid that is long in y."""
+    4.4.4.4 in FP64, 16.16 and 8.4.8 in FP32; 128: 8.2.8 and 4.8.4; 64: 8.8
+    and 4.4.4) against numpy, on a grid that is long in y."""
     import torch
     print(f"y plans ny={ny} prec={prec}", flush=True)
     nx, nz = 32, 16
     p = config_params(nx, 0, 2.0, 10, precision=prec).replace(ny=ny, nz=nz, ly=float(ny), lz=float(nz))
     sim = fipi_amd.Simulation(p)
     default = sim.tuning()["y_values_per_thread"]
-    plans = {8, 16} if ny == 256 else {8, 4}
+    plans = ({8, 4} if prec == FP64 else {8, 16}) if ny == 256 else {8, 4}
     assert default in plans
     rng = np.random.default_rng(ny + prec)
     x = rng.standard_normal((nz, ny, nx))
