@@ -216,18 +216,9 @@ template <> struct PlanE<256, 4> { static constexpr int M = 4; };
 template <int N, int E> struct PlanM2 { static constexpr int v = 0; };
 template <> struct PlanM2<256, 4> { static constexpr int v = 4; };
 
-#ifndef FIPI_E256_F64
-#define FIPI_E256_F64 8
-#endif
-#ifndef FIPI_E256_F32
-#define FIPI_E256_F32 8
-#endif
-
 template <typename T, int N> struct PlanE_for {
-    static constexpr int E = (N == 64 || N == 128) ? 8 : 4;
+    static constexpr int E = (N == 64 || N == 128 || N == 256) ? 8 : 4;
 };
-template <> struct PlanE_for<double, 256> { static constexpr int E = FIPI_E256_F64; };
-template <> struct PlanE_for<float, 256> { static constexpr int E = FIPI_E256_F32; };
 
 // EV: values per thread of a kernel that does not use the default
 template <typename T, int N, int EV = PlanE_for<T, N>::E> struct Line {
@@ -242,12 +233,9 @@ template <typename T, int N, int EV = PlanE_for<T, N>::E> struct Line {
 // passes) pad one slot per 16 instead of per 4: fewer bank conflicts in the
 // 4.4.4.4 plan's exchanges (measured: the FP64 convection pass 0.339 ->
 // 0.317 ms).  Slots stay below LL = N + N/E either way.
-#ifndef FIPI_PAD4
-#define FIPI_PAD4 16
-#endif
 template <int E, int S = 1> __device__ __forceinline__ int lpad(int i)
 {
-    return i + i / ((S == 1 && E == 4) ? FIPI_PAD4 : E);
+    return i + i / ((S == 1 && E == 4) ? 16 : E);
 }
 
 // W_N^(SIGN idx) from the forward table (conjugated for SIGN > 0)

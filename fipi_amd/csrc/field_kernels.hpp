@@ -574,9 +574,11 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
 // picks between the two.
 template <typename T, int N> struct YE {
     static constexpr int v = (sizeof(T) == 4 && N == 256) ? 16 : PlanE_for<T, N>::E;
-    // FP64 256: 4.4.4.4 (16.16 would not fit the registers of two waves per SIMD)
+    // FP64 256: 4.4.4.4 (16.16 would not fit the registers of two waves per
+    // SIMD); FP32 256: 8.4.8; 64 and 128: the 4-value plans
     static constexpr int alt =
-        N == 256 ? (v == 16 ? 8 : (sizeof(T) == 8 ? 4 : 16)) : ((N == 64 || N == 128) ? 4 : 0);
+        N == 256 ? (v == 16 ? 8 : (v == 8 ? 4 : 8)) : ((N == 64 || N == 128) ? (v == 4 ? 8 : 4) : 0);
+    static_assert(alt != v, "the alternative y plan must differ from the default");
 };
 
 // Element offsets within one box (the box offset is applied to the

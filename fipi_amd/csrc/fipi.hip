@@ -555,7 +555,10 @@ struct Impl {
         const bool ns = ctx->p.switch_ns != 0;
         const bool parts = ctx->n > 0;
         C *h0 = H(ctx->h[0]), *h1 = H(ctx->h[1]), *h2 = H(ctx->h[2]), *h3 = H(ctx->h[3]), *fk = H(ctx->fk);
-        const double Rb = (double)ctx->G * sizeof(T), Hb = (double)ctx->GH * 2 * sizeof(T);
+        // algorithmic bytes of one real field and one half spectrum (nx/2+1
+        // columns; the padding of the row pitch is not counted)
+        const double Rb = (double)ctx->G * ctx->nbox * sizeof(T);
+        const double Hb = (double)d.nxh * d.ny * d.nz * ctx->nbox * 2 * sizeof(T);
         const bool mixed = !std::is_same<T, double>::value;
         if (ns) {
             if (!ctx->conv_ready) {
@@ -813,6 +816,37 @@ int check_params(const fipi_params *p)
     return FIPI_OK;
 }
 
+// Particle-kernel parameters of one box of p (a slab of nbox such boxes)
+// with n particles in all; the contact grid is cluster.cpp:26-32.
+PartParams part_params_of(const fipi_params &p, int n, int nbox)
+{
+    const double L[3] = {p.lx, p.ly, p.lz};
+    int nc[3];
+    double dc[3];
+    for (int a = 0; a < 3; a++) {
+        nc[a] = (int)floor(L[a] / p.r_c + 2 * p.margin / p.r_c);
+        dc[a] = fmod(L[a] + 2 * p.margin, p.r_c) / nc[a] + p.r_c;
+    }
+    PartParams pp{};
+    pp.n = n;
+    pp.nx = p.nx; pp.ny = p.ny; pp.nz = p.nz;
+    pp.dx = p.lx / p.nx; pp.dy = p.ly / p.ny; pp.dz = p.lz / p.nz;
+    pp.lx = p.lx; pp.ly = p.ly; pp.lz = p.lz;
+    pp.ncx = nc[0]; pp.ncy = nc[1]; pp.ncz = nc[2];
+    pp.dcx = dc[0]; pp.dcy = dc[1]; pp.dcz = dc[2];
+    pp.margin = p.margin;
+    pp.r_p = p.r_p; pp.r_eq = p.r_eq; pp.k_n = p.k_n;
+    pp.pi = p.pi; pp.mu = p.mu; pp.dt = p.dt; pp.epsilon = p.epsilon; pp.sigma = p.sigma; pp.a_pi = p.a_pi;
+    pp.tanh2 = p.tanh2;
+    pp.sqrt2 = std::pow(2.0, 0.5);
+    pp.has_ctheta = 0;
+    pp.nbox = nbox;
+    pp.ncell_box = (nc[0] > 0 && nc[1] > 0 && nc[2] > 0) ? nc[0] * nc[1] * nc[2] : 1;
+    pp.nlines_box = p.ny * p.nz;
+    pp.rs = (size_t)p.nx * p.ny * p.nz;
+    return pp;
+}
+
 template <typename T>
 int setup_buffers(fipi_ctx *ctx)
 {
@@ -863,31 +897,8 @@ int setup_buffers(fipi_ctx *ctx)
     HIPCHK(ctx, hipMemcpy(ctx->twz, tz.data(), sizeof(T) * 2 * p.nz, hipMemcpyHostToDevice));
     HIPCHK(ctx, hipMemset(ctx->d_flags, 0, sizeof(int) * 4));
     HIPCHK(ctx, hipHostMalloc((void **)&ctx->h_flags, sizeof(int) * 4, hipHostMallocDefault));
-    // contact grid (cluster.cpp:26-32)
-    const double L[3] = {p.lx, p.ly, p.lz};
-    int nc[3];
-    double dc[3];
-    for (int a = 0; a < 3; a++) {
-        nc[a] = (int)floor(L[a] / p.r_c + 2 * p.margin / p.r_c);
-        dc[a] = fmod(L[a] + 2 * p.margin, p.r_c) / nc[a] + p.r_c;
-    }
     PartParams &pp = ctx->pp;
-    pp.n = ctx->n;
-    pp.nx = p.nx; pp.ny = p.ny; pp.nz = p.nz;
-    pp.dx = p.lx / p.nx; pp.dy = p.ly / p.ny; pp.dz = p.lz / p.nz;
-    pp.lx = p.lx; pp.ly = p.ly; pp.lz = p.lz;
-    pp.ncx = nc[0]; pp.ncy = nc[1]; pp.ncz = nc[2];
-    pp.dcx = dc[0]; pp.dcy = dc[1]; pp.dcz = dc[2];
-    pp.margin = p.margin;
-    pp.r_p = p.r_p; pp.r_eq = p.r_eq; pp.k_n = p.k_n;
-    pp.pi = p.pi; pp.mu = p.mu; pp.dt = p.dt; pp.epsilon = p.epsilon; pp.sigma = p.sigma; pp.a_pi = p.a_pi;
-    pp.tanh2 = p.tanh2;
-    pp.sqrt2 = std::pow(2.0, 0.5);
-    pp.has_ctheta = 0;
-    pp.nbox = ctx->nbox;
-    pp.ncell_box = (nc[0] > 0 && nc[1] > 0 && nc[2] > 0) ? nc[0] * nc[1] * nc[2] : 1;
-    pp.nlines_box = p.ny * p.nz;
-    pp.rs = ctx->G;
+    pp = part_params_of(p, ctx->n, ctx->nbox);
     ctx->ncell = pp.ncell_box * ctx->nbox;
     ctx->nlines = pp.nlines_box * ctx->nbox;
     ALLOC(ctx->cell_cnt, sizeof(int) * (ctx->ncell + 1));
@@ -1613,7 +1624,9 @@ constexpr int32_t SNAP_VERSION = 1;
 
 struct SnapHeader {
     char magic[8];
-    int32_t version, nx, ny, nz, n, nbox, precision, t, cur, conv_ready, has_ctheta, reserved;
+    // y_e: the y-pass FFT plan in use (values per thread, 0 = the default);
+    // it changes rounding, so a restore takes it over
+    int32_t version, nx, ny, nz, n, nbox, precision, t, cur, conv_ready, has_ctheta, y_e;
     int64_t bytes;
 };
 
@@ -1682,6 +1695,7 @@ int fipi_snapshot_save(fipi_ctx *ctx, void *buf, int64_t size)
     h.cur = ctx->cur;
     h.conv_ready = ctx->conv_ready ? 1 : 0;
     h.has_ctheta = ctx->pp.has_ctheta;
+    h.y_e = ctx->tune.y_e;
     h.bytes = need;
     std::memcpy(buf, &h, sizeof(h));
     char *o = (char *)buf + sizeof(h);
@@ -1705,6 +1719,11 @@ int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size)
         h.precision != ctx->p.precision)
         return set_err(ctx, FIPI_ERR_ARG, "snapshot of a different grid, particle count or precision");
     if (h.bytes != fipi_snapshot_size(ctx) || size < h.bytes) return set_err(ctx, FIPI_ERR_ARG, "snapshot size mismatch");
+    {
+        int def, alt;
+        y_plans_of(ctx, &def, &alt);
+        if (h.y_e != 0 && h.y_e != alt) return set_err(ctx, FIPI_ERR_ARG, "snapshot names an unknown y-pass plan");
+    }
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     const char *in = (const char *)buf + sizeof(h);
     for (auto &x : snapshot_buffers(ctx)) {
@@ -1717,6 +1736,7 @@ int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size)
     ctx->cur = h.cur;
     ctx->conv_ready = h.conv_ready != 0;
     ctx->pp.has_ctheta = h.has_ctheta;
+    ctx->tune.y_e = h.y_e;
     ctx->has_state = true;
     return FIPI_OK;
 }
@@ -1810,3 +1830,4 @@ int fipi_set_tuning(fipi_ctx *ctx, const fipi_tuning_t *in)
 }
 
 }  // extern "C"
+

@@ -171,10 +171,11 @@ int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32
 /* ---- snapshot / restart ------------------------------------------------ */
 /* The whole state of a context (fields, the SBDF2 history and ring index,
  * the FP64 mass mode, particles, step counters) as one host buffer: a
- * header (magic "FIPISNAP", version, grid, particle count, precision, t)
- * followed by the raw device arrays.  fipi_snapshot_load restores it into a
- * context created with the same parameters; stepping then continues bit
- * for bit.  The reference writes ASCII VTK output without the history
+ * header (magic "FIPISNAP", version, grid, particle count, precision, t,
+ * the y-pass FFT plan of fipi_set_tuning) followed by the raw device
+ * arrays.  fipi_snapshot_load restores it into a context created with the
+ * same parameters and takes over the plan (tile widths are kept; they do
+ * not change results); stepping then continues bit for bit.  The reference writes ASCII VTK output without the history
  * (field.cpp:469-518, cluster.cpp:415-456), which cannot restart a run. */
 int64_t fipi_snapshot_size(fipi_ctx *ctx);
 int fipi_snapshot_save(fipi_ctx *ctx, void *buf, int64_t size);

@@ -66,6 +66,8 @@ def lib():
         ip = C.POINTER(C.c_int)
         L.oracle_fft3d.argtypes = [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int]
         L.oracle_fft3d.restype = C.c_int
+        L.oracle_set_fft_order.argtypes = [C.c_int]
+        L.oracle_set_fft_order.restype = None
         L.oracle_cell_grid.argtypes = [C.POINTER(OracleParams), ip, dp]
         L.oracle_cells_build.argtypes = [C.POINTER(OracleParams), C.c_int, dp, ip, ip, ip]
         L.oracle_cells_build.restype = C.c_int
@@ -136,6 +138,12 @@ def fft3d(data: np.ndarray, sign: int, n_threads: int = 0) -> np.ndarray:
     rc = lib().oracle_fft3d(nx, ny, nz, _dp(buf), int(sign), int(n_threads))
     assert rc == 0
     return a
+
+
+def set_fft_order(order: int) -> None:
+    """Factor order of the oracle's FFTs made from now on (0: radix 4 then
+    2, the default; 1: radix 2 only).  Round-off differs, the DFT not."""
+    lib().oracle_set_fft_order(int(order))
 
 
 def cell_grid(p):

@@ -37,12 +37,21 @@ typedef struct {
     double *tw;     /* tw[2t], tw[2t+1] = cos, sin (2*pi*t/n), t < n */
 } fft_plan1d;
 
+/* Factor order of every plan made after the call: 0 (default) radix-4
+ * stages then radix 2, 1 radix-2 stages only.  The two orders round
+ * differently (~1e-16 per transform); the long-horizon tests use the pair
+ * to show how far two correct FP64 implementations of the step agree. */
+static int g_fft_order = 0;
+
+void oracle_set_fft_order(int order) { g_fft_order = order; }
+
 static void plan1d_init(fft_plan1d *pl, int n)
 {
     pl->n = n;
     pl->nf = 0;
     int m = n;
-    while (m % 4 == 0) { pl->fac[pl->nf++] = 4; m /= 4; }
+    if (g_fft_order == 0)
+        while (m % 4 == 0) { pl->fac[pl->nf++] = 4; m /= 4; }
     while (m % 2 == 0) { pl->fac[pl->nf++] = 2; m /= 2; }
     for (int r = 3; m > 1; r += 2)
         while (m % r == 0) { pl->fac[pl->nf++] = r; m /= r; }

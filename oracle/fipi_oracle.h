@@ -74,6 +74,10 @@ typedef struct oracle_params {
  * (FFTW's convention).  Sizes with prime factors 2, 3, 5 (and any other
  * prime through a generic odd-radix butterfly). */
 int oracle_fft3d(int nx, int ny, int nz, double *data, int sign, int n_threads);
+/* Factor order of the transforms made after the call (process-wide):
+ * 0 = radix 4 then 2 (default), 1 = radix 2 only.  Both compute the DFT;
+ * they differ in round-off only. */
+void oracle_set_fft_order(int order);
 
 /* ---- cell grid, cluster.cpp:21-40 ------------------------------------- */
 void oracle_cell_grid(const oracle_params *p, int *ncell3, double *dcell3);

@@ -116,4 +116,16 @@ def test_torch_autograd_harness(gpu):
     assert float(loss) == l0
     assert float(m.grad) == gm and float(lam.grad) == gl
     assert np.array_equal(th.grad.numpy(), gth)
+    # parameters on the GPU, theta in float32: gradients come back on the
+    # inputs' devices and dtypes
+    m = torch.tensor(1.0, dtype=torch.float64, device="cuda", requires_grad=True)
+    lam = torch.tensor(1.0, dtype=torch.float64, device="cuda", requires_grad=True)
+    th = torch.tensor(th0, dtype=torch.float32, device="cuda", requires_grad=True)
+    loss = calibration_loss(m, lam, th, adj)
+    loss.backward()
+    assert loss.device.type == "cuda"
+    assert m.grad.device.type == "cuda" and th.grad.device.type == "cuda" and th.grad.dtype == torch.float32
+    l1, gm1, gl1, gth1 = adj.gradient(1.0, 1.0, th.detach().cpu().double().numpy())
+    assert float(m.grad) == gm1 and float(lam.grad) == gl1
+    assert np.allclose(th.grad.cpu().double().numpy(), gth1, rtol=1e-6, atol=1e-6 * np.abs(gth1).max())
     adj.close()
