@@ -34,7 +34,7 @@ ABI_SYMBOLS = [
     "fipi_set_state", "fipi_set_velocity", "fipi_get_phi", "fipi_get_velocity",
     "fipi_get_grad_phi", "fipi_get_particles", "fipi_get_time", "fipi_diagnostics",
     "fipi_step", "fipi_step_async", "fipi_sync", "fipi_profile_steps",
-    "fipi_fft_r2c", "fipi_fft_c2r", "fipi_cells_build", "fipi_contact_pairs",
+    "fipi_fft_r2c", "fipi_fft_c2r", "fipi_cells_build", "fipi_cells_build_noalloc", "fipi_contact_pairs",
     "fipi_contact_forces", "fipi_interp", "fipi_spread",
     "fipi_snapshot_size", "fipi_snapshot_save", "fipi_snapshot_load", "fipi_tune", "fipi_get_tuning",
     "fipi_set_tuning",
@@ -140,6 +140,7 @@ def lib():
     L.fipi_fft_r2c.argtypes = [vp, vp, vp]
     L.fipi_fft_c2r.argtypes = [vp, vp, vp]
     L.fipi_cells_build.argtypes = [vp, vp, C.c_int32, vp, vp, ip]
+    L.fipi_cells_build_noalloc.argtypes = [C.POINTER(CParams), vp, C.c_int32, vp, vp, vp, ip, vp]
     L.fipi_contact_pairs.argtypes = [vp, vp, C.c_int32, vp, i64, C.POINTER(i64)]
     L.fipi_contact_forces.argtypes = [vp, vp, C.c_int32, C.c_int32, vp]
     L.fipi_interp.argtypes = [vp, vp, vp, C.c_int32, vp]

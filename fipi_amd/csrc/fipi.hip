@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1831,3 +1832,150 @@ int fipi_set_tuning(fipi_ctx *ctx, const fipi_tuning_t *in)
 
 }  // extern "C"
 
+// ---------------------------------------------------------------------------
+// Cell list without allocation: fipi_cells_build_noalloc.  Every temporary
+// of the build (positions in SoA, per-particle cells, cell counts and
+// cursors, scan block totals, argmax-y partials, the error flag) lives in
+// one static device array, `scratch`, of NOALLOC_SCRATCH_BYTES per GPU; its
+// address on each device comes from hipGetSymbolAddress on the first call
+// made with that device current.  The call allocates nothing and never
+// synchronises: counters are cleared by a kernel at the start of each call
+// (no memset node, so the call may be captured in a HIP graph).  Calls on
+// one GPU share the array: two calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int NA_MAX_N = FIPI_NOALLOC_MAX_PARTICLES;
+constexpr int NA_MAX_CELLS = FIPI_NOALLOC_MAX_CELLS;
+constexpr size_t NA_BSUM = (size_t)(NA_MAX_N > NA_MAX_CELLS ? NA_MAX_N : NA_MAX_CELLS) / SCAN_ITEMS + 2;
+constexpr size_t NA_PART = (size_t)NA_MAX_N / 256 + 2;
+// layout in bytes, every region 256-byte aligned
+constexpr size_t na_up(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NA_OFF_P = 0;                                               // 3 x double[N]
+constexpr size_t NA_OFF_CELL = NA_OFF_P + 3 * na_up(sizeof(double) * NA_MAX_N);  // int[N]
+constexpr size_t NA_OFF_CNT = NA_OFF_CELL + na_up(sizeof(int) * NA_MAX_N);      // int[C+1]
+constexpr size_t NA_OFF_CUR = NA_OFF_CNT + na_up(sizeof(int) * (NA_MAX_CELLS + 1));  // int[C+1]
+constexpr size_t NA_OFF_BSUM = NA_OFF_CUR + na_up(sizeof(int) * (NA_MAX_CELLS + 1));
+constexpr size_t NA_OFF_YMAX = NA_OFF_BSUM + na_up(sizeof(int) * NA_BSUM);
+constexpr size_t NA_OFF_YIDX = NA_OFF_YMAX + na_up(sizeof(double) * NA_PART);
+constexpr size_t NA_OFF_MISC = NA_OFF_YIDX + na_up(sizeof(int) * NA_PART);  // mark, error flags
+constexpr size_t NOALLOC_SCRATCH_BYTES = NA_OFF_MISC + 256;
+static_assert(NOALLOC_SCRATCH_BYTES < (size_t)16 << 20, "the no-allocation scratch stays below 16 MiB");
+
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+constexpr int NA_MAX_DEVICES = 64;
+std::mutex g_na_mutex;
+char *g_na_ptr[NA_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (resolved once per
+// device); nullptr and a message on failure.
+char *na_scratch(int *dev_out, std::string *err)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        *err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+        return nullptr;
+    }
+    if (dev < 0 || dev >= NA_MAX_DEVICES) {
+        *err = "device index beyond the no-allocation table";
+        return nullptr;
+    }
+    *dev_out = dev;
+    std::lock_guard<std::mutex> lk(g_na_mutex);
+    if (!g_na_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return nullptr;
+        }
+        g_na_ptr[dev] = (char *)p;
+    }
+    return g_na_ptr[dev];
+}
+
+// clear n ints (a kernel, not a memset: safe under graph capture)
+__global__ void k_fill_int(int *a, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+// the argmax-y result (it_mark) of the shared cell kernels starts at -1
+__global__ void k_na_start(int *misc)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        misc[0] = -1;  // mark
+        misc[1] = 0;   // error bits
+    }
+}
+
+// error bits of this call to the caller's device int
+__global__ void k_na_status(const int *misc, int *d_err)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *d_err = misc[1];
+}
+
+}  // namespace
+
+extern "C" int fipi_cells_build_noalloc(const fipi_params *params, const double *d_pos, int32_t n,
+                                        int32_t *d_cell_start, int32_t *d_ids, int32_t *d_err,
+                                        int32_t *ncell_out, void *stream)
+{
+    if (!params || n < 0 || (n > 0 && (!d_pos || !d_ids)) || !d_cell_start || !d_err)
+        return set_err(nullptr, FIPI_ERR_ARG, "invalid argument");
+    fipi_params q = *params;
+    q.switch_particle = 1;
+    q.n_particles = n;
+    int rc = check_params(&q);
+    if (rc != FIPI_OK) return rc;
+    const PartParams base = part_params_of(q, n, 1);
+    const int ncell = base.ncell_box;
+    if (ncell_out) *ncell_out = ncell;
+    if (n > NA_MAX_N)
+        return set_err(nullptr, FIPI_ERR_ARG,
+                       "n exceeds FIPI_NOALLOC_MAX_PARTICLES (" + std::to_string(NA_MAX_N) + ")");
+    if (ncell > NA_MAX_CELLS)
+        return set_err(nullptr, FIPI_ERR_ARG,
+                       "the contact grid exceeds FIPI_NOALLOC_MAX_CELLS (" + std::to_string(NA_MAX_CELLS) + ")");
+    int dev = 0;
+    std::string msg;
+    char *S = na_scratch(&dev, &msg);
+    if (!S) return set_err(nullptr, FIPI_ERR_HIP, msg);
+    hipStream_t st = (hipStream_t)stream;
+    double *px = (double *)(S + NA_OFF_P);
+    double *py = (double *)(S + NA_OFF_P + na_up(sizeof(double) * NA_MAX_N));
+    double *pz = (double *)(S + NA_OFF_P + 2 * na_up(sizeof(double) * NA_MAX_N));
+    int *cell = (int *)(S + NA_OFF_CELL), *cnt = (int *)(S + NA_OFF_CNT), *cur = (int *)(S + NA_OFF_CUR);
+    int *bsum = (int *)(S + NA_OFF_BSUM), *yidx = (int *)(S + NA_OFF_YIDX), *misc = (int *)(S + NA_OFF_MISC);
+    double *ymax = (double *)(S + NA_OFF_YMAX);
+    // counts and cursors start at zero whatever an earlier call left
+    hipLaunchKernelGGL(k_fill_int, grid1(ncell + 1, 256), dim3(256), 0, st, cnt, ncell + 1, 0);
+    hipLaunchKernelGGL(k_fill_int, grid1(ncell + 1, 256), dim3(256), 0, st, cur, ncell + 1, 0);
+    hipLaunchKernelGGL(k_na_start, dim3(1), dim3(64), 0, st, misc);
+    if (n > 0) hipLaunchKernelGGL(k_aos_to_soa, grid1(n, 256), dim3(256), 0, st, d_pos, px, py, pz, n);
+    PartState ps{};
+    ps.p[0] = px;
+    ps.p[1] = py;
+    ps.p[2] = pz;
+    ps.cell = cell;
+    ps.err = misc + 1;
+    ps.mark = misc;
+    // the kernels of fipi_cells_build: histogram, scan, scatter, per-cell order
+    hipLaunchKernelGGL(k_cell_count, grid1(n, 256), dim3(256), 0, st, base, (const double *)px, (const double *)py,
+                       (const double *)pz, (const int *)nullptr, cell, cnt, ps.err, ymax, yidx);
+    const int nb = ncell > 0 ? (ncell + SCAN_ITEMS - 1) / SCAN_ITEMS : 1;
+    hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, st, (const int *)cnt, d_cell_start, ncell, bsum);
+    hipLaunchKernelGGL(k_scan_finish, dim3(nb), dim3(256), 0, st, d_cell_start, ncell, (const int *)bsum, nb,
+                       (const double *)ymax, (const int *)yidx, (int)grid1(n, 256).x, misc);
+    hipLaunchKernelGGL(k_cell_scatter, grid1(n, 256), dim3(256), 0, st, n, (const int *)cell,
+                       (const int *)d_cell_start, cur, d_ids);
+    hipLaunchKernelGGL(k_cell_sort, grid1(ncell, 256), dim3(256), 0, st, ncell, (const int *)d_cell_start, d_ids, ps,
+                       cnt, cur);
+    hipLaunchKernelGGL(k_na_status, dim3(1), dim3(64), 0, st, (const int *)misc, d_err);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(nullptr, FIPI_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
+    return FIPI_OK;
+}

@@ -152,6 +152,23 @@ int fipi_fft_c2r(fipi_ctx *ctx, void *d_half, void *d_real);
  * d_ids[n] with ids ascending in each cell.  *ncell_out gets ncell. */
 int fipi_cells_build(fipi_ctx *ctx, const double *d_pos, int32_t n, int32_t *d_cell_start, int32_t *d_ids,
                      int32_t *ncell_out);
+/* The same cell list without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No context: the contact grid comes from params (r_c,
+ * margin, box).  Asynchronous on `stream` (a hipStream_t, NULL = the null
+ * stream); *d_err (device int) receives the error bits once the work has
+ * run: 0, or 1 when a particle lies outside the sorting range (the
+ * reference exits there, cluster.cpp:144-163).  *ncell_out (host, may be
+ * NULL) gets ncell at once.  The temporaries live in one static device
+ * array of at most 16 MiB per GPU, held until the process ends: n must not
+ * exceed FIPI_NOALLOC_MAX_PARTICLES and the contact grid
+ * FIPI_NOALLOC_MAX_CELLS cells (FIPI_ERR_ARG otherwise).  Every call on a
+ * GPU shares that array, so calls that overlap on one GPU race. */
+#define FIPI_NOALLOC_MAX_PARTICLES 262144
+#define FIPI_NOALLOC_MAX_CELLS 262144
+int fipi_cells_build_noalloc(const fipi_params *params, const double *d_pos, int32_t n, int32_t *d_cell_start,
+                             int32_t *d_ids, int32_t *d_err, int32_t *ncell_out, void *stream);
 /* Contact pairs (i<j, |d| < r_eq) of the 27-cell search of
  * cluster::compute_f_pp (cluster.cpp:193-262), sorted by (i, j).  Writes up
  * to max_pairs pairs to d_pairs[2*max_pairs]; *count gets the total. */
