@@ -576,8 +576,8 @@ int adj_start(fipi_adjoint *a, double mobility, double lambda, const double *the
     if (rc) return adj_err(a, rc, ctx->err);
     if (a->n > 0) {
         // contact forces of the fixed particles (full ramp), spread every step
-        if (ctx->fp64) Impl<double>::cells_build(ctx, ctx->pp, ctx->ps, ctx->cell_ids, ctx->cell_start, false, ctx->ymax, ctx->yidx);
-        else Impl<float>::cells_build(ctx, ctx->pp, ctx->ps, ctx->cell_ids, ctx->cell_start, false, ctx->ymax, ctx->yidx);
+        if (ctx->fp64) Impl<double>::cells_build(ctx, ctx->pp, ctx->ps, ctx->cell_ids, ctx->cell_start, ctx->ymax, ctx->yidx);
+        else Impl<float>::cells_build(ctx, ctx->pp, ctx->ps, ctx->cell_ids, ctx->cell_start, ctx->ymax, ctx->yidx);
         launch_contact(ctx->stream, a->n, ctx->pp, ctx->ps, ctx->cell_start, ctx->cell_ids, 1.0, (const int *)nullptr,
                        0.0, ctx->d_flags + 1);
         if (ctx->fp64) Adj<double>::contact_field(a);

@@ -394,13 +394,6 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_f(const cplx<T> *__rest
     store_r2c<T, N>(cv, L, fout, rA, rB, t);
 }
 
-template <typename T>
-struct SpreadEntry {
-    int x;
-    int pid;
-    T v[3];
-};
-
 // Spread sum of one grid node: x on its line and the three force components.
 template <typename T>
 struct SpreadNode {

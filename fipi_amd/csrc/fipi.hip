@@ -31,6 +31,7 @@
 #include "../../include/fipi.h"
 #include "field_kernels.hpp"
 #include "particle_kernels.hpp"
+#include "particle_stage.hpp"
 
 using namespace fipi;
 
@@ -124,7 +125,14 @@ struct fipi_ctx {
     int *cell_cnt = nullptr, *cell_start = nullptr, *cell_ids = nullptr, *cell_cursor = nullptr;
     int nlines = 0;
     int *line_cnt = nullptr, *line_start = nullptr, *line_cursor = nullptr, *node_cnt = nullptr;
-    void *spr = nullptr, *spr_tmp = nullptr;
+    void *spr_tmp = nullptr;  // spread node pool (8n nodes)
+    // the step's particle stages: cell tables of fixed capacity (tab[0]:
+    // the positions at the start of a step, tab[1]: after stage 1) and the
+    // spread's line buckets
+    CellTab tab[2] = {};
+    int tab_parts[2] = {0, 0};  // argmax partials each table holds
+    LineTab lines{};
+    bool tables_ready = false;
     int *bsum = nullptr;       // scan block totals
     double *ymax = nullptr;    // argmax-y block partials
     int *yidx = nullptr;
@@ -423,54 +431,42 @@ struct Impl {
         });
     }
 
+    // Spread of n forces (f0, f1, f2) at positions (px, py, pz): line
+    // buckets, then the per-line node sums (node pool, line_start/node_cnt)
     static void spread_build(fipi_ctx *ctx, const PartParams &pp, const double *px, const double *py,
                              const double *pz, const double *f0, const double *f1, const double *f2,
                              const int *box)
     {
         hipStream_t st = ctx->stream;
-        const int nl = ctx->nlines;
-        const double n = pp.n, E = (double)sizeof(SpreadEntry<T>);
+        LineTab lt = ctx->lines;
+        ctx->spread_ready = false;  // the pool now holds these nodes
         {
-            ProfScope ps_(ctx, "spread_count", 24.0 * n + 32.0 * n);
-            hipLaunchKernelGGL(k_spread_count, grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, box,
-                               ctx->line_cnt, ctx->d_flags);
+            ProfScope ps_(ctx, "spread_lines", 24.0 * pp.n + 16.0 * pp.n);
+            hipLaunchKernelGGL(k_tab_clear, dim3(1), dim3(64), 0, st, lt.cnt, 0, lt.ovf_cnt, (int *)nullptr,
+                               lt.pool_top);
+            hipLaunchKernelGGL(k_spread_lines, grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, box, lt,
+                               ctx->d_flags);
         }
-        {
-            ProfScope ps_(ctx, "spread_scan", 8.0 * nl);
-            scan(ctx, ctx->line_cnt, ctx->line_start, nl, ctx->bsum, nullptr, nullptr, 0, nullptr);
-        }
-        {
-            ProfScope ps_(ctx, "spread_scatter", 48.0 * n + 8.0 * n * E);
-            hipLaunchKernelGGL((k_spread_scatter<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, f0, f1,
-                               f2, box, ctx->line_start, ctx->line_cursor, (SpreadEntry<T> *)ctx->spr_tmp);
-        }
-        {
-            ProfScope ps_(ctx, "spread_rank", 16.0 * n * E + 4.0 * nl);
-            hipLaunchKernelGGL((k_spread_rank<T>), grid1(nl, SPREAD_LPB), dim3(256), 0, st, nl, ctx->line_start,
-                               (const SpreadEntry<T> *)ctx->spr_tmp, (SpreadEntry<T> *)ctx->spr);
-        }
-        {
-            // per-node sums into the (now free) scatter buffer
-            ProfScope ps_(ctx, "spread_nodes", 8.0 * n * E + 8.0 * n * sizeof(SpreadNode<T>) + 8.0 * nl);
-            hipLaunchKernelGGL((k_spread_nodes<T>), grid1(nl, SPREAD_LPB), dim3(256), 0, st, nl, ctx->line_start,
-                               (const SpreadEntry<T> *)ctx->spr, (SpreadNode<T> *)ctx->spr_tmp, ctx->node_cnt,
-                               ctx->line_cnt, ctx->line_cursor);
-        }
+        spread_nodes(ctx, pp, px, py, pz, f0, f1, f2);
     }
 
+    static void spread_nodes(fipi_ctx *ctx, const PartParams &pp, const double *px, const double *py,
+                             const double *pz, const double *f0, const double *f1, const double *f2)
+    {
+        const int nl = ctx->nlines;
+        ProfScope ps_(ctx, "spread_nodes", 16.0 * pp.n + 4 * 48.0 * pp.n + 8.0 * pp.n * sizeof(SpreadNode<T>) + 12.0 * nl);
+        hipLaunchKernelGGL((k_spread_nodes<T>), grid1(nl, NODE_LPB), dim3(256), 0, ctx->stream, pp, ctx->lines, px, py,
+                           pz, f0, f1, f2, (SpreadNode<T> *)ctx->spr_tmp, nl);
+    }
+
+    // CSR cell list with ids ascending per cell (unit entries, adjoint)
     static void cells_build(fipi_ctx *ctx, const PartParams &pp, const PartState &ps, int *ids, int *start,
-                            bool interp, double *ymax, int *yidx)
+                            double *ymax, int *yidx)
     {
         hipStream_t st = ctx->stream;
         const double n = pp.n, nc = ctx->ncell;
-        if (interp) {
-            ProfScope ps_(ctx, "interp_fpi_count", n * (24.0 + 32.0 * sizeof(T) + 16.0 + 56.0 + 4.0));
-            hipLaunchKernelGGL((k_interp_fpi_count<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, ps, R(ctx->c),
-                               R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2]), ctx->cell_cnt, ymax, yidx);
-        } else {
-            hipLaunchKernelGGL(k_cell_count, grid1(pp.n, 256), dim3(256), 0, st, pp, ps.p[0], ps.p[1], ps.p[2],
-                               ps.box, ps.cell, ctx->cell_cnt, ps.err, ymax, yidx);
-        }
+        hipLaunchKernelGGL(k_cell_count, grid1(pp.n, 256), dim3(256), 0, st, pp, ps.p[0], ps.p[1], ps.p[2],
+                           ps.box, ps.cell, ctx->cell_cnt, ps.err, ymax, yidx);
         {
             ProfScope ps_(ctx, "cells_scan_mark", 8.0 * nc + 12.0 * (pp.n / 256 + 1));
             scan(ctx, ctx->cell_cnt, start, ctx->ncell, ctx->bsum, ymax, yidx, (int)grid1(pp.n, 256).x, ps.mark);
@@ -503,6 +499,13 @@ struct Impl {
         spread_build(ctx, ctx->pp, ps.p[0], ps.p[1], ps.p[2], ps.fpp[0], ps.fpp[1], ps.fpp[2], ps.box);
     }
 
+    // the particle stage on pstream from here (one stage pending at a time)
+    static void fork_stage(fipi_ctx *ctx)
+    {
+        join_particles(ctx);
+        particle_stages_beside(ctx);
+    }
+
     // both Heun stages, the step counter and the next step's spread on
     // pstream, after the work queued so far on the main stream
     static void particle_stages_beside(fipi_ctx *ctx)
@@ -514,10 +517,6 @@ struct Impl {
         particle_stage(ctx, 1);
         particle_stage(ctx, 2);
         hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, ctx->pstream, ctx->t_dev);
-        if (ctx->p.switch_ns != 0) {
-            spread_now(ctx);
-            ctx->spread_ready = true;
-        }
         (void)hipEventRecord(ctx->ev_join, ctx->pstream);
         ctx->stream = main;
         ctx->p_pending = true;
@@ -531,25 +530,65 @@ struct Impl {
         return 1.0;
     }
 
+    // tab[0] from the current positions (after the state was set or restored)
+    static void tables_init(fipi_ctx *ctx)
+    {
+        const PartParams &pp = ctx->pp;
+        const PartState &ps = ctx->ps;
+        const CellTab &T0 = ctx->tab[0];
+        hipLaunchKernelGGL(k_tab_clear, grid1(ctx->ncell, 256), dim3(256), 0, ctx->stream, T0.cnt, ctx->ncell,
+                           T0.ovf_cnt, (int *)nullptr, (int *)nullptr);
+        hipLaunchKernelGGL(k_cells_append, grid1(pp.n, 256), dim3(256), 0, ctx->stream, pp, (const double *)ps.p[0],
+                           (const double *)ps.p[1], (const double *)ps.p[2], ps.box, T0, ctx->d_flags);
+        ctx->tab_parts[0] = (int)grid1(pp.n, 256).x;
+        ctx->tables_ready = true;
+    }
+
+    // One Heun stage (main.cpp:198-217): interpolation and f_pi (A), then
+    // contacts, advection and the next sort (B) over tab[stage-1] into
+    // tab[2-stage]; stage 2 also spreads f_pp at the new positions for the
+    // next step's force pass (when the Stokes solve is on).
     static void particle_stage(fipi_ctx *ctx, int stage)
     {
         hipStream_t st = ctx->stream;
         const PartParams &pp = ctx->pp;
-        cells_build(ctx, pp, ctx->ps, ctx->cell_ids, ctx->cell_start, true, ctx->ymax, ctx->yidx);
-        const double rmp = ramp((double)(ctx->t + 1), ctx->p.ramp_t);
+        const PartState &ps = ctx->ps;
+        if (!ctx->tables_ready) tables_init(ctx);
+        const int cur = stage - 1, nxt = 2 - stage;
+        const bool spread = stage == 2 && ctx->p.switch_ns != 0;
+        const double n = pp.n;
         {
-            ProfScope ps_(ctx, "contact", pp.n * (24.0 + 4.0 + 24.0 + 8.0 + 24.0));
-            launch_contact(st, pp.n, pp, ctx->ps, ctx->cell_start, ctx->cell_ids, rmp, (const int *)ctx->t_dev,
-                           ctx->p.ramp_t, ctx->d_flags + 1);
+            ProfScope ps_(ctx, "stage_a", n * (24.0 + 32.0 * sizeof(T) + 8.0 + 24.0 + 24.0 + 24.0 + 16.0));
+            hipLaunchKernelGGL((k_stage_a<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, ps, (const T *)R(ctx->c),
+                               (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
+                               ctx->tab[cur], ctx->tab_parts[cur], ctx->tab[nxt], ctx->ncell, ctx->lines,
+                               spread ? 1 : 0);
         }
+        const double rmp = ramp((double)(ctx->t + 1), ctx->p.ramp_t);
+        LineTab lt = ctx->lines;
+        if (!spread) lt.cnt = nullptr;
         {
-            ProfScope ps_(ctx, "advect", pp.n * (5 * 24.0 + 8.0 + 2 * 24.0));
-            hipLaunchKernelGGL(k_advect, grid1(pp.n, 256), dim3(256), 0, st, pp, ctx->ps, stage);
+            ProfScope ps_(ctx, "contact_tile", n * (4.0 + 32.0 * 2.3 + 24.0 + 8.0 + 24.0 + 5 * 24.0 + 32.0 + (spread ? 16.0 : 0.0)));
+            const dim3 grid = contact_tile_grid(pp);
+            if (stage == 1)
+                hipLaunchKernelGGL(k_contact_tile<1>, grid, dim3(256), 0, st, pp, ps, ctx->tab[cur], ctx->tab[nxt], lt,
+                                   (const int *)ctx->t_dev, ctx->p.ramp_t, rmp, ctx->d_flags + 1);
+            else
+                hipLaunchKernelGGL(k_contact_tile<2>, grid, dim3(256), 0, st, pp, ps, ctx->tab[cur], ctx->tab[nxt], lt,
+                                   (const int *)ctx->t_dev, ctx->p.ramp_t, rmp, ctx->d_flags + 1);
+            ctx->tab_parts[nxt] = (int)(grid.x * grid.y * grid.z);
+        }
+        if (spread) {
+            spread_nodes(ctx, pp, ps.p[0], ps.p[1], ps.p[2], ps.fpp[0], ps.fpp[1], ps.fpp[2]);
+            ctx->spread_ready = true;
         }
     }
 
-    // overlap_next: this step's particle stage may run beside the next step
-    static void step(fipi_ctx *ctx, bool overlap_next = false)
+    // beside: this step's particle stage runs on pstream after the step's
+    // passes, beside the next step's first passes, until the next step's
+    // force pass (which reads its spread); join_end: the main stream waits
+    // for it at the end of this step (the last step of a call)
+    static void step(fipi_ctx *ctx, bool beside = false, bool join_end = true)
     {
         const Dims d = ctx->d;
         hipStream_t st = ctx->stream;
@@ -561,6 +600,7 @@ struct Impl {
         const double Rb = (double)ctx->G * ctx->nbox * sizeof(T);
         const double Hb = (double)d.nxh * d.ny * d.nz * ctx->nbox * 2 * sizeof(T);
         const bool mixed = !std::is_same<T, double>::value;
+        const bool fork = beside && parts && !ctx->fix_particles;
         if (ns) {
             if (!ctx->conv_ready) {
                 ProdConv<T> pr{R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2])};
@@ -682,10 +722,14 @@ struct Impl {
                 ctx->conv_ready = true;
             }
         }
-        join_particles(ctx);
-        if (parts && !ctx->fix_particles && overlap_next) {
-            particle_stages_beside(ctx);
+        if (fork) {
+            // beside the next step's first passes, which touch no particle
+            // state (measured: starting it right after the force pass, beside
+            // the Stokes passes, is slower)
+            fork_stage(ctx);
+            if (join_end) join_particles(ctx);
         } else {
+            join_particles(ctx);
             if (parts && !ctx->fix_particles) {
                 particle_stage(ctx, 1);
                 particle_stage(ctx, 2);
@@ -804,6 +848,8 @@ int check_params(const fipi_params *p)
     if (p->precision != FIPI_FP64 && p->precision != FIPI_FP32_MIXED)
         return set_err(nullptr, FIPI_ERR_ARG, "precision must be FIPI_FP64 or FIPI_FP32_MIXED");
     if (p->n_particles < 0) return set_err(nullptr, FIPI_ERR_ARG, "n_particles < 0");
+    // the contact kernel orders partners by a 32-bit key of (cell, id)
+    if (p->n_particles >= (1 << 26)) return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "n_particles must be below 2^26");
     if (!(p->lx > 0 && p->ly > 0 && p->lz > 0 && p->dt > 0))
         return set_err(nullptr, FIPI_ERR_ARG, "box lengths and dt must be positive");
     if (p->switch_particle && p->n_particles > 0) {
@@ -917,8 +963,37 @@ int setup_buffers(fipi_ctx *ctx)
     HIPCHK(ctx, hipMemset(ctx->line_cursor, 0, sizeof(int) * (ctx->nlines + 1)));
     const size_t n = ctx->n > 0 ? ctx->n : 1;
     ALLOC(ctx->cell_ids, sizeof(int) * n);
-    ALLOC(ctx->spr, sizeof(SpreadEntry<T>) * 8 * n);
-    ALLOC(ctx->spr_tmp, sizeof(SpreadEntry<T>) * 8 * n);
+    ALLOC(ctx->spr_tmp, sizeof(SpreadNode<T>) * spread_pool_nodes(ctx->nlines, (int)n));
+    // the step's cell tables and spread line buckets
+    for (int k = 0; k < 2; k++) {
+        CellTab &tb = ctx->tab[k];
+        ALLOC(tb.cnt, sizeof(int) * (ctx->ncell + 1));
+        ALLOC(tb.rec, sizeof(CellRec) * CELL_CAP * (size_t)ctx->ncell);
+        ALLOC(tb.ovf_cnt, sizeof(int));
+        ALLOC(tb.ovf, sizeof(CellRec) * n);
+        {
+            // argmax partials: one per contact tile or per 256 particles
+            const dim3 tg = contact_tile_grid(pp);
+            const size_t parts = std::max<size_t>((size_t)tg.x * tg.y * tg.z, n / 256 + 1) + 2;
+            ALLOC(tb.ymax, sizeof(double) * parts);
+            ALLOC(tb.yidx, sizeof(int) * parts);
+        }
+        HIPCHK(ctx, hipMemset(tb.cnt, 0, sizeof(int) * (ctx->ncell + 1)));
+        HIPCHK(ctx, hipMemset(tb.ovf_cnt, 0, sizeof(int)));
+    }
+    {
+        LineTab &lt = ctx->lines;
+        lt.cnt = ctx->line_cnt;
+        ALLOC(lt.pid, sizeof(int) * LINE_CAP * (size_t)ctx->nlines);
+        ALLOC(lt.ovf_cnt, sizeof(int));
+        ALLOC(lt.ovf, sizeof(int2) * 4 * n);
+        lt.node_base = ctx->line_start;
+        lt.node_cnt = ctx->node_cnt;
+        ALLOC(lt.pool_top, sizeof(int));
+        HIPCHK(ctx, hipMemset(lt.ovf_cnt, 0, sizeof(int)));
+        HIPCHK(ctx, hipMemset(lt.pool_top, 0, sizeof(int)));
+        HIPCHK(ctx, hipMemset(ctx->node_cnt, 0, sizeof(int) * (ctx->nlines + 1)));
+    }
     {
         const size_t m = std::max<size_t>(std::max<size_t>(ctx->ncell, ctx->nlines), n);
         ALLOC(ctx->bsum, sizeof(int) * (m / SCAN_ITEMS + 2));
@@ -1045,8 +1120,8 @@ int unit_cells(fipi_ctx *ctx, TmpParts &tp, int *start, int *ids)
     int *yidx, rc;
     if ((rc = tp.alloc((void **)&ymax, sizeof(double) * (tp.n / 256 + 2)))) return rc;
     if ((rc = tp.alloc((void **)&yidx, sizeof(int) * (tp.n / 256 + 2)))) return rc;
-    if (ctx->fp64) Impl<double>::cells_build(ctx, pp, tp.ps, ids, start, false, ymax, yidx);
-    else Impl<float>::cells_build(ctx, pp, tp.ps, ids, start, false, ymax, yidx);
+    if (ctx->fp64) Impl<double>::cells_build(ctx, pp, tp.ps, ids, start, ymax, yidx);
+    else Impl<float>::cells_build(ctx, pp, tp.ps, ids, start, ymax, yidx);
     return FIPI_OK;
 }
 
@@ -1067,6 +1142,8 @@ static int create_ctx(const fipi_params *params, int nbox, int n_total, fipi_ctx
     int rc = check_params(params);
     if (rc != FIPI_OK) return rc;
     if (nbox < 1) return set_err(nullptr, FIPI_ERR_ARG, "nbox < 1");
+    if (n_total < 0 || n_total >= (1 << 26))
+        return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "the particle count must be below 2^26");
     fipi_ctx *ctx = new fipi_ctx();
     ctx->p = *params;
     ctx->nbox = nbox;
@@ -1150,6 +1227,8 @@ static int set_state_slab(fipi_ctx *ctx, const double *phi, const double *pos, c
     ctx->t = 0;
     ctx->cur = 0;
     ctx->conv_ready = false;
+    ctx->spread_ready = false;
+    ctx->tables_ready = false;
     {
         // FP64 k = 0 mode of each box: c_hat(0) = sum of phi (used by the FP32-mixed path)
         std::vector<double> k0h((size_t)K0_STRIDE * ctx->nbox, 0.0);
@@ -1352,14 +1431,15 @@ int fipi_step_async(fipi_ctx *ctx, int32_t nsteps)
     if (!ctx) return set_err(ctx, FIPI_ERR_ARG, "ctx is NULL");
     if (!ctx->has_state) return set_err(ctx, FIPI_ERR_STATE, "fipi_set_state must be called first");
     if (nsteps < 0) return set_err(ctx, FIPI_ERR_ARG, "nsteps < 0");
-    // each step's particle stage runs beside the next step of this call
-    // (never under per-launch timing, whose events are on the main stream);
-    // the last step's runs in order, so all work ends on the main stream
-    const bool overlap = ctx->c_alt != nullptr && !ctx->prof_on && !ctx->fix_particles;
+    // each step's particle stage runs on a second stream beside the next
+    // step's first passes (never under per-launch timing, whose events are
+    // on the main stream); the main stream waits for the last one, so all
+    // work ends there
+    const bool beside = ctx->c_alt != nullptr && !ctx->prof_on && !ctx->fix_particles;
     for (int s = 0; s < nsteps; s++) {
-        const bool next = overlap && s + 1 < nsteps;
-        if (ctx->fp64) Impl<double>::step(ctx, next);
-        else Impl<float>::step(ctx, next);
+        const bool last = s + 1 == nsteps;
+        if (ctx->fp64) Impl<double>::step(ctx, beside, last);
+        else Impl<float>::step(ctx, beside, last);
     }
     HIPCHK(ctx, hipGetLastError());
     return FIPI_OK;
@@ -1738,6 +1818,9 @@ int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size)
     ctx->conv_ready = h.conv_ready != 0;
     ctx->pp.has_ctheta = h.has_ctheta;
     ctx->tune.y_e = h.y_e;
+    ctx->spread_ready = false;
+    ctx->tables_ready = false;
+    ctx->p_pending = false;
     ctx->has_state = true;
     return FIPI_OK;
 }
@@ -1814,6 +1897,8 @@ int fipi_tune(fipi_ctx *ctx, int32_t steps)
     ctx->t = t0;
     ctx->cur = cur0;
     ctx->conv_ready = conv0;
+    ctx->spread_ready = false;
+    ctx->tables_ready = false;
     return rc;
 }
 

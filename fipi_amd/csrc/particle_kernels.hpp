@@ -172,7 +172,7 @@ __device__ __forceinline__ void argmax_take(double &by, int &bi, double oy, int 
     }
 }
 
-__device__ __forceinline__ void block_argmax_y(double y, int idx, double *ymax, int *yidx)
+__device__ __forceinline__ void block_argmax_y(double y, int idx, double *ymax, int *yidx, int slot = -1)
 {
     __shared__ double sy[4];
     __shared__ int si[4];
@@ -187,8 +187,9 @@ __device__ __forceinline__ void block_argmax_y(double y, int idx, double *ymax, 
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int w = 1; w < 4; w++) argmax_take(by, bi, sy[w], si[w]);
-        ymax[blockIdx.x] = by;
-        yidx[blockIdx.x] = bi;
+        const int k = slot >= 0 ? slot : (int)blockIdx.x;
+        ymax[k] = by;
+        yidx[k] = bi;
     }
 }
 
@@ -711,96 +712,10 @@ __device__ __forceinline__ bool grid_cell_ok(const PartParams &P, double px, dou
     return i >= 0 && i < P.nx && j >= 0 && j < P.ny && k >= 0 && k < P.nz;
 }
 
-// Interpolate c and grad c (element.cpp:55-71; u is not interpolated,
-// element.cpp:38-52), then f_pi (element.cpp:75-97), and the contact cell
-// histogram of the following sort (cluster.cpp:116-169).
-template <typename T>
-__global__ __launch_bounds__(256) void k_interp_fpi_count(PartParams P, PartState S, const T *__restrict__ c,
-                                                          const T *__restrict__ g0, const T *__restrict__ g1,
-                                                          const T *__restrict__ g2, int *__restrict__ cnt,
-                                                          double *__restrict__ ymax, int *__restrict__ yidx)
-{
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    block_argmax_y(a < P.n ? S.p[1][a] : -1.0, a, ymax, yidx);
-    if (a >= P.n) return;
-    const double px = S.p[0][a], py = S.p[1][a], pz = S.p[2][a];
-    if (!grid_cell_ok(P, px, py, pz)) {
-        atomicOr(S.err, 2);
-        S.cell[a] = -1;
-        return;
-    }
-    const int bx = part_box(S.box, a);
-    c += bx * P.rs;
-    g0 += bx * P.rs;
-    g1 += bx * P.rs;
-    g2 += bx * P.rs;
-    const double phi = interp1(P, c, px, py, pz);
-    const double gx = interp1(P, g0, px, py, pz);
-    const double gy = interp1(P, g1, px, py, pz);
-    const double gz = interp1(P, g2, px, py, pz);
-    S.cloc[a] = phi;
-    S.dcdx[0][a] = gx;
-    S.dcdx[1][a] = gy;
-    S.dcdx[2][a] = gz;
-    double f[3] = {0.0, 0.0, 0.0};
-    if (fabs(phi) <= P.tanh2) {
-        const double gn = vnorm(gx, gy, gz);
-        if (gn != 0.0) {
-            const double r = S.rad[a];
-            double d = 0.5 * log((1.0 + phi) / (1.0 - phi)) * P.sqrt2 * P.epsilon;
-            if (P.has_ctheta) {
-                const double ct = S.cth[a];
-                if (ct != 0.0) d = d - r * ct;
-            }
-            double fm = 0.0;
-            if (fabs(d) < r) fm = -P.a_pi * P.pi * d * P.sigma;
-            f[0] = gx / gn * fm;
-            f[1] = gy / gn * fm;
-            f[2] = gz / gn * fm;
-        }
-    }
-    S.fpi[0][a] = f[0];
-    S.fpi[1][a] = f[1];
-    S.fpi[2][a] = f[2];
-    int ci, cj, ck;
-    const int cc = cell_of(P, px, py, pz, &ci, &cj, &ck);
-    if (cc < 0) { S.cell[a] = -1; atomicOr(S.err, 1); return; }
-    const int gc = cc + bx * P.ncell_box;
-    S.cell[a] = gc;
-    atomicAdd(&cnt[gc], 1);
-}
-
-// Heun stage (element.cpp:156-173; uf = 0, f_ext = 0) and periodic wrap
-// (element.cpp:177-186).
-__global__ void k_advect(PartParams P, PartState S, int stage)
-{
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= P.n) return;
-    const double r = S.rad[a];
-    const double L[3] = {P.lx, P.ly, P.lz};
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const double fs = S.fpi[q][a] + 0.0 + S.fpp[q][a];
-        const double v = 0.0 + fs / 6.0 / P.pi / P.mu / r;
-        double x;
-        if (stage == 1) {
-            S.uo[q][a] = v;
-            S.po[q][a] = S.p[q][a];
-            x = S.p[q][a] + v * P.dt;
-        } else {
-            S.u[q][a] = v;
-            x = S.po[q][a] + (S.uo[q][a] + v) * (0.5 * P.dt);
-        }
-        S.p[q][a] = fmod(x + L[q], L[q]);
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Spreading (cluster.cpp:344-410) as a line-sorted segmented reduction.
-// Each particle yields 8 entries (node x, particle id, force * weight); they
-// are bucketed by real grid line (y + ny*z), sorted by (x, id) inside each
-// line, and the x pass of the force transform sums them per node in that
-// order.
+// Spreading (cluster.cpp:344-410): grid cell and fractions of a particle;
+// the per-line node sums are built in particle_stage.hpp and added by the
+// x pass of the force transform.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void spread_corner(const PartParams &P, double px, double py, double pz, int *x, int *y,
                                               int *z, double *o, double *p, double *q)
@@ -811,168 +726,6 @@ __device__ __forceinline__ void spread_corner(const PartParams &P, double px, do
     *o = px / P.dx - floor(px / P.dx);
     *p = py / P.dy - floor(py / P.dy);
     *q = pz / P.dz - floor(pz / P.dz);
-}
-
-__global__ void k_spread_count(PartParams P, const double *__restrict__ px, const double *__restrict__ py,
-                               const double *__restrict__ pz, const int *__restrict__ box,
-                               int *__restrict__ line_cnt, int *__restrict__ err)
-{
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= P.n) return;
-    int x, y, z;
-    double o, p, q;
-    spread_corner(P, px[a], py[a], pz[a], &x, &y, &z, &o, &p, &q);
-    if (x < 0 || x >= P.nx || y < 0 || y >= P.ny || z < 0 || z >= P.nz) { atomicOr(err, 2); return; }
-    line_cnt += part_box(box, a) * P.nlines_box;
-    const int y1 = (y + 1) % P.ny, z1 = (z + 1) % P.nz;
-    atomicAdd(&line_cnt[y + P.ny * z], 2);
-    atomicAdd(&line_cnt[y1 + P.ny * z], 2);
-    atomicAdd(&line_cnt[y + P.ny * z1], 2);
-    atomicAdd(&line_cnt[y1 + P.ny * z1], 2);
-}
-
-template <typename T>
-__global__ void k_spread_scatter(PartParams P, const double *__restrict__ px, const double *__restrict__ py,
-                                 const double *__restrict__ pz, const double *__restrict__ f0,
-                                 const double *__restrict__ f1, const double *__restrict__ f2,
-                                 const int *__restrict__ box, const int *__restrict__ line_start,
-                                 int *__restrict__ cursor, SpreadEntry<T> *__restrict__ out)
-{
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= P.n) return;
-    int x, y, z;
-    double o, p, q;
-    spread_corner(P, px[a], py[a], pz[a], &x, &y, &z, &o, &p, &q);
-    if (x < 0 || x >= P.nx || y < 0 || y >= P.ny || z < 0 || z >= P.nz) return;
-    {
-        const int lb = part_box(box, a) * P.nlines_box;
-        line_start += lb;
-        cursor += lb;
-    }
-    // f = (f_pp + f_ext)/dx/dy/dz, f_ext = 0
-    const double f[3] = {(f0[a] + 0.0) / P.dx / P.dy / P.dz, (f1[a] + 0.0) / P.dx / P.dy / P.dz,
-                         (f2[a] + 0.0) / P.dx / P.dy / P.dz};
-    const int x1 = (x + 1) % P.nx, y1 = (y + 1) % P.ny, z1 = (z + 1) % P.nz;
-    // corners in the reference order (cluster.cpp:368-406)
-    const int cx[8] = {x, x1, x1, x1, x1, x, x, x};
-    const int cy[8] = {y, y, y1, y, y1, y, y1, y1};
-    const int cz[8] = {z, z, z, z1, z1, z1, z, z1};
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        SpreadEntry<T> e;
-        e.x = cx[c];
-        e.pid = a;
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            double w;
-            switch (c) {
-            case 0: w = f[d] * (1 - o) * (1 - p) * (1 - q); break;
-            case 1: w = f[d] * o * (1 - p) * (1 - q); break;
-            case 2: w = f[d] * o * p * (1 - q); break;
-            case 3: w = f[d] * o * (1 - p) * q; break;
-            case 4: w = f[d] * o * p * q; break;
-            case 5: w = f[d] * (1 - o) * (1 - p) * q; break;
-            case 6: w = f[d] * (1 - o) * p * (1 - q); break;
-            default: w = f[d] * (1 - o) * p * q; break;
-            }
-            e.v[d] = (T)w;
-        }
-        const int line = cy[c] + P.ny * cz[c];
-        const int slot = atomicAdd(&cursor[line], 1);
-        out[line_start[line] + slot] = e;
-    }
-}
-
-// Lanes per grid line in the spread's sort and node passes: a line holds
-// 8 entries per sphere that touches it, about a dozen at the headline density.
-constexpr int SPREAD_LANES = 16;
-constexpr int SPREAD_LPB = 256 / SPREAD_LANES;  // lines per workgroup
-
-// Sort each line's entries by (x, particle id) into `out`: SPREAD_LANES
-// lanes per line; each chunk of SPREAD_LANES entries is held one per lane,
-// and every lane ranks its entry by counting smaller keys among all entries
-// of the line (keys of the chunk in hand are broadcast from their lanes).
-// Within a line the keys are unique (a particle meets a line at two
-// different x), so the ranks are a permutation and the result is
-// deterministic.
-template <typename T>
-__global__ __launch_bounds__(256) void k_spread_rank(int nlines, const int *__restrict__ line_start,
-                                                     const SpreadEntry<T> *__restrict__ in,
-                                                     SpreadEntry<T> *__restrict__ out)
-{
-    constexpr int W = SPREAD_LANES;
-    const int line = blockIdx.x * SPREAD_LPB + (int)threadIdx.x / W;
-    const int lane = threadIdx.x % W;
-    if (line >= nlines) return;  // whole segments leave together
-    const int b = line_start[line], e = line_start[line + 1];
-    for (int i0 = b; i0 < e; i0 += W) {
-        const int i = i0 + lane;
-        const bool mine = i < e;
-        SpreadEntry<T> v{};
-        if (mine) v = in[i];
-        int r = 0;
-        for (int j0 = b; j0 < e; j0 += W) {
-            // chunk j0 of the line, one key per lane
-            const int jj = j0 + lane;
-            const int kx = jj < e ? in[jj].x : 0x7fffffff, kp = jj < e ? in[jj].pid : 0x7fffffff;
-            const int m = min(W, e - j0);
-            for (int k = 0; k < m; k++) {
-                const int xj = __shfl(kx, k, W), pj = __shfl(kp, k, W);
-                r += (xj < v.x || (xj == v.x && pj < v.pid)) ? 1 : 0;
-            }
-        }
-        if (mine) out[b + r] = v;
-    }
-}
-
-// Per-node sums of the sorted entries of each line, SPREAD_LANES lanes per
-// line: the lane at the head of each run of equal x sums the run in
-// ascending particle order (the reference's accumulation order,
-// cluster.cpp:355-407) and the heads are compacted with a ballot + popcount
-// over the line's lanes.  Node k of line l is stored at
-// nodes[line_start[l] + k]; node_cnt[l] gets the count.  line_cnt and
-// line_cursor of the line are cleared for the next build.
-template <typename T>
-__global__ __launch_bounds__(256) void k_spread_nodes(int nlines, const int *__restrict__ line_start,
-                                                      const SpreadEntry<T> *__restrict__ e,
-                                                      SpreadNode<T> *__restrict__ nodes, int *__restrict__ node_cnt,
-                                                      int *__restrict__ line_cnt, int *__restrict__ line_cursor)
-{
-    constexpr int W = SPREAD_LANES;
-    const int line = blockIdx.x * SPREAD_LPB + (int)threadIdx.x / W;
-    const int lane = threadIdx.x % W;
-    const int shift = (int)(threadIdx.x & 63) - lane;  // first wave lane of this line
-    if (line >= nlines) return;
-    if (lane == 0) {
-        line_cnt[line] = 0;
-        line_cursor[line] = 0;
-    }
-    const int b = line_start[line], en = line_start[line + 1];
-    int base = 0;
-    for (int c0 = b; c0 < en; c0 += W) {
-        const int i = c0 + lane;
-        const bool valid = i < en;
-        const bool head = valid && (i == b || e[i - 1].x != e[i].x);
-        const unsigned long long seg = (__ballot(head) >> shift) & ((1ull << W) - 1ull);
-        if (head) {
-            const int x = e[i].x;
-            T s0 = T(0), s1 = T(0), s2 = T(0);
-            for (int j = i; j < en && e[j].x == x; j++) {
-                s0 += e[j].v[0];
-                s1 += e[j].v[1];
-                s2 += e[j].v[2];
-            }
-            const int rank = __popcll(seg & ((1ull << lane) - 1ull));
-            SpreadNode<T> nd;
-            nd.x = x;
-            nd.v[0] = s0;
-            nd.v[1] = s1;
-            nd.v[2] = s2;
-            nodes[b + base + rank] = nd;
-        }
-        base += __popcll(seg);
-    }
-    if (lane == 0) node_cnt[line] = base;
 }
 
 // Dense spread (unit entry point): one thread per grid line adds its

@@ -1,0 +1,932 @@
+// One Heun particle stage of the time step in two launches, and the
+// spread of the particle forces in one more:
+//
+//   k_stage_a   per particle: trilinear interpolation of c and grad c
+//               (element.cpp:55-71), f_pi (element.cpp:75-97); block 0
+//               reduces it_mark (cluster.cpp:128-138) from the argmax
+//               partials of the positions the current cell table holds;
+//               the next table's counters are cleared.
+//   k_contact_tile  per tile of contact cells, staged in LDS: contact forces
+//               over the 27 neighbour cells (cluster.cpp:174-275), summed in
+//               the reference's order; the Heun stage and the periodic wrap
+//               (element.cpp:156-186); the new position is appended to the
+//               next table (the sort of the following stage,
+//               cluster.cpp:116-169); in stage 2 the particle id is also
+//               appended to the four grid lines its spread touches.
+//   k_spread_nodes  per grid line: the two spread entries of every particle
+//               on the line, sorted by (x, particle id) and summed per node
+//               in ascending particle order (cluster.cpp:344-410); the nodes
+//               go to a pool the force pass reads.
+//
+// Tables have a fixed capacity per cell / line; entries beyond it go to an
+// overflow list that exact slow paths scan, so any input is handled.  Slot
+// order inside a cell or line depends on atomic timing, but every sum is
+// taken in an order fixed by particle ids, so results are deterministic and
+// bitwise those of the reference restatement.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "particle_kernels.hpp"
+
+namespace fipi {
+
+constexpr int CELL_CAP = 16;  // records per contact cell before overflow
+constexpr int LINE_CAP = 32;  // particles per grid line before overflow
+
+// A particle as the cell table holds it: its position at the time of the
+// append and its id (cell: the global cell, used by overflow records).
+struct __align__(16) CellRec {
+    double x, y, z;
+    int id, cell;
+};
+
+struct CellTab {
+    int *cnt;       // [ncell] records appended (may exceed CELL_CAP)
+    CellRec *rec;   // [ncell * CELL_CAP]
+    int *ovf_cnt;   // records in the overflow list
+    CellRec *ovf;   // [n] records beyond a cell's capacity
+    double *ymax;   // argmax-y partials of the appended positions, per block
+    int *yidx;
+};
+
+struct LineTab {
+    int *cnt;        // [nlines] particles appended (may exceed LINE_CAP)
+    int *pid;        // [nlines * LINE_CAP]
+    int *ovf_cnt;    // entries in the overflow list
+    int2 *ovf;       // [4n] (line, particle) beyond a line's capacity
+    int *node_base;  // [nlines] first node of each line in the pool
+    int *node_cnt;   // [nlines]
+    int *pool_top;   // nodes allocated
+};
+
+__device__ __forceinline__ void cell_append(const PartParams &P, CellTab T, int g, double x, double y, double z,
+                                            int id)
+{
+    const int slot = atomicAdd(&T.cnt[g], 1);
+    CellRec r;
+    r.x = x;
+    r.y = y;
+    r.z = z;
+    r.id = id;
+    r.cell = g;
+    if (slot < CELL_CAP) {
+        T.rec[(size_t)g * CELL_CAP + slot] = r;
+    } else {
+        const int o = atomicAdd(T.ovf_cnt, 1);
+        T.ovf[o] = r;
+    }
+}
+
+// the four grid lines of the spread of a particle at (px, py, pz)
+// (cluster.cpp:368-406: corners at y or y+1 and z or z+1); false when the
+// particle lies outside the grid
+__device__ __forceinline__ bool spread_lines(const PartParams &P, double px, double py, double pz, int box,
+                                             int *l4)
+{
+    const int x = (int)floor(px / P.dx), y = (int)floor(py / P.dy), z = (int)floor(pz / P.dz);
+    if (x < 0 || x >= P.nx || y < 0 || y >= P.ny || z < 0 || z >= P.nz) return false;
+    const int y1 = (y + 1) % P.ny, z1 = (z + 1) % P.nz, lb = box * P.nlines_box;
+    l4[0] = lb + y + P.ny * z;
+    l4[1] = lb + y1 + P.ny * z;
+    l4[2] = lb + y + P.ny * z1;
+    l4[3] = lb + y1 + P.ny * z1;
+    return true;
+}
+
+__device__ __forceinline__ void line_append(LineTab T, int line, int id)
+{
+    const int slot = atomicAdd(&T.cnt[line], 1);
+    if (slot < LINE_CAP) {
+        T.pid[(size_t)line * LINE_CAP + slot] = id;
+    } else {
+        const int o = atomicAdd(T.ovf_cnt, 1);
+        T.ovf[o] = make_int2(line, id);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Table from positions (after the state is set or restored) and its argmax
+// partials; the table's counters must be zero.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cells_append(PartParams P, const double *__restrict__ px,
+                                                      const double *__restrict__ py, const double *__restrict__ pz,
+                                                      const int *__restrict__ box, CellTab T, int *__restrict__ err)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    block_argmax_y(a < P.n ? py[a] : -1.0, a, T.ymax, T.yidx);
+    if (a >= P.n) return;
+    int i, j, k;
+    const double x = px[a], y = py[a], z = pz[a];
+    const int cc = cell_of(P, x, y, z, &i, &j, &k);
+    if (cc < 0) {
+        atomicOr(err, 1);
+        return;
+    }
+    cell_append(P, T, cc + part_box(box, a) * P.ncell_box, x, y, z, a);
+}
+
+// counters of a table (and optionally of the spread lines) to zero; a
+// kernel, not a memset, so that graph capture records no memset node
+__global__ void k_tab_clear(int *cnt, int n, int *ovf_cnt, int *line_ovf, int *pool_top)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int q = i; q < n; q += gridDim.x * blockDim.x) cnt[q] = 0;
+    if (i == 0) {
+        *ovf_cnt = 0;
+        if (line_ovf) *line_ovf = 0;
+        if (pool_top) *pool_top = 0;
+    }
+}
+
+// spread lines of n particles (the spread outside the step's stage 2)
+__global__ __launch_bounds__(256) void k_spread_lines(PartParams P, const double *__restrict__ px,
+                                                      const double *__restrict__ py, const double *__restrict__ pz,
+                                                      const int *__restrict__ box, LineTab T, int *__restrict__ err)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    int l4[4];
+    if (!spread_lines(P, px[a], py[a], pz[a], part_box(box, a), l4)) {
+        atomicOr(err, 2);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) line_append(T, l4[q], a);
+}
+
+// ---------------------------------------------------------------------------
+// Stage kernel A
+// ---------------------------------------------------------------------------
+// it_mark from nparts argmax partials: the strictly largest y > 0, first
+// index on ties; the previous value is kept if no y > 0 (cluster.cpp:128-138)
+__device__ void reduce_mark(const double *__restrict__ ymax, const int *__restrict__ yidx, int nparts, int *mark)
+{
+    __shared__ double sy[256];
+    __shared__ int si[256];
+    double by = 0.0;
+    int bi = -1;
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) argmax_take(by, bi, ymax[k], yidx[k]);
+    sy[threadIdx.x] = by;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            double y = sy[threadIdx.x];
+            int i = si[threadIdx.x];
+            argmax_take(y, i, sy[threadIdx.x + o], si[threadIdx.x + o]);
+            sy[threadIdx.x] = y;
+            si[threadIdx.x] = i;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && si[0] >= 0) *mark = si[0];
+}
+
+// clear_lines: also reset the spread's overflow list and node pool (the
+// stage whose B appends spread lines)
+template <typename T>
+__global__ __launch_bounds__(256) void k_stage_a(PartParams P, PartState S, const T *__restrict__ c,
+                                                 const T *__restrict__ g0, const T *__restrict__ g1,
+                                                 const T *__restrict__ g2, CellTab cur, int nparts, CellTab nxt,
+                                                 int ncell, LineTab lt, int clear_lines)
+{
+    if (blockIdx.x == 0) reduce_mark(cur.ymax, cur.yidx, nparts, S.mark);
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int q = a; q < ncell; q += gridDim.x * blockDim.x) nxt.cnt[q] = 0;
+    if (a == 0) {
+        *nxt.ovf_cnt = 0;
+        if (clear_lines) {
+            *lt.ovf_cnt = 0;
+            *lt.pool_top = 0;
+        }
+    }
+    if (a >= P.n) return;
+    const double px = S.p[0][a], py = S.p[1][a], pz = S.p[2][a];
+    if (!grid_cell_ok(P, px, py, pz)) {
+        atomicOr(S.err, 2);
+        return;
+    }
+    const int bx = part_box(S.box, a);
+    c += bx * P.rs;
+    g0 += bx * P.rs;
+    g1 += bx * P.rs;
+    g2 += bx * P.rs;
+    const double phi = interp1(P, c, px, py, pz);
+    const double gx = interp1(P, g0, px, py, pz);
+    const double gy = interp1(P, g1, px, py, pz);
+    const double gz = interp1(P, g2, px, py, pz);
+    S.cloc[a] = phi;
+    S.dcdx[0][a] = gx;
+    S.dcdx[1][a] = gy;
+    S.dcdx[2][a] = gz;
+    double f[3] = {0.0, 0.0, 0.0};
+    if (fabs(phi) <= P.tanh2) {
+        const double gn = vnorm(gx, gy, gz);
+        if (gn != 0.0) {
+            const double r = S.rad[a];
+            double d = 0.5 * log((1.0 + phi) / (1.0 - phi)) * P.sqrt2 * P.epsilon;
+            if (P.has_ctheta) {
+                const double ct = S.cth[a];
+                if (ct != 0.0) d = d - r * ct;
+            }
+            double fm = 0.0;
+            if (fabs(d) < r) fm = -P.a_pi * P.pi * d * P.sigma;
+            f[0] = gx / gn * fm;
+            f[1] = gy / gn * fm;
+            f[2] = gz / gn * fm;
+        }
+    }
+    S.fpi[0][a] = f[0];
+    S.fpi[1][a] = f[1];
+    S.fpi[2][a] = f[2];
+}
+
+// ---------------------------------------------------------------------------
+// Stage kernel B
+// ---------------------------------------------------------------------------
+// Visit the records of global cell co in slot order (the table part, then
+// the overflow records of that cell): f(rec, ref) with ref >= 0 a table
+// slot and ref < 0 overflow entry -1-ref.
+template <class F>
+__device__ __forceinline__ void cell_visit(const CellTab &T, int co, F &&f)
+{
+    const int m = T.cnt[co];
+    const int mt = m < CELL_CAP ? m : CELL_CAP;
+    for (int e = 0; e < mt; e++) f(T.rec[(size_t)co * CELL_CAP + e], co * CELL_CAP + e);
+    if (m > CELL_CAP) {
+        const int no = *T.ovf_cnt;
+        for (int k = 0; k < no; k++) {
+            const CellRec r = T.ovf[k];
+            if (r.cell == co) f(r, -1 - k);
+        }
+    }
+}
+
+__device__ __forceinline__ CellRec cell_rec(const CellTab &T, int ref)
+{
+    return ref >= 0 ? T.rec[ref] : T.ovf[-1 - ref];
+}
+
+// Force on a (at ax, ay, az) from the partner record r; w[3] = 0.5 F.d.
+__device__ __forceinline__ void contact_rec(const PartParams &P, double ax, double ay, double az, int a,
+                                            const CellRec &r, double rmp, double *w)
+{
+    double d[3];
+    pair_disp_pos(P, ax, ay, az, r.x, r.y, r.z, a, r.id, d);
+    const double nrm = vnorm(d[0], d[1], d[2]);
+    const double sc = (P.r_eq - nrm) * rmp * P.k_n;
+    const double f0 = (d[0] / nrm) * sc, f1 = (d[1] / nrm) * sc, f2 = (d[2] / nrm) * sc;
+    w[0] = f0;
+    w[1] = f1;
+    w[2] = f2;
+    w[3] = 0.5 * ((f0 * d[0] + f1 * d[1]) + f2 * d[2]);
+}
+
+// Exact reference-order sum by repeated scans over the table (waves whose
+// partner lists overflow the fast path).
+__device__ void contact_slow_tab(const PartParams &P, const CellTab &T, int a, double ax, double ay, double az,
+                                 int o, int pp, int q, int cb, double rmp, bool tau_on, double *fo, double *tau_out)
+{
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    int last = -1;
+    for (;;) {
+        int best = 0x7fffffff;
+        CellRec br{};
+        for (int k = q - 1; k <= q + 1; k++)
+            for (int j = pp - 1; j <= pp + 1; j++)
+                for (int i = o - 1; i <= o + 1; i++) {
+                    const int co = cb + wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                    cell_visit(T, co, [&](const CellRec &r, int) {
+                        const int b = r.id;
+                        if (b >= a || b <= last || b >= best) return;
+                        double d[3];
+                        pair_disp_pos(P, ax, ay, az, r.x, r.y, r.z, a, b, d);
+                        if (vnorm(d[0], d[1], d[2]) < P.r_eq) {
+                            best = b;
+                            br = r;
+                        }
+                    });
+                }
+        if (best == 0x7fffffff) break;
+        double w[4];
+        contact_rec(P, ax, ay, az, a, br, rmp, w);
+        fx = fx + w[0];
+        fy = fy + w[1];
+        fz = fz + w[2];
+        last = best;
+    }
+    double tau = 0.0;
+    for (int k = q - 1; k <= q + 1; k++)
+        for (int j = pp - 1; j <= pp + 1; j++)
+            for (int i = o - 1; i <= o + 1; i++) {
+                const int co = cb + wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
+                // partners b > a of one cell in ascending id (list order)
+                int lastc = a;
+                for (;;) {
+                    int nb = 0x7fffffff;
+                    CellRec nr{};
+                    cell_visit(T, co, [&](const CellRec &r, int) {
+                        if (r.id > lastc && r.id < nb) {
+                            nb = r.id;
+                            nr = r;
+                        }
+                    });
+                    if (nb == 0x7fffffff) break;
+                    lastc = nb;
+                    double d[3];
+                    pair_disp_pos(P, ax, ay, az, nr.x, nr.y, nr.z, a, nb, d);
+                    if (vnorm(d[0], d[1], d[2]) < P.r_eq) {
+                        double w[4];
+                        contact_rec(P, ax, ay, az, a, nr, rmp, w);
+                        fx = fx + w[0];
+                        fy = fy + w[1];
+                        fz = fz + w[2];
+                        if (tau_on) tau += w[3];
+                    }
+                }
+            }
+    fo[0] = fx;
+    fo[1] = fy;
+    fo[2] = fz;
+    *tau_out = tau;
+}
+
+// ---------------------------------------------------------------------------
+// Stage kernel B: contacts over LDS tiles of contact cells
+// ---------------------------------------------------------------------------
+// A workgroup owns a tile of TILE_X x TILE_Y x TILE_Z contact cells (one
+// thread per cell) and stages the records of the tile and its one-cell halo
+// (periodic) in LDS, each cell's records sorted by id.  Every thread then
+// handles the particles of its cell from LDS: a bit per (neighbour cell,
+// slot) marks the partners in contact, the partners b < a are summed in
+// ascending id and the partners b > a in traversal order (neighbour cells
+// k, j, i; ids ascending in a cell), as cluster.cpp:174-275 accumulates;
+// then the Heun stage, the wrap, the next sort and (stage 2) the spread
+// lines.  A tile whose halo holds an overflowing cell or more records than
+// the LDS takes the exact global-memory path (contact_slow_tab).
+constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 2;
+constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;   // 128 cells, ~170 spheres at 20 %
+constexpr int HALO_X = TILE_X + 2, HALO_Y = TILE_Y + 2, HALO_Z = TILE_Z + 2;
+constexpr int HALO_N = HALO_X * HALO_Y * HALO_Z;       // 400 cells
+constexpr int HALO_PER = (HALO_N + 255) / 256;         // halo cells per thread
+constexpr int TILE_RECS = 1024;                         // records staged in LDS
+constexpr int HIT_WORDS = (27 * CELL_CAP + 63) / 64;    // bits per (neighbour, slot)
+static_assert(CELL_CAP == 16, "a neighbour's hit bits stay inside one 64-bit word");
+static_assert(TILE_CELLS <= 256, "a thread per tile cell");
+
+// tiles per box along each axis, and the launch grid of a slab
+__host__ __device__ inline int tiles_of(int nc, int t) { return (nc + t - 1) / t; }
+
+// (d2, d): displacement of b from a with the reference's minimum image and
+// the coincident-pair jitter; in contact when sqrt(d2) < r_eq.  The squared
+// distance is the one vnorm() takes the root of, so the test below is the
+// reference's, with the root skipped where d2 is clearly beyond r_eq^2.
+__device__ __forceinline__ bool in_contact(const PartParams &P, double ax, double ay, double az, int a,
+                                           const CellRec &r, double r_eq2_hi)
+{
+    double d[3] = {ax - r.x, ay - r.y, az - r.z};
+    const double L[3] = {P.lx, P.ly, P.lz};
+    const double th[3] = {P.dcx * (P.ncx - 2), P.dcy * (P.ncy - 2), P.dcz * (P.ncz - 2)};
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        if (d[q] > th[q]) d[q] -= L[q];
+        else if (d[q] < -th[q]) d[q] += L[q];
+    }
+    double d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+    if (d2 == 0.0) {
+        // vnorm(d) == 0: the jitter of the coincident pair (as pair_disp_pos)
+        const int b = r.id;
+        const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double jv = jitter(lo, hi, q, P.r_p);
+            d[q] = a < b ? jv : -jv;
+        }
+        d2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+    }
+    if (d2 > r_eq2_hi) return false;
+    return sqrt(d2) < P.r_eq;
+}
+
+// LDS index of neighbour n (0..26, traversal order) of halo cell hc, and
+// the first and end record of that cell
+__device__ __forceinline__ int halo_nb(int hc, int n)
+{
+    const int dz = n / 9 - 1, dy = (n / 3) % 3 - 1, dx = n % 3 - 1;
+    return hc + dx + HALO_X * (dy + HALO_Y * dz);
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S, CellTab cur, CellTab nxt, LineTab lt,
+                                                      const int *__restrict__ t_dev, double ramp_t, double rmp_host,
+                                                      int *__restrict__ overflow)
+{
+    __shared__ int hoff[HALO_N + 1];
+    __shared__ CellRec rec[TILE_RECS];
+    __shared__ int plist[TILE_RECS];   // the tile's particles: LDS record
+    __shared__ short pcell[TILE_RECS];  // and halo cell
+    __shared__ int wsum[4];
+    const int tid = threadIdx.x;
+    const int ntx = tiles_of(P.ncx, TILE_X), nty = tiles_of(P.ncy, TILE_Y), ntz = tiles_of(P.ncz, TILE_Z);
+    const int box = blockIdx.z / ntz;
+    const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y, z0 = (blockIdx.z % ntz) * TILE_Z;
+    const int cb = box * P.ncell_box;
+    const double rmp = t_dev ? ramp_dev((double)(*t_dev + 1), ramp_t) : rmp_host;
+    // slightly above r_eq^2: below it the exact root decides
+    const double r_eq2_hi = P.r_eq * P.r_eq * (1.0 + 1e-12);
+    (void)nty;
+    (void)ntx;
+    // 1. record counts of the halo cells (HALO_PER consecutive cells a thread)
+    int cntv[HALO_PER], cov[HALO_PER];
+    int over = 0;
+#pragma unroll
+    for (int q = 0; q < HALO_PER; q++) {
+        const int h = HALO_PER * tid + q;
+        cntv[q] = 0;
+        cov[q] = 0;
+        if (h < HALO_N) {
+            const int hx = h % HALO_X, hy = (h / HALO_X) % HALO_Y, hz = h / (HALO_X * HALO_Y);
+            const int gx = wrapc(x0 + hx - 1, P.ncx), gy = wrapc(y0 + hy - 1, P.ncy), gz = wrapc(z0 + hz - 1, P.ncz);
+            cov[q] = cb + gx + P.ncx * (gy + P.ncy * gz);
+            const int m = cur.cnt[cov[q]];
+            cntv[q] = m < CELL_CAP ? m : CELL_CAP;
+            if (m > CELL_CAP) over = 1;
+        }
+    }
+    const bool any_over = __syncthreads_or(over) != 0;
+    // 2. exclusive offsets over the halo (block scan of per-thread sums)
+    int mysum = 0;
+#pragma unroll
+    for (int q = 0; q < HALO_PER; q++) mysum += cntv[q];
+    const int lane = tid & 63, wid = tid >> 6;
+    const int inc = wave_incl_scan(mysum);
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    int off = inc - mysum;
+    for (int w = 0; w < wid; w++) off += wsum[w];
+    const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+#pragma unroll
+    for (int q = 0; q < HALO_PER; q++) {
+        const int h = HALO_PER * tid + q;
+        if (h < HALO_N) hoff[h] = off;
+        off += cntv[q];
+    }
+    if (tid == 0) hoff[HALO_N] = total;
+    const bool fb = any_over || total > TILE_RECS;
+    // 3. records into LDS, each cell's sorted by id (a cell holds a few)
+    __syncthreads();
+    if (!fb) {
+#pragma unroll
+        for (int q = 0; q < HALO_PER; q++) {
+            const int h = HALO_PER * tid + q;
+            if (h >= HALO_N) continue;
+            const int b0 = hoff[h], m = cntv[q];
+            for (int e = 0; e < m; e++) rec[b0 + e] = cur.rec[(size_t)cov[q] * CELL_CAP + e];
+            for (int i = b0 + 1; i < b0 + m; i++) {
+                const CellRec v = rec[i];
+                int j = i - 1;
+                while (j >= b0 && rec[j].id > v.id) {
+                    rec[j + 1] = rec[j];
+                    j--;
+                }
+                rec[j + 1] = v;
+            }
+        }
+    }
+    __syncthreads();
+    // 4. the tile's particles, one per thread: a block scan over the tile
+    // cells' counts lists them (LDS record and halo cell of each)
+    const int lx = tid % TILE_X, ly = (tid / TILE_X) % TILE_Y, lz = tid / (TILE_X * TILE_Y);
+    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    const bool cell_on = tid < TILE_CELLS && gx < P.ncx && gy < P.ncy && gz < P.ncz;
+    const int hc_t = (lx + 1) + HALO_X * ((ly + 1) + HALO_Y * (lz + 1));
+    const int co = cb + gx + P.ncx * (gy + P.ncy * gz);
+    double by = 0.0;  // argmax-y of this thread's particles after the stage
+    int bi = -1;
+    // per-particle inputs of the stage's end, loaded before the contact
+    // search so that their latency hides behind it
+    struct Pre {
+        double fpi[3], r, po[3], uo[3];
+    };
+    auto prefetch = [&](int a) {
+        Pre q;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            q.fpi[d] = S.fpi[d][a];
+            q.po[d] = STAGE == 2 ? S.po[d][a] : 0.0;
+            q.uo[d] = STAGE == 2 ? S.uo[d][a] : 0.0;
+        }
+        q.r = S.rad[a];
+        return q;
+    };
+    auto finish = [&](int a, double ax, double ay, double az, const double *fo, double tau, const Pre &q) {
+        S.fpp[0][a] = fo[0];
+        S.fpp[1][a] = fo[1];
+        S.fpp[2][a] = fo[2];
+        S.tau_i[a] = tau;
+        // Heun stage (element.cpp:156-173; uf = 0, f_ext = 0), periodic
+        // wrap (element.cpp:177-186)
+        const double r = q.r;
+        const double L[3] = {P.lx, P.ly, P.lz};
+        const double p0[3] = {ax, ay, az};
+        double np[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double fs = q.fpi[d] + 0.0 + fo[d];
+            const double v = 0.0 + fs / 6.0 / P.pi / P.mu / r;
+            double x;
+            if (STAGE == 1) {
+                S.uo[d][a] = v;
+                S.po[d][a] = p0[d];
+                x = p0[d] + v * P.dt;
+            } else {
+                S.u[d][a] = v;
+                x = q.po[d] + (q.uo[d] + v) * (0.5 * P.dt);
+            }
+            np[d] = fmod(x + L[d], L[d]);
+            S.p[d][a] = np[d];
+        }
+        // the sort of the next stage (cluster.cpp:116-169)
+        int ci, cj, ck;
+        const int cc = cell_of(P, np[0], np[1], np[2], &ci, &cj, &ck);
+        if (cc < 0) atomicOr(S.err, 1);
+        else cell_append(P, nxt, cc + cb, np[0], np[1], np[2], a);
+        argmax_take(by, bi, np[1], np[1] > 0.0 ? a : -1);
+        if (STAGE == 2 && lt.cnt) {
+            // the spread of f_pp at the new position for the next step's force
+            int l4[4];
+            if (!spread_lines(P, np[0], np[1], np[2], box, l4)) {
+                atomicOr(S.err, 2);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) line_append(lt, l4[k], a);
+            }
+        }
+    };
+    // tau counts when f_pi of the particle and of it_mark are non-zero
+    // (cluster.cpp:256-259)
+    const int mk = *S.mark;
+    const bool mark_on = mk >= 0 && vnorm(S.fpi[0][mk], S.fpi[1][mk], S.fpi[2][mk]) != 0.0;
+    auto tau_gate = [&](const Pre &q) { return mark_on && vnorm(q.fpi[0], q.fpi[1], q.fpi[2]) != 0.0; };
+    if (fb) {
+        // exact path from global memory (overflowing cells or a crowded
+        // halo), one tile cell a thread
+        if (cell_on) {
+            atomicOr(overflow, 1);
+            int o_, pp_, q_;
+            cell_visit(cur, co, [&](const CellRec &ra, int) {
+                const int a = ra.id;
+                const Pre q = prefetch(a);
+                cell_of(P, ra.x, ra.y, ra.z, &o_, &pp_, &q_);
+                double fo[3], tau = 0.0;
+                contact_slow_tab(P, cur, a, ra.x, ra.y, ra.z, o_, pp_, q_, cb, rmp, tau_gate(q), fo, &tau);
+                finish(a, ra.x, ra.y, ra.z, fo, tau, q);
+            });
+        }
+    } else {
+        const int mine = cell_on ? hoff[hc_t + 1] - hoff[hc_t] : 0;
+        const int pinc = wave_incl_scan(mine);
+        if (lane == 63) wsum[wid] = pinc;
+        __syncthreads();
+        int pb = pinc - mine;
+        for (int w = 0; w < wid; w++) pb += wsum[w];
+        const int ptot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        for (int k = 0; k < mine; k++) {
+            plist[pb + k] = hoff[hc_t] + k;
+            pcell[pb + k] = (short)hc_t;
+        }
+        __syncthreads();
+        for (int pi = tid; pi < ptot; pi += 256) {
+            const CellRec ra = rec[plist[pi]];
+            const int hc = pcell[pi];
+            const int a = ra.id;
+            const Pre q = prefetch(a);
+            const double ax = ra.x, ay = ra.y, az = ra.z;
+            uint64_t hit[HIT_WORDS];
+#pragma unroll
+            for (int w = 0; w < HIT_WORDS; w++) hit[w] = 0;
+            // 9 rows of three x-neighbouring cells, contiguous in LDS
+#pragma unroll
+            for (int row = 0; row < 9; row++) {
+                const int hm = hc + HALO_X * (row % 3 - 1 + HALO_Y * (row / 3 - 1));  // the dx = 0 cell
+                const int r0 = hoff[hm - 1], r1 = hoff[hm], r2 = hoff[hm + 1], r3 = hoff[hm + 2];
+                for (int e = r0; e < r3; e++) {
+                    const CellRec rb = rec[e];
+                    if (rb.id == a) continue;
+                    if (in_contact(P, ax, ay, az, a, rb, r_eq2_hi)) {
+                        // neighbour n = 3 row + j (j: which of the three cells)
+                        const int j = (e >= r1) + (e >= r2);
+                        const int k = e - (j == 0 ? r0 : (j == 1 ? r1 : r2));
+#pragma unroll
+                        for (int jj = 0; jj < 3; jj++)
+                            if (j == jj) {
+                                const int n = 3 * row + jj;
+                                hit[n / 4] |= 1ull << ((n % 4) * CELL_CAP + k);
+                            }
+                    }
+                }
+            }
+            auto rec_of = [&](int bit) {
+                const int n = bit / CELL_CAP;
+                return rec[hoff[halo_nb(hc, n)] + bit % CELL_CAP];
+            };
+            double fx = 0.0, fy = 0.0, fz = 0.0, tau = 0.0;
+            // partners b < a in ascending id
+            int last = -1;
+            for (;;) {
+                int best = 0x7fffffff, bbit = -1;
+#pragma unroll
+                for (int w = 0; w < HIT_WORDS; w++) {
+                    uint64_t m = hit[w];
+                    while (m) {
+                        const int bit = w * 64 + __ffsll((unsigned long long)m) - 1;
+                        m &= m - 1;
+                        const int b = rec_of(bit).id;
+                        if (b < a && b > last && b < best) {
+                            best = b;
+                            bbit = bit;
+                        }
+                    }
+                }
+                if (bbit < 0) break;
+                double wv[4];
+                contact_rec(P, ax, ay, az, a, rec_of(bbit), rmp, wv);
+                fx = fx + wv[0];
+                fy = fy + wv[1];
+                fz = fz + wv[2];
+                last = best;
+            }
+            // partners b > a in traversal order
+            const bool tau_on = tau_gate(q);
+#pragma unroll
+            for (int w = 0; w < HIT_WORDS; w++) {
+                uint64_t m = hit[w];
+                while (m) {
+                    const int bit = w * 64 + __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                    const CellRec rb = rec_of(bit);
+                    if (rb.id < a) continue;
+                    double wv[4];
+                    contact_rec(P, ax, ay, az, a, rb, rmp, wv);
+                    fx = fx + wv[0];
+                    fy = fy + wv[1];
+                    fz = fz + wv[2];
+                    if (tau_on) tau += wv[3];
+                }
+            }
+            const double fo[3] = {fx, fy, fz};
+            finish(a, ax, ay, az, fo, tau, q);
+        }
+    }
+    // argmax-y partial of this workgroup's particles (block_argmax_y takes
+    // one candidate per thread)
+    block_argmax_y(bi >= 0 ? by : -1.0, bi, nxt.ymax, nxt.yidx,
+                   (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)));
+}
+
+// workgroups of k_contact_tile for a slab
+inline dim3 contact_tile_grid(const PartParams &P)
+{
+    return dim3((unsigned)tiles_of(P.ncx, TILE_X), (unsigned)tiles_of(P.ncy, TILE_Y),
+                (unsigned)(tiles_of(P.ncz, TILE_Z) * P.nbox));
+}
+
+// ---------------------------------------------------------------------------
+// Spread nodes (cluster.cpp:344-410) per grid line
+// ---------------------------------------------------------------------------
+// The spread entries of particle a on line (yl, zl): the corners of
+// cluster.cpp:368-406 whose (y, z) is the line's, at x and x+1, with the
+// weights w = f * (1-o) * (1-p) * (1-q) etc. evaluated in that order;
+// f = (f_pp + f_ext)/dx/dy/dz, f_ext = 0.
+template <typename T>
+__device__ __forceinline__ void line_entries(const PartParams &P, double px, double py, double pz, double f0,
+                                             double f1, double f2, int yl, int zl, int *x0, int *x1, T *w0, T *w1)
+{
+    int x, y, z;
+    double o, p, q;
+    spread_corner(P, px, py, pz, &x, &y, &z, &o, &p, &q);
+    const double f[3] = {(f0 + 0.0) / P.dx / P.dy / P.dz, (f1 + 0.0) / P.dx / P.dy / P.dz,
+                         (f2 + 0.0) / P.dx / P.dy / P.dz};
+    const bool ylo = y == yl, zlo = z == zl;
+    *x0 = x;
+    *x1 = (x + 1) % P.nx;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double a, b;  // at x, at x1
+        if (ylo && zlo) {        // corners 0 (x) and 1 (x1)
+            a = f[d] * (1 - o) * (1 - p) * (1 - q);
+            b = f[d] * o * (1 - p) * (1 - q);
+        } else if (!ylo && zlo) {  // corners 6 (x) and 2 (x1)
+            a = f[d] * (1 - o) * p * (1 - q);
+            b = f[d] * o * p * (1 - q);
+        } else if (ylo) {          // corners 5 (x) and 3 (x1)
+            a = f[d] * (1 - o) * (1 - p) * q;
+            b = f[d] * o * (1 - p) * q;
+        } else {                   // corners 7 (x) and 4 (x1)
+            a = f[d] * (1 - o) * p * q;
+            b = f[d] * o * p * q;
+        }
+        w0[d] = (T)a;
+        w1[d] = (T)b;
+    }
+}
+
+constexpr int NODE_LANES = 16;                 // lanes per grid line
+constexpr int NODE_LPB = 256 / NODE_LANES;     // lines per workgroup
+constexpr int NODE_MAXE = 2 * LINE_CAP;        // entries per line on the fast path
+constexpr int NODE_REGION = 32 * NODE_LPB;     // pool nodes per workgroup (32 per line)
+constexpr int NODE_PER = NODE_MAXE / NODE_LANES;  // entries per lane
+
+// Nodes of the spread pool: every workgroup of the node kernel owns a
+// region of NODE_REGION nodes; lines that do not fit their workgroup's
+// region, and overflowing lines, take nodes from the shared tail.
+__host__ __device__ constexpr size_t spread_pool_nodes(int nlines, int n)
+{
+    return (size_t)((nlines + NODE_LPB - 1) / NODE_LPB) * NODE_REGION + 8 * (size_t)(n > 0 ? n : 1);
+}
+
+// Per line: the entries of its particles sorted by (x, id), summed per node
+// in ascending id; node_base / node_cnt locate them in the pool and the
+// line's counter is cleared.  Lines beyond LINE_CAP particles take an exact
+// repeated-selection path over the overflow list.
+template <typename T>
+__global__ __launch_bounds__(256) void k_spread_nodes(PartParams P, LineTab lt, const double *__restrict__ px,
+                                                      const double *__restrict__ py, const double *__restrict__ pz,
+                                                      const double *__restrict__ f0, const double *__restrict__ f1,
+                                                      const double *__restrict__ f2, SpreadNode<T> *__restrict__ nodes,
+                                                      int nlines)
+{
+    constexpr int W = NODE_LANES;
+    __shared__ uint64_t key[NODE_LPB][NODE_MAXE];
+    __shared__ uint64_t skey[NODE_LPB][NODE_MAXE];
+    __shared__ T sval[NODE_LPB][NODE_MAXE][3];
+    __shared__ int hc[NODE_LPB];
+    const int sub = threadIdx.x / W, lane = threadIdx.x % W;
+    const int line = blockIdx.x * NODE_LPB + sub;
+    const int shift = (int)(threadIdx.x & 63) - lane;
+    const bool on = line < nlines;
+    const int cnt = on ? lt.cnt[line] : 0;
+    const bool fast = cnt <= LINE_CAP;
+    const int lb = on ? line % P.nlines_box : 0;
+    const int yl = lb % P.ny, zl = lb / P.ny;
+    const int m = fast ? cnt : 0;
+    const int ne = 2 * m;
+    // this lane's entries (particles lane, lane + W): keys to LDS, values kept
+    uint64_t mk[NODE_PER];
+    T mv[NODE_PER][3];
+#pragma unroll
+    for (int t = 0; t < NODE_PER / 2; t++) {
+        const int k = lane + t * W;
+        if (k < m) {
+            const int a = lt.pid[(size_t)line * LINE_CAP + k];
+            int x0, x1;
+            T w0[3], w1[3];
+            line_entries<T>(P, px[a], py[a], pz[a], f0[a], f1[a], f2[a], yl, zl, &x0, &x1, w0, w1);
+            mk[2 * t] = ((uint64_t)(uint32_t)x0 << 32) | (uint32_t)a;
+            mk[2 * t + 1] = ((uint64_t)(uint32_t)x1 << 32) | (uint32_t)a;
+            key[sub][2 * k] = mk[2 * t];
+            key[sub][2 * k + 1] = mk[2 * t + 1];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                mv[2 * t][d] = w0[d];
+                mv[2 * t + 1][d] = w1[d];
+            }
+        }
+    }
+    __syncthreads();
+    // rank sort (keys are unique: a particle meets a line at two x)
+#pragma unroll
+    for (int t = 0; t < NODE_PER / 2; t++) {
+        const int k = lane + t * W;
+        if (k < m) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const uint64_t v = mk[2 * t + h];
+                int r = 0;
+                for (int j = 0; j < ne; j++) r += key[sub][j] < v ? 1 : 0;
+                skey[sub][r] = v;
+#pragma unroll
+                for (int d = 0; d < 3; d++) sval[sub][r][d] = mv[2 * t + h][d];
+            }
+        }
+    }
+    __syncthreads();
+    // nodes of each line (heads of runs of equal x), then the line's place
+    // in the workgroup's region: a prefix sum over the workgroup's lines
+    int heads = 0;
+    for (int c0 = 0; c0 < ne; c0 += W) {
+        const int i = c0 + lane;
+        const bool head = i < ne && (i == 0 || (skey[sub][i - 1] >> 32) != (skey[sub][i] >> 32));
+        heads += __popcll((__ballot(head) >> shift) & ((1ull << W) - 1ull));
+    }
+    if (lane == 0) hc[sub] = fast ? heads : 0;
+    __syncthreads();
+    if (!on) return;
+    if (fast) {
+        int off = 0;
+        for (int k = 0; k < sub; k++) off += hc[k];
+        int base;
+        if (off + heads <= NODE_REGION) base = blockIdx.x * NODE_REGION + off;
+        else {
+            const int tail = ((nlines + NODE_LPB - 1) / NODE_LPB) * NODE_REGION;
+            base = 0;
+            if (lane == 0) base = tail + atomicAdd(lt.pool_top, heads);
+            base = __shfl(base, 0, W);
+        }
+        int done = 0;
+        for (int c0 = 0; c0 < ne; c0 += W) {
+            const int i = c0 + lane;
+            const bool head = i < ne && (i == 0 || (skey[sub][i - 1] >> 32) != (skey[sub][i] >> 32));
+            const unsigned long long seg = (__ballot(head) >> shift) & ((1ull << W) - 1ull);
+            if (head) {
+                const uint32_t x = (uint32_t)(skey[sub][i] >> 32);
+                T s0 = T(0), s1 = T(0), s2 = T(0);
+                for (int j = i; j < ne && (uint32_t)(skey[sub][j] >> 32) == x; j++) {
+                    s0 += sval[sub][j][0];
+                    s1 += sval[sub][j][1];
+                    s2 += sval[sub][j][2];
+                }
+                SpreadNode<T> nd;
+                nd.x = (int)x;
+                nd.v[0] = s0;
+                nd.v[1] = s1;
+                nd.v[2] = s2;
+                nodes[base + done + __popcll(seg & ((1ull << lane) - 1ull))] = nd;
+            }
+            done += __popcll(seg);
+        }
+        if (lane == 0) {
+            lt.node_base[line] = base;
+            lt.node_cnt[line] = heads;
+            lt.cnt[line] = 0;
+        }
+        return;
+    }
+    // overflow: the first lane walks the nodes in ascending x and, per
+    // node, the particles in ascending id (table part and overflow list)
+    if (lane != 0) return;
+    const int tail = ((nlines + NODE_LPB - 1) / NODE_LPB) * NODE_REGION;
+    const int no = *lt.ovf_cnt;
+    auto visit = [&](auto &&f) {
+        for (int k = 0; k < LINE_CAP; k++) f(lt.pid[(size_t)line * LINE_CAP + k]);
+        for (int k = 0; k < no; k++)
+            if (lt.ovf[k].x == line) f(lt.ovf[k].y);
+    };
+    int nh = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        int base = 0;
+        if (pass == 1 && nh > 0) base = tail + atomicAdd(lt.pool_top, nh);
+        int k = 0;
+        int64_t last_x = -1;
+        for (;;) {
+            // next node: the smallest x above last_x
+            int64_t nxn = INT64_MAX;
+            visit([&](int a) {
+                int x0, x1;
+                T w0[3], w1[3];
+                line_entries<T>(P, px[a], py[a], pz[a], f0[a], f1[a], f2[a], yl, zl, &x0, &x1, w0, w1);
+                if (x0 > last_x && x0 < nxn) nxn = x0;
+                if (x1 > last_x && x1 < nxn) nxn = x1;
+            });
+            if (nxn == INT64_MAX) break;
+            if (pass == 1) {
+                T sm[3] = {T(0), T(0), T(0)};
+                int64_t last_id = -1;
+                for (;;) {
+                    int64_t nid = INT64_MAX;
+                    T wn[3] = {T(0), T(0), T(0)};
+                    visit([&](int a) {
+                        if (a <= last_id || a >= nid) return;
+                        int x0, x1;
+                        T w0[3], w1[3];
+                        line_entries<T>(P, px[a], py[a], pz[a], f0[a], f1[a], f2[a], yl, zl, &x0, &x1, w0, w1);
+                        if (x0 == nxn) { nid = a; wn[0] = w0[0]; wn[1] = w0[1]; wn[2] = w0[2]; }
+                        else if (x1 == nxn) { nid = a; wn[0] = w1[0]; wn[1] = w1[1]; wn[2] = w1[2]; }
+                    });
+                    if (nid == INT64_MAX) break;
+                    sm[0] += wn[0];
+                    sm[1] += wn[1];
+                    sm[2] += wn[2];
+                    last_id = nid;
+                }
+                SpreadNode<T> nd;
+                nd.x = (int)nxn;
+                nd.v[0] = sm[0];
+                nd.v[1] = sm[1];
+                nd.v[2] = sm[2];
+                nodes[base + k] = nd;
+            }
+            k++;
+            last_x = nxn;
+        }
+        if (pass == 0) nh = k;
+        else {
+            lt.node_base[line] = base;
+            lt.node_cnt[line] = nh;
+        }
+    }
+    lt.cnt[line] = 0;
+}
+
+}  // namespace fipi
