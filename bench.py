@@ -216,6 +216,54 @@ def step_bytes(p, n_particles):
     return 20 * R + 18 * H + n_particles * bp
 
 
+def precision_run(p, phi0, pos, cos_theta, steps, fixture_path):
+    """FP32-mixed against FP64 after `steps` steps from the same state
+    (outside the timed region): relative L2 of phi, relative error of the
+    interface area, largest sphere displacement, mean phi; and both
+    precisions against the FP64 oracle's checkpoints where a long-horizon
+    fixture of this workload exists (phi at seeded points, positions)."""
+    import fipi_amd
+    out = {}
+    for name, prec in (("fp64", fipi_amd.FP64), ("fp32_mixed", fipi_amd.FP32_MIXED)):
+        sim = fipi_amd.Simulation(p.replace(precision=prec))
+        sim.set_state(phi0, pos, None, cos_theta)
+        sim.step(steps)
+        out[name] = (sim.phi(), sim.particles()["p"], sim.diagnostics())
+        sim.close()
+    (a_phi, a_p, a_d), (m_phi, m_p, m_d) = out["fp64"], out["fp32_mixed"]
+    L = np.array([p.lx, p.ly, p.lz])
+    dp = m_p - a_p
+    dp -= np.round(dp / L) * L
+    res = {
+        "after_steps": steps,
+        "phi_rel_l2": float(np.linalg.norm(m_phi - a_phi) / np.linalg.norm(a_phi)),
+        "interface_area_rel_err": float(abs(m_d["interface_area"] / a_d["interface_area"] - 1.0)),
+        "max_particle_disp_err": float(np.abs(dp).max()) if dp.size else 0.0,
+        "mean_phi_abs_err": float(abs(m_d["mean_phi"] - a_d["mean_phi"])),
+    }
+    if fixture_path and os.path.exists(fixture_path):
+        fx = np.load(fixture_path)
+        if int(fx["particles"]) != pos.shape[0]:
+            return res  # a fixture of another workload
+        cks = [int(t) for t in fx["ck_steps"]]
+        pss = [int(t) for t in fx["pos_steps"]]
+        if steps in cks:
+            k = cks.index(steps)
+            idx, rphi = fx["sample_idx"], fx["ck_phi"][k]
+            vo = {"fixture": os.path.relpath(fixture_path, ROOT), "step": steps}
+            for name, (phi, pp, _) in out.items():
+                ph = phi.reshape(-1)[idx]
+                vo[f"{name}_phi_rel_l2_at_samples"] = float(np.linalg.norm(ph - rphi) / np.linalg.norm(rphi))
+                if steps in pss:
+                    rp = fx["pos"][pss.index(steps)]
+                    d = pp[fx["pos_idx"]] - rp
+                    d -= np.round(d / L) * L
+                    vo[f"{name}_positions_rel_l2"] = float(np.linalg.norm(d) / np.linalg.norm(rp))
+                    vo[f"{name}_max_particle_disp"] = float(np.abs(d).max())
+            res["vs_oracle"] = vo
+    return res
+
+
 def host_cpu():
     """CPU model name and the host cores this process may use."""
     model = "unknown"
@@ -311,6 +359,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--particles", type=int, default=100000)
     ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--precision-steps", type=int, default=1000,
+                    help="steps of the FP32-mixed vs FP64 comparison (outside the timed region)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--single", action="store_true", help="skip the other precision")
@@ -456,19 +506,14 @@ def main():
     if other_run is not None:
         f64 = main_run if prec == fipi_amd.FP64 else other_run
         mix = other_run if prec == fipi_amd.FP64 else main_run
-        a64, amx = f64["diag"]["interface_area"], mix["diag"]["interface_area"]
-        dp = mix["pos"] - f64["pos"]
-        dp -= np.round(dp / p.lx) * p.lx
         precision_cmp = {
             "fp64_steps_per_s": round(args.steps / f64["wall"], 3),
             "fp32_mixed_steps_per_s": round(args.steps / mix["wall"], 3),
             "mixed_speedup": round(f64["wall"] / mix["wall"], 3),
-            "after_steps": args.warmup + args.steps,
-            "phi_rel_l2": float(np.linalg.norm(mix["phi"] - f64["phi"]) / np.linalg.norm(f64["phi"])),
-            "interface_area_rel_err": float(abs(amx / a64 - 1.0)),
-            "max_particle_disp_err": float(np.abs(dp).max()) if dp.size else 0.0,
-            "mean_phi_abs_err": float(abs(mix["diag"]["mean_phi"] - f64["diag"]["mean_phi"])),
         }
+        if args.precision_steps > 0:
+            fixture = os.path.join(ROOT, "tests", "golden", f"long_{args.n}.npz")
+            precision_cmp.update(precision_run(p, phi0, pos, cos_theta, args.precision_steps, fixture))
 
     ens = None
     if not args.no_ensemble and args.ensemble_boxes > 0:

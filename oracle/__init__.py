@@ -45,6 +45,8 @@ class OracleParams(C.Structure):
         ("switch_ch", C.c_int),
         ("switch_ns", C.c_int),
         ("n_threads", C.c_int),
+        ("wetting", C.c_double),
+        ("fix_particles", C.c_int),
     ]
 
 
@@ -122,7 +124,7 @@ def to_struct(p) -> OracleParams:
         except (KeyError, AttributeError):
             if name == "lambda_":
                 v = get("lam")
-            elif name == "n_threads":
+            elif name in ("n_threads", "wetting", "fix_particles"):
                 v = 0
             else:
                 raise
@@ -251,9 +253,14 @@ def p_nz(p):
 class Sim:
     """Reference time stepping on the CPU (main.cpp:75-222)."""
 
-    def __init__(self, p, phi0, pos=None, radius=None, cos_theta=None, n_threads=0):
+    def __init__(self, p, phi0, pos=None, radius=None, cos_theta=None, n_threads=0, wetting=0.0,
+                 fix_particles=False):
+        """wetting, fix_particles: the calibration model of libfipi's
+        adjoint (see oracle_params); the defaults are the reference step."""
         self._s = to_struct(p)
         self._s.n_threads = n_threads
+        self._s.wetting = float(wetting)
+        self._s.fix_particles = 1 if fix_particles else 0
         self.nx, self.ny, self.nz = self._s.nx, self._s.ny, self._s.nz
         self.n = self._s.n_particles if self._s.switch_particle else 0
         phi0 = np.ascontiguousarray(phi0, dtype=np.float64).reshape(-1)

@@ -589,6 +589,7 @@ struct oracle_sim {
     int *cell_start, *ids;
     int it_mark;
     double tau;
+    double *W; /* wetting source (calibration model) or NULL */
 };
 
 static double *cfield(size_t G)
@@ -647,6 +648,25 @@ oracle_sim *oracle_create(const oracle_params *p, const double *phi0,
     s->cell_start = (int *)calloc((size_t)nc[0] * nc[1] * nc[2] + 1, sizeof(int));
     s->ids = (int *)calloc((size_t)(n > 0 ? n : 1), sizeof(int));
     s->it_mark = -1;  /* cluster.cpp:39 */
+    /* calibration model: W = -a_w sum_p cos(theta_p) S_p, the first
+     * component of the spread of (w_p, 0, 0) (cluster.cpp:344-410) */
+    if (p->wetting != 0.0 && n > 0 && cos_theta) {
+        double *w = (double *)calloc(3 * (size_t)n, sizeof(double));
+        double *o = (double *)calloc(3 * G, sizeof(double));
+        for (int i = 0; i < n; i++) w[3 * i] = -p->wetting * cos_theta[i];
+        oracle_spread(p, n, s->pos, w, o);
+        s->W = (double *)malloc(sizeof(double) * G);
+        memcpy(s->W, o, sizeof(double) * G);
+        free(w);
+        free(o);
+    }
+    /* calibration model: the contact forces of the fixed particles at the
+     * full ramp (t beyond ramp_t, primitive.cpp:122-129) */
+    if (p->fix_particles && n > 0) {
+        double tau = 0.0;
+        const int tfull = (int)ceil(p->ramp_t) + 1;
+        oracle_contact_forces(p, n, s->pos, NULL, -1, tfull, s->f_pp, &tau);
+    }
     return s;
 }
 
@@ -660,6 +680,7 @@ void oracle_destroy(oracle_sim *s)
     free(s->pos); free(s->pos_old); free(s->pu); free(s->pu_old); free(s->f_pp);
     free(s->f_pi); free(s->dcdx); free(s->uf); free(s->c_loc); free(s->rad); free(s->cth);
     free(s->cell_start); free(s->ids);
+    free(s->W);
     free(s);
 }
 
@@ -697,6 +718,7 @@ static void compute_f(oracle_sim *s)
     #pragma omp parallel for num_threads(s->nt) schedule(static)
     for (size_t it = 0; it < G; it++) {
         f[2 * it] = (pow(c[2 * it], 3.0) - c[2 * it]) * sc;
+        if (s->W) f[2 * it] += s->W[it];
         f[2 * it + 1] = 0.0;
     }
 }
@@ -930,7 +952,7 @@ int oracle_step(oracle_sim *s, int nsteps)
         }
 
         /* ---------------- Particles, main.cpp:186-219 ------------------ */
-        if (P->switch_particle && s->n > 0) {
+        if (P->switch_particle && s->n > 0 && !P->fix_particles) {
             if (P->switch_ch) {
                 fftc(s, s->c, -1);               /* :190 */
                 compute_grad(s);                 /* :191 */

@@ -66,6 +66,15 @@ typedef struct oracle_params {
     int    switch_ch;         /* head.hpp:73 */
     int    switch_ns;         /* head.hpp:74 */
     int    n_threads;         /* OpenMP threads used by the oracle */
+    /* Calibration model of libfipi's adjoint (no counterpart in the
+     * reference; 0 and 0 give the reference step):
+     *   wetting a_w: f = (lambda/eps^2)(c^3 - c) + W with
+     *   W = -a_w sum_p cos(theta_p) S_p, S_p the spread kernel of particle p
+     *   over dx^3 (cos_theta of oracle_create);
+     *   fix_particles: the particles stay where they are; their contact
+     *   forces at the full ramp are computed once and spread every step. */
+    double wetting;
+    int    fix_particles;
 } oracle_params;
 
 /* ---- FFT (replaces FFTW's fftw_execute_dft, field.cpp:438-455) -------- */

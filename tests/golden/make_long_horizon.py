@@ -22,6 +22,11 @@ phi0 = U(-0.05, 0.05), dt = 0.01, FP64.
 
 Usage: python tests/golden/make_long_horizon.py --n 128 --particles 12500
        [--steps 1000] [--every 50] [--threads 8] [--pos-sample 0]
+       [--pos-steps 500,800,900,1000]
+
+The committed fixtures were made with
+  --n 128 --particles 12500 --steps 1000 --every 50 --pos-steps 500,800,900,1000
+  --n 256 --particles 100000 --steps 1000 --every 100 --pos-steps 500,1000
 """
 from __future__ import annotations
 
@@ -70,10 +75,13 @@ def main():
                     help="store this many seeded spheres' positions (0: all)")
     ap.add_argument("--pos-every", type=int, default=0,
                     help="store positions at multiples of this step (0: every checkpoint)")
+    ap.add_argument("--pos-steps", default="",
+                    help="comma-separated checkpoint steps that store positions (overrides --pos-every)")
     ap.add_argument("--out", default=None)
     args = ap.parse_args()
 
     n, npart, steps = args.n, args.particles, args.steps
+    pos_steps_set = {int(x) for x in args.pos_steps.split(",") if x}
     p, phi0, pos, cth = workload(n, npart, steps)
     G = p.G
     L = np.array([p.lx, p.ly, p.lz])
@@ -90,6 +98,20 @@ def main():
     ck_steps, ck_phi, ck_u, ck_phin, ck_un, ck_ab_phi, ck_ab_u = [], [], [], [], [], [], []
     pos_steps, pos_store = [], []
     pnorm = np.linalg.norm(pos)
+    out = args.out or os.path.join(os.path.dirname(os.path.abspath(__file__)), f"long_{n}.npz")
+
+    def save(t_done):
+        # everything recorded through step t_done (rewritten at every
+        # position checkpoint, so a long run leaves a usable fixture)
+        np.savez_compressed(
+            out, n=n, particles=npart, steps=t_done, sample_idx=idx, pos_idx=pidx.astype(np.int32),
+            ab_max_disp=ab_maxdp[:t_done], ab_max_sphere=ab_maxid[:t_done], ab_pos_rel=ab_pos_rel[:t_done],
+            ck_steps=np.array(ck_steps, np.int32), ck_phi=np.array(ck_phi), ck_u=np.array(ck_u),
+            ck_phi_norm=np.array(ck_phin), ck_u_norm=np.array(ck_un), ck_ab_phi=np.array(ck_ab_phi),
+            ck_ab_u=np.array(ck_ab_u), pos_steps=np.array(pos_steps, np.int32), pos=np.array(pos_store),
+            pos0_norm=pnorm)
+        print("wrote", out, "through step", t_done, flush=True)
+
     t0 = time.time()
     for t in range(1, steps + 1):
         oracle.set_fft_order(0)
@@ -111,21 +133,15 @@ def main():
             ck_un.append(np.linalg.norm(ua))
             ck_ab_phi.append(np.linalg.norm(phia - phib) / np.linalg.norm(phia))
             ck_ab_u.append(np.linalg.norm(ua - ub) / np.linalg.norm(ua))
-            if args.pos_every <= 0 or t % args.pos_every == 0:
+            keep = (t in pos_steps_set) if pos_steps_set else (args.pos_every <= 0 or t % args.pos_every == 0)
+            if keep:
                 pos_steps.append(t)
                 pos_store.append(pa[pidx])
+                save(t)
             print(f"t={t} {time.time() - t0:.0f}s  A-B: phi {ck_ab_phi[-1]:.2e} u {ck_ab_u[-1]:.2e} "
                   f"pos rel {ab_pos_rel[t - 1]:.2e} max|dp| {ab_maxdp[t - 1]:.2e} (sphere {ab_maxid[t - 1]})",
                   flush=True)
-    out = args.out or os.path.join(os.path.dirname(os.path.abspath(__file__)), f"long_{n}.npz")
-    np.savez_compressed(
-        out, n=n, particles=npart, steps=steps, sample_idx=idx, pos_idx=pidx.astype(np.int32),
-        ab_max_disp=ab_maxdp, ab_max_sphere=ab_maxid, ab_pos_rel=ab_pos_rel,
-        ck_steps=np.array(ck_steps, np.int32), ck_phi=np.array(ck_phi), ck_u=np.array(ck_u),
-        ck_phi_norm=np.array(ck_phin), ck_u_norm=np.array(ck_un), ck_ab_phi=np.array(ck_ab_phi),
-        ck_ab_u=np.array(ck_ab_u), pos_steps=np.array(pos_steps, np.int32), pos=np.array(pos_store),
-        pos0_norm=pnorm)
-    print("wrote", out, flush=True)
+    save(steps)
 
 
 if __name__ == "__main__":

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import fipi_amd
+from _cases import rel_l2
 from fipi_amd import FP32_MIXED, FP64, config_params
 
 pytestmark = pytest.mark.gpu
@@ -128,4 +129,55 @@ def test_torch_autograd_harness(gpu):
     l1, gm1, gl1, gth1 = adj.gradient(1.0, 1.0, th.detach().cpu().double().numpy())
     assert float(m.grad) == gm1 and float(lam.grad) == gl1
     assert np.allclose(th.grad.cpu().double().numpy(), gth1, rtol=1e-6, atol=1e-6 * np.abs(gth1).max())
+    adj.close()
+
+
+def _oracle_loss(p, phi0, pos, theta, m, lam, target, steps):
+    import oracle
+    q = p.replace(mobility=m, lam=lam)
+    s = oracle.Sim(q, phi0, pos, None, np.cos(theta), wetting=A_W, fix_particles=True)
+    s.step(steps)
+    phi = s.phi()
+    return float(((phi - target) ** 2).sum()), phi
+
+
+def test_adjoint_forward_and_gradient_vs_oracle(gpu):
+    """The adjoint's forward model (wetting source a_w = 0.5, particles held
+    fixed with full-ramp contact forces) against the oracle's calibration
+    model, and the adjoint gradient against FP64 central differences of
+    the oracle itself (the FP64 reference gradient)."""
+    print("adjoint vs oracle calibration model", flush=True)
+    n, npart, steps, seed = 32, 40, 30, 4
+    p = config_params(n, npart, 2.0, steps, precision=FP64)
+    phi0 = fipi_amd.init_phi((n, n, n), 0.0, 0.05, seed)
+    pos = fipi_amd.init_rsa(npart, 2.0, -0.5, (n, n, n), seed + 1)
+    assert pos.shape[0] == npart
+    th_true = np.deg2rad(fipi_amd.init_uniform(npart, 70.0, 110.0, seed + 2))
+    _, target = _oracle_loss(p, phi0, pos, th_true, 1.3, 0.8, np.zeros((n, n, n)), steps)
+    adj = fipi_amd.Adjoint(p, phi0, pos, A_W, steps, 7)
+    adj.set_target(target)
+    th0 = np.deg2rad(90.0 + 5.0 * np.sin(np.arange(npart)))
+    # forward at two parameter sets against the oracle
+    for m, lam, th in ((1.0, 1.0, th0), (1.3, 0.8, th_true)):
+        g_loss, g_phi = adj.forward(m, lam, th)
+        r_loss, r_phi = _oracle_loss(p, phi0, pos, th, m, lam, target, steps)
+        assert rel_l2(g_phi, r_phi) < 1e-10, (m, lam, rel_l2(g_phi, r_phi))
+        if r_loss > 0:
+            assert abs(g_loss / r_loss - 1) < 1e-8, (g_loss, r_loss)
+    # the FP64 reference gradient: central differences of the oracle
+    loss, gm, gl, gth = adj.gradient(1.0, 1.0, th0)
+    h = 1e-4
+    f = lambda mm, ll, tt: _oracle_loss(p, phi0, pos, tt, mm, ll, target, steps)[0]
+    fm = (f(1.0 + h, 1.0, th0) - f(1.0 - h, 1.0, th0)) / (2 * h)
+    fl = (f(1.0, 1.0 + h, th0) - f(1.0, 1.0 - h, th0)) / (2 * h)
+    print(f"dM {gm:.10e} oracle fd {fm:.10e}; dlambda {gl:.10e} oracle fd {fl:.10e}", flush=True)
+    assert abs(gm - fm) <= 1e-6 * abs(fm)
+    assert abs(gl - fl) <= 1e-6 * abs(fl)
+    gs = np.abs(gth).max()
+    for i in np.argsort(-np.abs(gth))[:3]:
+        tp, tm = th0.copy(), th0.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (f(1.0, 1.0, tp) - f(1.0, 1.0, tm)) / (2 * h)
+        assert abs(gth[i] - fd) <= 1e-6 * gs, (i, gth[i], fd)
     adj.close()

@@ -1,0 +1,78 @@
+"""FP64 over 1000 steps against oracle fixtures (tests/golden/long_*.npz,
+written by tests/golden/make_long_horizon.py in this repository).
+
+The fixture holds, every 50 steps, phi and u at seeded grid points and
+their global L2 norms from the oracle, the positions of the spheres at a
+few checkpoints, and the divergence of two oracle runs that differ only in
+their FFT factor order (radix 4 + 2 against radix 2).  phi and u must agree
+at 1e-10 throughout.  Positions must agree at 1e-10 up to the step where
+the two oracle runs still agree to 1e-12; beyond it (a sphere on a
+vanishing grad c turns its capillary force with the round-off, and any two
+FP64 implementations part there) the GPU may differ from the oracle by no
+more than 100 times what the two oracle runs differ by.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import fipi_amd
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _fixture(n):
+    path = os.path.join(HERE, "golden", f"long_{n}.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"fixture {path} not generated")
+    return np.load(path)
+
+
+@pytest.mark.parametrize("n", [128, 256])
+def test_fp64_1000_steps_against_oracle_fixture(gpu, n):
+    fx = _fixture(n)
+    npart, steps = int(fx["particles"]), int(fx["steps"])
+    print(f"long horizon {n}^3 + {npart}, {steps} steps", flush=True)
+    p = fipi_amd.config_params(n, npart, 2.0, steps)
+    phi0 = fipi_amd.init_phi((n, n, n), 0.0, 0.05, 1)
+    pos = fipi_amd.init_rsa(npart, 2.0, 0.5, (p.lx, p.ly, p.lz), 2)
+    assert pos.shape[0] == npart
+    cth = np.cos(np.deg2rad(fipi_amd.init_uniform(npart, 60.0, 120.0, 3)))
+    idx, pidx = fx["sample_idx"], fx["pos_idx"]
+    ab = fx["ab_pos_rel"]
+    # last step at which the two oracle runs still agree to 1e-12
+    apart = np.nonzero(ab > 1e-12)[0]
+    horizon = int(apart[0]) if apart.size else steps
+    L = np.array([p.lx, p.ly, p.lz])
+    pos_at = {int(t): k for k, t in enumerate(fx["pos_steps"])}
+    sim = fipi_amd.Simulation(p)
+    sim.set_state(phi0, pos, None, cth)
+    t = 0
+    for k, tc in enumerate(fx["ck_steps"]):
+        sim.step(int(tc) - t)
+        t = int(tc)
+        phi = sim.phi().reshape(-1)
+        u = sim.velocity().reshape(3, -1)
+        rphi, ru = fx["ck_phi"][k], fx["ck_u"][k]
+        e_phi = np.linalg.norm(phi[idx] - rphi) / np.linalg.norm(rphi)
+        e_u = np.linalg.norm(u[:, idx] - ru) / np.linalg.norm(ru)
+        n_phi = abs(np.linalg.norm(phi) / fx["ck_phi_norm"][k] - 1)
+        n_u = abs(np.linalg.norm(u) / fx["ck_u_norm"][k] - 1)
+        msg = f"t={t}: phi {e_phi:.2e} (norm {n_phi:.1e}) u {e_u:.2e} (norm {n_u:.1e})"
+        if t in pos_at:
+            gp = sim.particles()["p"][pidx]
+            rp = fx["pos"][pos_at[t]]
+            d = gp - rp
+            d -= L * np.round(d / L)
+            e_p = np.linalg.norm(d) / np.linalg.norm(rp)
+            msg += f" positions {e_p:.2e} (two oracles {ab[t - 1]:.2e})"
+            if t <= horizon:
+                assert e_p < 1e-10, msg
+            else:
+                assert e_p < 100 * ab[t - 1] + 1e-10, msg
+        print(msg, flush=True)
+        assert e_phi < 1e-10 and e_u < 1e-10, msg
+        assert n_phi < 1e-12 and n_u < 1e-12, msg
+    sim.close()
+    print(f"oracle orders agree to 1e-12 through step {horizon}", flush=True)

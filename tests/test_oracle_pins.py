@@ -196,3 +196,36 @@ def test_snapshot_style_restart_of_the_oracle_is_a_pure_function_of_state(steps)
     b.step(steps)
     assert np.array_equal(a.phi(), b.phi())
     assert np.array_equal(a.particles()["p"], b.particles()["p"])
+
+
+def test_calibration_model_pins():
+    """The oracle's calibration model (the forward model of libfipi's
+    adjoint): zero wetting weights give the reference step bitwise; the
+    wetting source integrates to -a_w sum cos(theta) (the spread conserves
+    the weights); with particles held fixed they do not move and their
+    contact forces are the full-ramp forces of cluster.cpp:174-275."""
+    p, phi0, pos = make_case(32, 60, r_p=2.0, steps=20, ns=1, seed=6, gap=-0.5)
+    a = oracle.Sim(p, phi0, pos)
+    b = oracle.Sim(p, phi0, pos, None, np.zeros(60), wetting=0.7)
+    a.step(3)
+    b.step(3)
+    assert np.array_equal(a.phi(), b.phi())
+    # sum W dx^3 = -a_w sum cos(theta): one step of pure diffusion-free
+    # bookkeeping is not needed, the spread itself is checked
+    ct = np.cos(np.deg2rad(np.linspace(60.0, 120.0, 60)))
+    w = np.zeros((60, 3))
+    w[:, 0] = -0.7 * ct
+    W = oracle.spread(p, pos, w)[0]
+    assert abs(W.sum() * p.dx ** 3 - (-0.7 * ct.sum())) < 1e-12
+    # fixed particles: positions unchanged, f_pp at the full ramp, and the
+    # wetting source changes phi
+    c = oracle.Sim(p, phi0, pos, None, ct, wetting=0.7, fix_particles=True)
+    c.step(4)
+    got = c.particles()
+    assert np.array_equal(got["p"], pos)
+    _, f_full, _ = oracle.contact_forces(p, pos, t=int(np.ceil(p.ramp_t)) + 1)
+    assert np.array_equal(got["f_pp"], f_full)
+    assert np.abs(f_full).max() > 0
+    d = oracle.Sim(p, phi0, pos, None, ct, wetting=0.0, fix_particles=True)
+    d.step(4)
+    assert rel_l2(c.phi(), d.phi()) > 1e-6
