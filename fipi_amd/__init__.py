@@ -36,6 +36,8 @@ ABI_SYMBOLS = [
     "fipi_step", "fipi_step_async", "fipi_sync", "fipi_profile_steps",
     "fipi_fft_r2c", "fipi_fft_c2r", "fipi_cells_build", "fipi_cells_build_noalloc", "fipi_contact_pairs",
     "fipi_contact_forces", "fipi_interp", "fipi_spread",
+    "fipi_buffer_create", "fipi_buffer_destroy", "fipi_buffer_upload", "fipi_buffer_download", "fipi_field_fft",
+    "fipi_field_op", "fipi_capillary_forces", "fipi_advect",
     "fipi_snapshot_size", "fipi_snapshot_save", "fipi_snapshot_load", "fipi_tune", "fipi_get_tuning",
     "fipi_set_tuning",
     "fipi_ensemble_create", "fipi_ensemble_destroy", "fipi_ensemble_last_error", "fipi_ensemble_set_box",
@@ -141,6 +143,15 @@ def lib():
     L.fipi_fft_c2r.argtypes = [vp, vp, vp]
     L.fipi_cells_build.argtypes = [vp, vp, C.c_int32, vp, vp, ip]
     L.fipi_cells_build_noalloc.argtypes = [C.POINTER(CParams), vp, C.c_int32, vp, vp, vp, ip, vp]
+    L.fipi_buffer_create.argtypes = [vp, i64, C.POINTER(vp)]
+    L.fipi_buffer_destroy.argtypes = [vp, vp]
+    L.fipi_buffer_upload.argtypes = [vp, vp, vp, i64]
+    L.fipi_buffer_download.argtypes = [vp, vp, vp, i64]
+    L.fipi_field_fft.argtypes = [vp, vp, C.c_int32, C.c_int32]
+    L.fipi_field_op.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, C.c_int32, C.c_int32, vp, C.c_int32, C.c_int32,
+                                C.c_double]
+    L.fipi_capillary_forces.argtypes = [vp, vp, vp, vp, vp, C.c_int32, vp]
+    L.fipi_advect.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, vp, vp]
     L.fipi_contact_pairs.argtypes = [vp, vp, C.c_int32, vp, i64, C.POINTER(i64)]
     L.fipi_contact_forces.argtypes = [vp, vp, C.c_int32, C.c_int32, vp]
     L.fipi_interp.argtypes = [vp, vp, vp, C.c_int32, vp]

@@ -133,6 +133,8 @@ struct fipi_ctx {
     int tab_parts[2] = {0, 0};  // argmax partials each table holds
     LineTab lines{};
     bool tables_ready = false;
+    // scratch of the facade's complex transform (allocated on first use)
+    void *fs_re = nullptr, *fs_im = nullptr, *fs_h1 = nullptr, *fs_h2 = nullptr;
     int *bsum = nullptr;       // scan block totals
     double *ymax = nullptr;    // argmax-y block partials
     int *yidx = nullptr;
@@ -1690,6 +1692,7 @@ int tuning_in(fipi_ctx *ctx, const fipi_tuning_t *in)
 
 #include "ensemble.hpp"
 #include "adjoint.hpp"
+#include "facade_ops.hpp"
 
 // ---------------------------------------------------------------------------
 // Snapshot / restart: the whole state of a context (fields, SBDF2 history

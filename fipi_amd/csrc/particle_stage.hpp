@@ -156,6 +156,38 @@ __global__ __launch_bounds__(256) void k_spread_lines(PartParams P, const double
 }
 
 // ---------------------------------------------------------------------------
+// Shared per-particle arithmetic
+// ---------------------------------------------------------------------------
+// element::get_f_pi (element.cpp:75-97): the capillary attachment force of
+// a sphere of radius r at local phase field phi and gradient (gx, gy, gz);
+// ct = cos(contact angle) centres the spring on r ct (0: the reference)
+__device__ __forceinline__ void capillary_force(const PartParams &P, double phi, double gx, double gy, double gz,
+                                                double r, double ct, double *f)
+{
+    f[0] = f[1] = f[2] = 0.0;
+    if (fabs(phi) <= P.tanh2) {
+        const double gn = vnorm(gx, gy, gz);
+        if (gn != 0.0) {
+            double d = 0.5 * log((1.0 + phi) / (1.0 - phi)) * P.sqrt2 * P.epsilon;
+            if (ct != 0.0) d = d - r * ct;
+            double fm = 0.0;
+            if (fabs(d) < r) fm = -P.a_pi * P.pi * d * P.sigma;
+            f[0] = gx / gn * fm;
+            f[1] = gy / gn * fm;
+            f[2] = gz / gn * fm;
+        }
+    }
+}
+
+// element::advect (element.cpp:156-173; uf = 0, f_ext = 0), one component:
+// the velocity and the unwrapped new coordinate
+__device__ __forceinline__ double heun_velocity(const PartParams &P, double fpi, double fpp, double r)
+{
+    const double fs = fpi + 0.0 + fpp;
+    return 0.0 + fs / 6.0 / P.pi / P.mu / r;
+}
+
+// ---------------------------------------------------------------------------
 // Stage kernel A
 // ---------------------------------------------------------------------------
 // it_mark from nparts argmax partials: the strictly largest y > 0, first
@@ -220,23 +252,8 @@ __global__ __launch_bounds__(256) void k_stage_a(PartParams P, PartState S, cons
     S.dcdx[0][a] = gx;
     S.dcdx[1][a] = gy;
     S.dcdx[2][a] = gz;
-    double f[3] = {0.0, 0.0, 0.0};
-    if (fabs(phi) <= P.tanh2) {
-        const double gn = vnorm(gx, gy, gz);
-        if (gn != 0.0) {
-            const double r = S.rad[a];
-            double d = 0.5 * log((1.0 + phi) / (1.0 - phi)) * P.sqrt2 * P.epsilon;
-            if (P.has_ctheta) {
-                const double ct = S.cth[a];
-                if (ct != 0.0) d = d - r * ct;
-            }
-            double fm = 0.0;
-            if (fabs(d) < r) fm = -P.a_pi * P.pi * d * P.sigma;
-            f[0] = gx / gn * fm;
-            f[1] = gy / gn * fm;
-            f[2] = gz / gn * fm;
-        }
-    }
+    double f[3];
+    capillary_force(P, phi, gx, gy, gz, S.rad[a], P.has_ctheta ? S.cth[a] : 0.0, f);
     S.fpi[0][a] = f[0];
     S.fpi[1][a] = f[1];
     S.fpi[2][a] = f[2];
@@ -533,8 +550,7 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
         double np[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const double fs = q.fpi[d] + 0.0 + fo[d];
-            const double v = 0.0 + fs / 6.0 / P.pi / P.mu / r;
+            const double v = heun_velocity(P, q.fpi[d], fo[d], r);
             double x;
             if (STAGE == 1) {
                 S.uo[d][a] = v;

@@ -185,6 +185,72 @@ int fipi_interp(fipi_ctx *ctx, const void *d_grid, const double *d_pos, int32_t 
  * the contributions are summed in ascending particle order. */
 int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32_t n, void *d_grid3);
 
+/* ---- device fields of the class facade --------------------------------- */
+/* The reference's `field` holds fftw_malloc'ed complex arrays and an
+ * in_fspace flag (src/field.hpp:35-93, field.cpp:41-46); the facade
+ * (fipi_amd/facade/fipi.hpp) keeps its fields in device buffers of the
+ * context and runs the class's operators here.  A field is `dim`
+ * components (1, 3 or 9) of G = nx*ny*nz complex values (interleaved re,
+ * im), component-major.  Buffers are zero-filled; they are the caller's to
+ * destroy (fftw_free, field.cpp:458-462).  These entries need a single-box
+ * FP64 context and are synchronous. */
+int fipi_buffer_create(fipi_ctx *ctx, int64_t bytes, void **d_ptr);
+int fipi_buffer_destroy(fipi_ctx *ctx, void *d_ptr);
+int fipi_buffer_upload(fipi_ctx *ctx, void *d_dst, const void *h_src, int64_t bytes);
+int fipi_buffer_download(fipi_ctx *ctx, void *h_dst, const void *d_src, int64_t bytes);
+/* field::fft / field::ifft (field.cpp:438-455): the complex 3-D DFT of
+ * each component in place, sign -1 forward unnormalised, +1 backward
+ * normalised by 1/G (primitive.cpp:54-62), as two real transforms of the
+ * real and imaginary parts. */
+int fipi_field_fft(fipi_ctx *ctx, double *d_v, int32_t dim, int32_t sign);
+/* The field operators of src/field.cpp with their space rules (complex
+ * products when an operand is in K space, imaginary parts zeroed by
+ * real-space operators, division by zero giving zero).  In-place
+ * operators act on d_out, whose space is a_kspace:
+ *   ADD, SUB       out +=/-= b           field.cpp:525-640 (same dim, space)
+ *   ADD_S, SUB_S   out +=/-= s           :574-662 (real space only)
+ *   MUL_S, DIV_S   out *=,/= s           :711-830
+ *   MULEQ, DIVEQ   out *=,/= scalar b    :666-786 (b real space for /=)
+ * Others write d_out from the operands:
+ *   MUL, DIV       out = a * b, a / b    :946-1046 (component (d1+1)(d2+1)-1)
+ *   DOT            out = a . b           :364-405
+ *   POWER          out = sum_d a_d^s     :408-431 (real space)
+ *   GRAD           out = i k a, zero on each axis' Nyquist plane; b = the
+ *                  wave-vector field     :288-328
+ *   COPY           out = a               :836-866
+ *   RESET          out = 0               :179-193
+ *   REAL           real parts of a into d_out as doubles (dim * G)
+ *   FROM_REAL      d_a as doubles (out_dim * G) into a real-space field */
+#define FIPI_OP_ADD 0
+#define FIPI_OP_SUB 1
+#define FIPI_OP_ADD_S 2
+#define FIPI_OP_SUB_S 3
+#define FIPI_OP_MUL_S 4
+#define FIPI_OP_DIV_S 5
+#define FIPI_OP_MULEQ 6
+#define FIPI_OP_DIVEQ 7
+#define FIPI_OP_MUL 8
+#define FIPI_OP_DIV 9
+#define FIPI_OP_DOT 10
+#define FIPI_OP_POWER 11
+#define FIPI_OP_GRAD 12
+#define FIPI_OP_COPY 13
+#define FIPI_OP_RESET 14
+#define FIPI_OP_REAL 15
+#define FIPI_OP_FROM_REAL 16
+int fipi_field_op(fipi_ctx *ctx, int32_t op, double *d_out, int32_t out_dim, const double *d_a, int32_t a_dim,
+                  int32_t a_kspace, const double *d_b, int32_t b_dim, int32_t b_kspace, double s);
+/* element::get_f_pi (element.cpp:75-97) for n particles: local phase field
+ * c_loc[n], gradient dcdx[3n], radius[n] and cos(contact angle)[n] (either
+ * may be NULL: r_p, neutral wetting) -> f_pi[3n]. */
+int fipi_capillary_forces(fipi_ctx *ctx, const double *d_c_loc, const double *d_dcdx, const double *d_radius,
+                          const double *d_cos_theta, int32_t n, double *d_f_pi);
+/* element::advect (element.cpp:156-173; uf = 0, f_ext = 0) for stage 1 or
+ * 2, element::enforce_bc (element.cpp:177-186) for stage 0 (only d_pos is
+ * used); vectors [3n]. */
+int fipi_advect(fipi_ctx *ctx, int32_t stage, int32_t n, double *d_pos, double *d_pos_old, double *d_u, double *d_u_old,
+                const double *d_f_pi, const double *d_f_pp, const double *d_radius);
+
 /* ---- snapshot / restart ------------------------------------------------ */
 /* The whole state of a context (fields, the SBDF2 history and ring index,
  * the FP64 mass mode, particles, step counters) as one host buffer: a
