@@ -35,7 +35,8 @@ ABI_SYMBOLS = [
     "fipi_get_grad_phi", "fipi_get_particles", "fipi_get_time", "fipi_diagnostics",
     "fipi_step", "fipi_step_async", "fipi_sync", "fipi_profile_steps",
     "fipi_fft_r2c", "fipi_fft_c2r", "fipi_cells_build", "fipi_cells_build_noalloc", "fipi_contact_pairs",
-    "fipi_contact_forces", "fipi_interp", "fipi_spread",
+    "fipi_contact_forces", "fipi_interp", "fipi_spread", "fipi_interp2", "fipi_brownian_forces",
+    "fipi_get_particle_fluid",
     "fipi_buffer_create", "fipi_buffer_destroy", "fipi_buffer_upload", "fipi_buffer_download", "fipi_field_fft",
     "fipi_field_op", "fipi_capillary_forces", "fipi_advect",
     "fipi_snapshot_size", "fipi_snapshot_save", "fipi_snapshot_load", "fipi_tune", "fipi_get_tuning",
@@ -74,6 +75,7 @@ class CParams(C.Structure):
         ("tanh2", C.c_double),
         ("switch_particle", C.c_int32), ("switch_ch", C.c_int32), ("switch_ns", C.c_int32),
         ("precision", C.c_int32),
+        ("interp_u", C.c_int32), ("kt", C.c_double), ("noise_seed", C.c_uint64),
     ]
 
 
@@ -156,6 +158,9 @@ def lib():
     L.fipi_contact_forces.argtypes = [vp, vp, C.c_int32, C.c_int32, vp]
     L.fipi_interp.argtypes = [vp, vp, vp, C.c_int32, vp]
     L.fipi_spread.argtypes = [vp, vp, vp, C.c_int32, vp]
+    L.fipi_interp2.argtypes = [vp, vp, vp, C.c_int32, vp]
+    L.fipi_brownian_forces.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, vp]
+    L.fipi_get_particle_fluid.argtypes = [vp, vp, vp, vp]
     L.fipi_init_phi.argtypes = [i64, C.c_double, C.c_double, C.c_uint64, dp]
     L.fipi_init_rsa.argtypes = [i64, C.c_double, C.c_double, C.c_double, C.c_double,
                                 C.c_double, C.c_uint64, i64, dp]
@@ -320,6 +325,14 @@ class Simulation:
         keys = ["p", "p_old", "u", "u_old", "f_pp", "f_pi"]
         return {k: a[: self.n] for k, a in zip(keys, arrs)}
 
+    def particle_fluid(self) -> dict:
+        """uf, f_rand and f_drag = -6 pi r mu (u - uf) after the last Heun
+        stage, (n, 3) each (zero unless interp_u / kt are set)."""
+        n = max(self.n, 1)
+        arrs = [np.zeros((n, 3)) for _ in range(3)]
+        self._c(lib().fipi_get_particle_fluid(self._h, *[_dp(a) for a in arrs]))
+        return {k: a[: self.n] for k, a in zip(["uf", "f_rand", "f_drag"], arrs)}
+
     @property
     def t(self) -> int:
         v = C.c_int32()
@@ -435,6 +448,26 @@ class Simulation:
         self._sync_torch()
         self._c(lib().fipi_interp(self._h, C.c_void_p(g.data_ptr()), C.c_void_p(pos.data_ptr()), n,
                                   C.c_void_p(out.data_ptr())))
+        return out[:n].cpu().numpy()
+
+    def interp2(self, grid, pos):
+        import torch
+        p = self.params
+        dt = torch.float64 if p.precision == FP64 else torch.float32
+        g = torch.as_tensor(np.ascontiguousarray(grid)).to(dt).cuda().contiguous()
+        pos = torch.as_tensor(np.ascontiguousarray(pos, dtype=np.float64)).cuda()
+        n = pos.shape[0]
+        out = torch.zeros(max(n, 1), dtype=torch.float64, device=pos.device)
+        self._sync_torch()
+        self._c(lib().fipi_interp2(self._h, C.c_void_p(g.data_ptr()), C.c_void_p(pos.data_ptr()), n,
+                                   C.c_void_p(out.data_ptr())))
+        return out[:n].cpu().numpy()
+
+    def brownian(self, t, stage, n):
+        import torch
+        out = torch.zeros((max(n, 1), 3), dtype=torch.float64, device="cuda")
+        self._sync_torch()
+        self._c(lib().fipi_brownian_forces(self._h, t, stage, n, C.c_void_p(out.data_ptr())))
         return out[:n].cpu().numpy()
 
     def spread(self, pos, force):

@@ -707,6 +707,8 @@ int fipi_adjoint_create(const fipi_params *params, const double *phi0, const dou
         return adj_err(nullptr, FIPI_ERR_ARG, "params, phi0, steps >= 1 and checkpoint_every >= 1 are required");
     if (params->rho1 != params->rho2)
         return adj_err(nullptr, FIPI_ERR_UNSUPPORTED, "the adjoint needs rho1 == rho2 (no buoyancy contrast)");
+    if (params->interp_u != 0 || params->kt != 0.0)
+        return adj_err(nullptr, FIPI_ERR_UNSUPPORTED, "the adjoint's particles are fixed: interp_u and kt must be 0");
     const int n = params->switch_particle ? params->n_particles : 0;
     if (n > 0 && !pos) return adj_err(nullptr, FIPI_ERR_ARG, "pos is required");
     auto *a = new fipi_adjoint();

@@ -251,7 +251,7 @@ __global__ void k_advect_unit(PartParams P, int stage, double *__restrict__ p, d
             p[q] = fmod(p[q] + L[d], L[d]);
             continue;
         }
-        const double v = heun_velocity(P, fpi[q], fpp[q], rad ? rad[a] : P.r_p);
+        const double v = heun_velocity(P, fpi[q], 0.0, fpp[q], rad ? rad[a] : P.r_p, 0.0);
         if (stage == 1) {
             uo[q] = v;
             po[q] = p[q];

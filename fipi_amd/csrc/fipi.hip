@@ -563,8 +563,9 @@ struct Impl {
             ProfScope ps_(ctx, "stage_a", n * (24.0 + 32.0 * sizeof(T) + 8.0 + 24.0 + 24.0 + 24.0 + 16.0));
             hipLaunchKernelGGL((k_stage_a<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, ps, (const T *)R(ctx->c),
                                (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
-                               ctx->tab[cur], ctx->tab_parts[cur], ctx->tab[nxt], ctx->ncell, ctx->lines,
-                               spread ? 1 : 0);
+                               (const T *)R(ctx->u[0]), (const T *)R(ctx->u[1]), (const T *)R(ctx->u[2]),
+                               (const int *)ctx->t_dev, stage, ctx->tab[cur], ctx->tab_parts[cur], ctx->tab[nxt],
+                               ctx->ncell, ctx->lines, spread ? 1 : 0);
         }
         const double rmp = ramp((double)(ctx->t + 1), ctx->p.ramp_t);
         LineTab lt = ctx->lines;
@@ -810,6 +811,32 @@ __global__ void k_interp_field(PartParams P, const T *grid, const double *x, con
     out[a] = grid_cell_ok(P, x[a], y[a], z[a]) ? interp1(P, grid, x[a], y[a], z[a]) : 0.0;
 }
 
+template <typename T>
+__global__ void k_interp2_field(PartParams P, const T *grid, const double *x, const double *y, const double *z,
+                                double *out)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    out[a] = grid_cell_ok(P, x[a], y[a], z[a]) ? interp2(P, grid, x[a], y[a], z[a]) : 0.0;
+}
+
+// element::get_f_rand of n particles (AoS out)
+__global__ void k_brownian_unit(PartParams P, int t, int stage, double *out)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    for (int d = 0; d < 3; d++) out[3 * a + d] = P.brown_amp * brownian_g(P.noise_seed, t, stage, a, d, P.pi);
+}
+
+// element::get_f_drag (element.cpp:147-151): -6 pi r mu (u - uf), AoS out
+__global__ void k_drag_unit(PartParams P, PartState S, double *out, int n)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const double c = -6 * P.pi * S.rad[a] * P.mu;
+    for (int d = 0; d < 3; d++) out[3 * a + d] = c * (S.u[d][a] - S.uf[d][a]);
+}
+
 // Diagnostics partial sums: mean phi, free energy density, |grad phi|.
 template <typename T>
 __global__ __launch_bounds__(256) void k_diag_partial(const T *c, const T *g0, const T *g1, const T *g2, size_t G,
@@ -840,6 +867,8 @@ __global__ __launch_bounds__(256) void k_diag_partial(const T *c, const T *g0, c
 
 constexpr int DIAG_BLOCKS = 1024;
 
+static_assert(sizeof(fipi_params) == 224, "fipi_params layout (the bindings mirror it)");
+
 int check_params(const fipi_params *p)
 {
     if (!p) return set_err(nullptr, FIPI_ERR_ARG, "params is NULL");
@@ -854,6 +883,8 @@ int check_params(const fipi_params *p)
     if (p->n_particles >= (1 << 26)) return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "n_particles must be below 2^26");
     if (!(p->lx > 0 && p->ly > 0 && p->lz > 0 && p->dt > 0))
         return set_err(nullptr, FIPI_ERR_ARG, "box lengths and dt must be positive");
+    if (p->interp_u < 0 || p->interp_u > 2) return set_err(nullptr, FIPI_ERR_ARG, "interp_u must be 0, 1 or 2");
+    if (!(p->kt >= 0.0 && std::isfinite(p->kt))) return set_err(nullptr, FIPI_ERR_ARG, "kt must be finite and >= 0");
     if (p->switch_particle && p->n_particles > 0) {
         if (!(p->r_c > 0)) return set_err(nullptr, FIPI_ERR_ARG, "r_c must be positive");
         const double L[3] = {p->lx, p->ly, p->lz};
@@ -889,6 +920,10 @@ PartParams part_params_of(const fipi_params &p, int n, int nbox)
     pp.tanh2 = p.tanh2;
     pp.sqrt2 = std::pow(2.0, 0.5);
     pp.has_ctheta = 0;
+    // uf needs the Stokes solve (cluster.cpp:313); element.cpp:139-143
+    pp.interp_u = p.switch_ns ? p.interp_u : 0;
+    pp.brown_amp = p.kt > 0.0 ? std::sqrt(12.0 * p.pi * p.r_p * p.mu * p.kt / p.dt) : 0.0;
+    pp.noise_seed = p.noise_seed;
     pp.nbox = nbox;
     pp.ncell_box = (nc[0] > 0 && nc[1] > 0 && nc[2] > 0) ? nc[0] * nc[1] * nc[2] : 1;
     pp.nlines_box = p.ny * p.nz;
@@ -1012,6 +1047,8 @@ int setup_buffers(fipi_ctx *ctx)
         ALLOC(ps.fpi[a], sizeof(double) * n);
         ALLOC(ps.dcdx[a], sizeof(double) * n);
         ALLOC(ps.sp[a], sizeof(double) * n);
+        ALLOC(ps.uf[a], sizeof(double) * n);
+        ALLOC(ps.frand[a], sizeof(double) * n);
     }
     ALLOC(ps.rad, sizeof(double) * n);
     ALLOC(ps.cth, sizeof(double) * n);
@@ -1144,6 +1181,8 @@ static int create_ctx(const fipi_params *params, int nbox, int n_total, fipi_ctx
     int rc = check_params(params);
     if (rc != FIPI_OK) return rc;
     if (nbox < 1) return set_err(nullptr, FIPI_ERR_ARG, "nbox < 1");
+    if (nbox > 1 && (params->interp_u != 0 || params->kt != 0.0))
+        return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "interp_u and kt need a single-box context");
     if (n_total < 0 || n_total >= (1 << 26))
         return set_err(nullptr, FIPI_ERR_UNSUPPORTED, "the particle count must be below 2^26");
     fipi_ctx *ctx = new fipi_ctx();
@@ -1270,6 +1309,8 @@ static int set_state_slab(fipi_ctx *ctx, const double *phi, const double *pos, c
             HIPCHK(ctx, hipMemsetAsync(ps.fpp[a], 0, sizeof(double) * n, ctx->stream));
             HIPCHK(ctx, hipMemsetAsync(ps.fpi[a], 0, sizeof(double) * n, ctx->stream));
             HIPCHK(ctx, hipMemsetAsync(ps.dcdx[a], 0, sizeof(double) * n, ctx->stream));
+            HIPCHK(ctx, hipMemsetAsync(ps.uf[a], 0, sizeof(double) * n, ctx->stream));
+            HIPCHK(ctx, hipMemsetAsync(ps.frand[a], 0, sizeof(double) * n, ctx->stream));
         }
         HIPCHK(ctx, hipMemsetAsync(ps.tau_i, 0, sizeof(double) * n, ctx->stream));
         HIPCHK(ctx, hipMemsetAsync(ps.mark, 0xff, sizeof(int), ctx->stream));
@@ -1650,6 +1691,70 @@ int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32
     return check_flags(ctx);
 }
 
+int fipi_interp2(fipi_ctx *ctx, const void *d_grid, const double *d_pos, int32_t n, double *d_out)
+{
+    if (!ctx || !d_grid || n < 0 || (n > 0 && (!d_pos || !d_out))) return set_err(ctx, FIPI_ERR_ARG, "invalid argument");
+    TmpParts tp;
+    int rc = tp.init(ctx, d_pos, n);
+    if (rc) return rc;
+    PartParams pp = ctx->pp;
+    pp.n = n;
+    if (n > 0) {
+        if (ctx->fp64)
+            hipLaunchKernelGGL((k_interp2_field<double>), grid1(n, 256), dim3(256), 0, ctx->stream, pp,
+                               (const double *)d_grid, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], d_out);
+        else
+            hipLaunchKernelGGL((k_interp2_field<float>), grid1(n, 256), dim3(256), 0, ctx->stream, pp,
+                               (const float *)d_grid, tp.ps.p[0], tp.ps.p[1], tp.ps.p[2], d_out);
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipGetLastError());
+    return FIPI_OK;
+}
+
+int fipi_brownian_forces(fipi_ctx *ctx, int32_t t, int32_t stage, int32_t n, double *d_out)
+{
+    if (!ctx || n < 0 || (n > 0 && !d_out) || (stage != 1 && stage != 2) || t < 0)
+        return set_err(ctx, FIPI_ERR_ARG, "invalid argument");
+    if (n == 0) return FIPI_OK;
+    PartParams pp = ctx->pp;
+    pp.n = n;
+    pp.brown_amp = ctx->p.kt > 0.0 ? std::sqrt(12.0 * ctx->p.pi * ctx->p.r_p * ctx->p.mu * ctx->p.kt / ctx->p.dt) : 0.0;
+    hipLaunchKernelGGL(k_brownian_unit, grid1(n, 256), dim3(256), 0, ctx->stream, pp, (int)t, (int)stage, d_out);
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipGetLastError());
+    return FIPI_OK;
+}
+
+int fipi_get_particle_fluid(fipi_ctx *ctx, double *uf, double *f_rand, double *f_drag)
+{
+    if (!ctx) return set_err(ctx, FIPI_ERR_ARG, "ctx is NULL");
+    if (ctx->nbox != 1) return set_err(ctx, FIPI_ERR_UNSUPPORTED, "single-box contexts only");
+    const int n = ctx->n;
+    if (n == 0) return FIPI_OK;
+    if (ctx->p_pending) HIPCHK(ctx, hipStreamSynchronize(ctx->pstream));
+    const PartState &ps = ctx->ps;
+    double *tmp;
+    HIPCHK(ctx, hipMalloc((void **)&tmp, sizeof(double) * 3 * n));
+    double *out[3] = {uf, f_rand, f_drag};
+    int rc = FIPI_OK;
+    for (int w = 0; w < 3 && rc == FIPI_OK; w++) {
+        if (!out[w]) continue;
+        if (w < 2) {
+            double *const *src = w == 0 ? ps.uf : ps.frand;
+            hipLaunchKernelGGL(k_soa_to_aos, grid1(n, 256), dim3(256), 0, ctx->stream, (const double *)src[0],
+                               (const double *)src[1], (const double *)src[2], tmp, n);
+        } else {
+            hipLaunchKernelGGL(k_drag_unit, grid1(n, 256), dim3(256), 0, ctx->stream, ctx->pp, ps, tmp, n);
+        }
+        hipError_t e = hipMemcpyAsync(out[w], tmp, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = set_err(ctx, FIPI_ERR_HIP, std::string("particle fluid copy: ") + hipGetErrorString(e));
+    }
+    (void)hipFree(tmp);
+    return rc;
+}
+
 }  // extern "C"
 
 namespace {
@@ -1737,6 +1842,8 @@ std::vector<std::pair<void *, size_t>> snapshot_buffers(fipi_ctx *ctx)
             v.push_back({ps.fpp[a], D});
             v.push_back({ps.fpi[a], D});
             v.push_back({ps.dcdx[a], D});
+            if (ctx->pp.interp_u) v.push_back({ps.uf[a], D});
+            if (ctx->pp.brown_amp > 0.0) v.push_back({ps.frand[a], D});
         }
         v.push_back({ps.rad, D});
         v.push_back({ps.cth, D});

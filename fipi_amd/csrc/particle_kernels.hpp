@@ -30,6 +30,12 @@ struct PartParams {
     double pi, mu, dt, epsilon, sigma, a_pi, tanh2;
     double sqrt2;                // pow(2.0, 0.5) as the reference's sqrt override
     int has_ctheta;
+    // the reference's unused physics: uf from u (0 off, 1 interpolate,
+    // 2 interpolate2) and the Brownian force amplitude sqrt(12 pi r_p mu
+    // kt/dt) (0 off) with its draw seed
+    int interp_u;
+    double brown_amp;
+    uint64_t noise_seed;
     // slab of nbox equal boxes: contact cell and grid line indices are
     // global, box*ncell_box + cell and box*nlines_box + line; box b's
     // grid arrays start b*rs elements in
@@ -41,6 +47,7 @@ struct PartState {
     double *p[3], *po[3], *u[3], *uo[3], *fpp[3], *fpi[3];
     double *rad, *cth;
     double *cloc, *dcdx[3];
+    double *uf[3], *frand[3];    // local fluid velocity and Brownian force of the last stage
     double *tau_i;               // per-particle packing-energy contributions
     double *sp[3];               // positions in cell-sorted order (sp[d][s] = p[d][ids[s]])
     int *cell;                   // contact cell of each particle
@@ -704,6 +711,57 @@ __device__ __forceinline__ double interp1(const PartParams &P, const T *__restri
     r += m * (1.0 - n) * o * (double)v[i1 + j * sy + k1 * sz];
     r += m * n * o * (double)v[i1 + j1 * sy + k1 * sz];
     return r;
+}
+
+// field::interpolate2, field.cpp:251-281 (size 3): the reference's nodes
+// are floor+1 and floor+2 ((1-size)/2 = -1 is subtracted) with the
+// fractions scaled to m/3 + 1/3
+template <typename T>
+__device__ __forceinline__ double interp2(const PartParams &P, const T *__restrict__ v, double px, double py,
+                                          double pz)
+{
+    const int Nx = P.nx, Ny = P.ny, Nz = P.nz;
+    const int i = (int)floor(px / P.dx), j = (int)floor(py / P.dy), k = (int)floor(pz / P.dz);
+    double m = px / P.dx - floor(px / P.dx);
+    double n = py / P.dy - floor(py / P.dy);
+    double o = pz / P.dz - floor(pz / P.dz);
+    m = m / 3 + 1.0 / 3;
+    n = n / 3 + 1.0 / 3;
+    o = o / 3 + 1.0 / 3;
+    const int i0 = (i + 1 + Nx) % Nx, i1 = (i + 2) % Nx;
+    const int j0 = (j + 1 + Ny) % Ny, j1 = (j + 2) % Ny;
+    const int k0 = (k + 1 + Nz) % Nz, k1 = (k + 2) % Nz;
+    const size_t sy = (size_t)Nx, sz = (size_t)Nx * Ny;
+    double r;
+    r = (1.0 - m) * (1.0 - n) * (1.0 - o) * (double)v[i0 + j0 * sy + k0 * sz];
+    r += (1.0 - m) * n * (1.0 - o) * (double)v[i0 + j1 * sy + k0 * sz];
+    r += m * (1.0 - n) * (1.0 - o) * (double)v[i1 + j0 * sy + k0 * sz];
+    r += m * n * (1.0 - o) * (double)v[i1 + j1 * sy + k0 * sz];
+    r += (1.0 - m) * (1.0 - n) * o * (double)v[i0 + j0 * sy + k1 * sz];
+    r += (1.0 - m) * n * o * (double)v[i0 + j1 * sy + k1 * sz];
+    r += m * (1.0 - n) * o * (double)v[i1 + j0 * sy + k1 * sz];
+    r += m * n * o * (double)v[i1 + j1 * sy + k1 * sz];
+    return r;
+}
+
+// element::get_f_rand (element.cpp:109-144), one component: g =
+// sqrt(-2 log r1) cos(2 pi r2), drawn again while |g| > 5, with r1, r2 in
+// {0, .., 999}/1000 from a counter-based draw keyed by (seed, step t, Heun
+// stage, particle a, component d) in place of rand() % 1000; g = 0 after 64
+// draws (an r1 of 0 is the only rejection, probability 1e-3 a draw)
+__device__ __forceinline__ double brownian_g(uint64_t seed, int t, int stage, int a, int d, double pi)
+{
+    uint64_t key = splitmix64(seed);
+    key = splitmix64(key ^ (uint64_t)(uint32_t)t);
+    key = splitmix64(key ^ (((uint64_t)(uint32_t)(3 * (stage - 1) + d) << 32) | (uint64_t)(uint32_t)a));
+    for (int k = 0; k < 64; k++) {
+        const uint64_t u = splitmix64(key + (uint64_t)(k + 1) * 0xD1B54A32D192ED03ull);
+        const double r1 = (double)((u >> 32) % 1000u) / 1000;
+        const double r2 = (double)((u & 0xffffffffull) % 1000u) / 1000;
+        const double g = sqrt(-2 * log(r1)) * cos(2 * pi * r2);
+        if (!(fabs(g) > 5.0)) return g;
+    }
+    return 0.0;
 }
 
 __device__ __forceinline__ bool grid_cell_ok(const PartParams &P, double px, double py, double pz)

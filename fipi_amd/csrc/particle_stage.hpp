@@ -179,12 +179,14 @@ __device__ __forceinline__ void capillary_force(const PartParams &P, double phi,
     }
 }
 
-// element::advect (element.cpp:156-173; uf = 0, f_ext = 0), one component:
-// the velocity and the unwrapped new coordinate
-__device__ __forceinline__ double heun_velocity(const PartParams &P, double fpi, double fpp, double r)
+// element::advect (element.cpp:156-173), one component of the stage
+// velocity uf + (f_pi + f_ext + f_pp)/(6 pi mu r); f_ext and uf are zero in
+// the reference step
+__device__ __forceinline__ double heun_velocity(const PartParams &P, double fpi, double fext, double fpp, double r,
+                                                double uf)
 {
-    const double fs = fpi + 0.0 + fpp;
-    return 0.0 + fs / 6.0 / P.pi / P.mu / r;
+    const double fs = fpi + fext + fpp;
+    return uf + fs / 6.0 / P.pi / P.mu / r;
 }
 
 // ---------------------------------------------------------------------------
@@ -217,11 +219,15 @@ __device__ void reduce_mark(const double *__restrict__ ymax, const int *__restri
 
 // clear_lines: also reset the spread's overflow list and node pool (the
 // stage whose B appends spread lines)
+// With P.interp_u also uf from u (element.cpp:43-49 / field.cpp:251-281),
+// with P.brown_amp the Brownian force of this step and stage.
 template <typename T>
 __global__ __launch_bounds__(256) void k_stage_a(PartParams P, PartState S, const T *__restrict__ c,
                                                  const T *__restrict__ g0, const T *__restrict__ g1,
-                                                 const T *__restrict__ g2, CellTab cur, int nparts, CellTab nxt,
-                                                 int ncell, LineTab lt, int clear_lines)
+                                                 const T *__restrict__ g2, const T *__restrict__ u0,
+                                                 const T *__restrict__ u1, const T *__restrict__ u2,
+                                                 const int *__restrict__ t_dev, int stage, CellTab cur, int nparts,
+                                                 CellTab nxt, int ncell, LineTab lt, int clear_lines)
 {
     if (blockIdx.x == 0) reduce_mark(cur.ymax, cur.yidx, nparts, S.mark);
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -257,6 +263,17 @@ __global__ __launch_bounds__(256) void k_stage_a(PartParams P, PartState S, cons
     S.fpi[0][a] = f[0];
     S.fpi[1][a] = f[1];
     S.fpi[2][a] = f[2];
+    if (P.interp_u) {
+        const T *us[3] = {u0 + bx * P.rs, u1 + bx * P.rs, u2 + bx * P.rs};
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            S.uf[d][a] = P.interp_u == 1 ? interp1(P, us[d], px, py, pz) : interp2(P, us[d], px, py, pz);
+    }
+    if (P.brown_amp > 0.0) {
+        const int t = *t_dev;
+#pragma unroll
+        for (int d = 0; d < 3; d++) S.frand[d][a] = P.brown_amp * brownian_g(P.noise_seed, t, stage, a, d, P.pi);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -526,6 +543,9 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
     struct Pre {
         double fpi[3], r, po[3], uo[3];
     };
+    // uf and the Brownian force (off in the reference step) are read at the
+    // stage's end only, so that they hold no registers across the search
+    const bool with_uf = P.interp_u != 0, with_noise = P.brown_amp > 0.0;
     auto prefetch = [&](int a) {
         Pre q;
 #pragma unroll
@@ -542,15 +562,17 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
         S.fpp[1][a] = fo[1];
         S.fpp[2][a] = fo[2];
         S.tau_i[a] = tau;
-        // Heun stage (element.cpp:156-173; uf = 0, f_ext = 0), periodic
-        // wrap (element.cpp:177-186)
+        // Heun stage (element.cpp:156-173), periodic wrap
+        // (element.cpp:177-186)
         const double r = q.r;
         const double L[3] = {P.lx, P.ly, P.lz};
         const double p0[3] = {ax, ay, az};
         double np[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            const double v = heun_velocity(P, q.fpi[d], fo[d], r);
+            const double uf = with_uf ? S.uf[d][a] : 0.0;
+            const double fext = with_noise ? S.frand[d][a] : 0.0;
+            const double v = heun_velocity(P, q.fpi[d], fext, fo[d], r, uf);
             double x;
             if (STAGE == 1) {
                 S.uo[d][a] = v;

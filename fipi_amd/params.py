@@ -45,6 +45,9 @@ class Params:
     switch_ch: int = 1
     switch_ns: int = 1
     precision: int = FP64
+    interp_u: int = 0
+    kt: float = 0.0
+    noise_seed: int = 0
 
     @property
     def G(self) -> int:

@@ -65,6 +65,17 @@ typedef struct fipi_params {
     int32_t switch_ch;          /* head.hpp:73 (must be 1) */
     int32_t switch_ns;          /* head.hpp:74 */
     int32_t precision;          /* FIPI_FP64 or FIPI_FP32_MIXED */
+    /* Physics the reference carries but main() never calls; zero (a
+     * zero-initialised struct) gives the reference step.  Single-box
+     * contexts only (the ensemble and the adjoint refuse them). */
+    int32_t interp_u;           /* local fluid velocity uf of element::advect (element.cpp:162,168):
+                                   0 uf = 0 (element.cpp:40), 1 field::interpolate of u (the
+                                   commented code, element.cpp:43-49), 2 field::interpolate2
+                                   (field.cpp:251-281); needs switch_ns */
+    double kt;                  /* thermal energy, head.hpp:68; kt > 0 adds the Brownian force of
+                                   element::get_f_rand (element.cpp:109-144) in the f_ext slot of
+                                   the advection's force sum (not spread onto the fluid) */
+    uint64_t noise_seed;        /* seed of the Brownian draws (counter-based, replaces rand()) */
 } fipi_params;
 
 /* Diagnostics (printed by the reference at output steps). */
@@ -184,6 +195,26 @@ int fipi_interp(fipi_ctx *ctx, const void *d_grid, const double *d_pos, int32_t 
  * cluster::compute_f_r (cluster.cpp:344-410).  Deterministic: per grid node
  * the contributions are summed in ascending particle order. */
 int fipi_spread(fipi_ctx *ctx, const double *d_pos, const double *d_force, int32_t n, void *d_grid3);
+/* field::interpolate2 (field.cpp:251-281, kernel size 3) of a real grid
+ * (element type as fipi_fft_r2c) to n particles.  As the reference writes
+ * it, the nodes of each axis are floor(x/dx)+1 and floor(x/dx)+2 with
+ * weights 1-m', m', m' = (m+1)/3 for the fractional position m. */
+int fipi_interp2(fipi_ctx *ctx, const void *d_grid, const double *d_pos, int32_t n, double *d_out);
+/* element::get_f_rand (element.cpp:109-144) of n particles at step t
+ * (0-based) and Heun stage 1 or 2 with the context's kt and noise_seed:
+ * d_out[3n] = sqrt(12 pi r_p mu kt/dt) g, g = sqrt(-2 log r1) cos(2 pi r2)
+ * redrawn while |g| > 5, r1, r2 in {0, .., 999}/1000.  rand() % 1000 is
+ * replaced by a counter-based draw: attempt k (< 64; g = 0 after that) of
+ * component d of particle a takes
+ *   key = h(h(h(seed) ^ t) ^ ((3 (stage-1) + d) << 32 | a)),
+ *   u = h(key + (k+1) * 0xD1B54A32D192ED03),
+ *   r1 = (u >> 32) % 1000, r2 = (u mod 2^32) % 1000,
+ * h the splitmix64 finaliser.  The step draws the same values. */
+int fipi_brownian_forces(fipi_ctx *ctx, int32_t t, int32_t stage, int32_t n, double *d_out);
+/* The particle quantities of the physics above after the last Heun stage
+ * (host arrays [3n], any may be NULL): uf, f_rand, and f_drag =
+ * -6 pi r mu (u - uf) of element::get_f_drag (element.cpp:147-151). */
+int fipi_get_particle_fluid(fipi_ctx *ctx, double *uf, double *f_rand, double *f_drag);
 
 /* ---- device fields of the class facade --------------------------------- */
 /* The reference's `field` holds fftw_malloc'ed complex arrays and an

@@ -47,6 +47,9 @@ class OracleParams(C.Structure):
         ("n_threads", C.c_int),
         ("wetting", C.c_double),
         ("fix_particles", C.c_int),
+        ("interp_u", C.c_int),
+        ("kt", C.c_double),
+        ("noise_seed", C.c_ulonglong),
     ]
 
 
@@ -79,6 +82,10 @@ def lib():
                                             C.c_int, C.c_int, dp, dp]
         L.oracle_contact_forces.restype = C.c_int
         L.oracle_interp.argtypes = [C.POINTER(OracleParams), dp, C.c_int, dp, dp]
+        L.oracle_interp2.argtypes = [C.POINTER(OracleParams), dp, C.c_int, dp, dp]
+        L.oracle_brownian.argtypes = [C.POINTER(OracleParams), C.c_int, C.c_int, C.c_int, dp]
+        L.oracle_drag.argtypes = [C.POINTER(OracleParams), C.c_int, dp, dp, dp, dp]
+        L.oracle_get_particle_fluid.argtypes = [C.c_void_p, dp, dp]
         L.oracle_spread.argtypes = [C.POINTER(OracleParams), C.c_int, dp, dp, dp]
         L.oracle_f_pi.argtypes = [C.POINTER(OracleParams), C.c_int, dp, dp, dp, dp, dp]
         L.oracle_create.argtypes = [C.POINTER(OracleParams), dp, dp, dp, dp]
@@ -124,7 +131,7 @@ def to_struct(p) -> OracleParams:
         except (KeyError, AttributeError):
             if name == "lambda_":
                 v = get("lam")
-            elif name in ("n_threads", "wetting", "fix_particles"):
+            elif name in ("n_threads", "wetting", "fix_particles", "interp_u", "kt", "noise_seed"):
                 v = 0
             else:
                 raise
@@ -201,6 +208,40 @@ def interp(p, grid: np.ndarray, pos: np.ndarray) -> np.ndarray:
     pos = np.ascontiguousarray(pos, dtype=np.float64)
     out = np.zeros(pos.shape[0])
     lib().oracle_interp(C.byref(s), _dp(g), pos.shape[0], _dp(pos), _dp(out))
+    return out
+
+
+def interp2(p, grid: np.ndarray, pos: np.ndarray) -> np.ndarray:
+    """field::interpolate2 (field.cpp:251-281) of a real grid to particles."""
+    s = to_struct(p)
+    g = np.ascontiguousarray(grid, dtype=np.float64)
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    out = np.zeros(pos.shape[0])
+    lib().oracle_interp2(C.byref(s), _dp(g), pos.shape[0], _dp(pos), _dp(out))
+    return out
+
+
+def brownian(p, t: int, stage: int, n: int, kt=None, noise_seed=None) -> np.ndarray:
+    """element::get_f_rand (element.cpp:109-144) of n particles at step t,
+    Heun stage 1 or 2: (n, 3)."""
+    s = to_struct(p)
+    if kt is not None:
+        s.kt = kt
+    if noise_seed is not None:
+        s.noise_seed = noise_seed
+    out = np.zeros((n, 3))
+    lib().oracle_brownian(C.byref(s), t, stage, n, _dp(out))
+    return out
+
+
+def drag(p, u: np.ndarray, uf: np.ndarray, radius=None) -> np.ndarray:
+    """element::get_f_drag (element.cpp:147-151): -6 pi r mu (u - uf)."""
+    s = to_struct(p)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    uf = np.ascontiguousarray(uf, dtype=np.float64)
+    r = None if radius is None else np.ascontiguousarray(radius, dtype=np.float64)
+    out = np.zeros_like(u)
+    lib().oracle_drag(C.byref(s), u.shape[0], _dp(u), _dp(uf), _dp(r), _dp(out))
     return out
 
 
@@ -314,3 +355,10 @@ class Sim:
 
     def tau(self):
         return lib().oracle_get_tau(self._h)
+
+    def particle_fluid(self):
+        """uf and f_rand of the last Heun stage, (n, 3) each."""
+        n = max(self.n, 1)
+        uf, fr = np.zeros((n, 3)), np.zeros((n, 3))
+        lib().oracle_get_particle_fluid(self._h, _dp(uf), _dp(fr))
+        return {"uf": uf[: self.n], "f_rand": fr[: self.n]}

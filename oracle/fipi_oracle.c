@@ -494,6 +494,91 @@ void oracle_interp(const oracle_params *p, const double *grid, int n,
         out[it] = interp_one(p, grid, 1, pos[3 * it], pos[3 * it + 1], pos[3 * it + 2]);
 }
 
+/* field::interpolate2, field.cpp:251-281, size = 3 */
+static double interp2_one(const oracle_params *p, const double *v, size_t vstride,
+                          double px, double py, double pz)
+{
+    const int Nx = p->nx, Ny = p->ny, Nz = p->nz;
+    const int size = 3;
+    const double dx = dx_of(p, 0), dy = dx_of(p, 1), dz = dx_of(p, 2);
+    int i = (int)floor(px / dx);
+    int j = (int)floor(py / dy);
+    int k = (int)floor(pz / dz);
+    double m = px / dx - floor(px / dx);
+    double n = py / dy - floor(py / dy);
+    double o = pz / dz - floor(pz / dz);
+    m = m / size + 1.0 / size;
+    n = n / size + 1.0 / size;
+    o = o / size + 1.0 / size;
+    /* the reference's node indices: i-(1-size)/2 = i+1 (C integer division)
+     * and i+(1+size)/2 = i+2, wrapped */
+    const int i0 = (i - (1 - size) / 2 + Nx) % Nx, i1 = (i + (1 + size) / 2) % Nx;
+    const int j0 = (j - (1 - size) / 2 + Ny) % Ny, j1 = (j + (1 + size) / 2) % Ny;
+    const int k0 = (k - (1 - size) / 2 + Nz) % Nz, k1 = (k + (1 + size) / 2) % Nz;
+#define V(ii, jj, kk) v[vstride * ((size_t)(ii) + (size_t)(jj) * Nx + (size_t)(kk) * Nx * Ny)]
+    double r;
+    r  = (1.0 - m) * (1.0 - n) * (1.0 - o) * V(i0, j0, k0);
+    r += (1.0 - m) * n * (1.0 - o) * V(i0, j1, k0);
+    r += m * (1.0 - n) * (1.0 - o) * V(i1, j0, k0);
+    r += m * n * (1.0 - o) * V(i1, j1, k0);
+    r += (1.0 - m) * (1.0 - n) * o * V(i0, j0, k1);
+    r += (1.0 - m) * n * o * V(i0, j1, k1);
+    r += m * (1.0 - n) * o * V(i1, j0, k1);
+    r += m * n * o * V(i1, j1, k1);
+#undef V
+    return r;
+}
+
+void oracle_interp2(const oracle_params *p, const double *grid, int n,
+                    const double *pos, double *out)
+{
+    for (int it = 0; it < n; it++)
+        out[it] = interp2_one(p, grid, 1, pos[3 * it], pos[3 * it + 1], pos[3 * it + 2]);
+}
+
+/* element::get_f_rand, element.cpp:109-144, one component: the do-while
+ * redraws while |g| > 5.0; rand() % 1000 becomes the counter-based draw of
+ * the header */
+#define NOISE_TRIES 64
+static double brownian_g(uint64_t seed, int t, int stage, int it, int d, double pi)
+{
+    uint64_t key = splitmix64(seed);
+    key = splitmix64(key ^ (uint64_t)(uint32_t)t);
+    key = splitmix64(key ^ (((uint64_t)(uint32_t)(3 * (stage - 1) + d) << 32) | (uint64_t)(uint32_t)it));
+    for (int k = 0; k < NOISE_TRIES; k++) {
+        const uint64_t u = splitmix64(key + (uint64_t)(k + 1) * 0xD1B54A32D192ED03ull);
+        const double r1 = (double)((u >> 32) % 1000u) / 1000;
+        const double r2 = (double)((u & 0xffffffffull) % 1000u) / 1000;
+        const double g = sqrt(-2 * log(r1)) * cos(2 * pi * r2);
+        if (!(fabs(g) > 5.0)) return g;
+    }
+    return 0.0;
+}
+
+static inline double brownian_amp(const oracle_params *p)
+{
+    /* element.cpp:139-143 */
+    return sqrt(12.0 * p->pi * p->r_p * p->mu * p->kt / p->dt);
+}
+
+void oracle_brownian(const oracle_params *p, int t, int stage, int n, double *out)
+{
+    const double amp = brownian_amp(p);
+    for (int it = 0; it < n; it++)
+        for (int d = 0; d < 3; d++)
+            out[3 * it + d] = amp * brownian_g(p->noise_seed, t, stage, it, d, p->pi);
+}
+
+void oracle_drag(const oracle_params *p, int n, const double *u, const double *uf,
+                 const double *radius, double *out)
+{
+    for (int it = 0; it < n; it++) {
+        const double r = radius ? radius[it] : p->r_p;
+        const double c = -6 * p->pi * r * p->mu;
+        for (int d = 0; d < 3; d++) out[3 * it + d] = c * (u[3 * it + d] - uf[3 * it + d]);
+    }
+}
+
 /* cluster::compute_f_r, cluster.cpp:344-410; out has stride `ostride`
  * doubles per grid point and component offset `ocomp`. */
 static void spread_into(const oracle_params *p, int n, const double *pos,
@@ -585,7 +670,7 @@ struct oracle_sim {
     double *kx, *ky, *kz; /* wave numbers per index, field.cpp:108-115 */
     /* particles */
     int n;
-    double *pos, *pos_old, *pu, *pu_old, *f_pp, *f_pi, *c_loc, *dcdx, *uf, *rad, *cth;
+    double *pos, *pos_old, *pu, *pu_old, *f_pp, *f_pi, *c_loc, *dcdx, *uf, *f_rand, *rad, *cth;
     int *cell_start, *ids;
     int it_mark;
     double tau;
@@ -630,6 +715,7 @@ oracle_sim *oracle_create(const oracle_params *p, const double *phi0,
     s->f_pi = (double *)calloc(n3, sizeof(double));
     s->dcdx = (double *)calloc(n3, sizeof(double));
     s->uf = (double *)calloc(n3, sizeof(double));
+    s->f_rand = (double *)calloc(n3, sizeof(double));
     s->c_loc = (double *)calloc(n3 / 3, sizeof(double));
     s->rad = (double *)calloc(n3 / 3, sizeof(double));
     s->cth = (double *)calloc(n3 / 3, sizeof(double));
@@ -678,7 +764,7 @@ void oracle_destroy(oracle_sim *s)
     for (int d = 0; d < 3; d++) { free(s->grad[d]); free(s->u[d]); free(s->ft[d]); }
     free(s->kx); free(s->ky); free(s->kz);
     free(s->pos); free(s->pos_old); free(s->pu); free(s->pu_old); free(s->f_pp);
-    free(s->f_pi); free(s->dcdx); free(s->uf); free(s->c_loc); free(s->rad); free(s->cth);
+    free(s->f_pi); free(s->dcdx); free(s->uf); free(s->f_rand); free(s->c_loc); free(s->rad); free(s->cth);
     free(s->cell_start); free(s->ids);
     free(s->W);
     free(s);
@@ -974,10 +1060,17 @@ static void particle_stage(oracle_sim *s, int stage, int tp1, int *err)
 {
     const oracle_params *P = &s->p;
     const int n = s->n;
-    /* cluster.cpp:307-323 -> element.cpp:38-71 (uf = 0) */
+    /* cluster.cpp:307-323 -> element.cpp:38-71: uf = 0 (element.cpp:40),
+     * or u interpolated (element.cpp:43-49 with interpolate or
+     * interpolate2) when interp_u is set */
     for (int it = 0; it < n; it++) {
         const double *pp = s->pos + 3 * it;
-        s->uf[3 * it] = s->uf[3 * it + 1] = s->uf[3 * it + 2] = 0.0;
+        for (int d = 0; d < 3; d++) {
+            double v = 0.0;
+            if (P->switch_ns && P->interp_u == 1) v = interp_one(P, s->u[d], 2, pp[0], pp[1], pp[2]);
+            else if (P->switch_ns && P->interp_u == 2) v = interp2_one(P, s->u[d], 2, pp[0], pp[1], pp[2]);
+            s->uf[3 * it + d] = v;
+        }
         if (P->switch_ch) {
             s->c_loc[it] = interp_one(P, s->c, 2, pp[0], pp[1], pp[2]);
             for (int d = 0; d < 3; d++)
@@ -998,12 +1091,15 @@ static void particle_stage(oracle_sim *s, int stage, int tp1, int *err)
         if (!*err) walk_pairs(P, n, s->pos, s->cell_start, s->ids, nc, dc, &c, force_cb);
         s->tau = c.tau;
     }
-    /* compute_f_ext: f_ext = 0 (element.cpp:100-104); advect, element.cpp:156-173 */
+    /* compute_f_ext: f_ext = 0 (element.cpp:100-104), or the Brownian
+     * force when kt > 0 (element.cpp:109-144); advect, element.cpp:156-173 */
+    if (P->kt > 0.0) oracle_brownian(P, tp1 - 1, stage, n, s->f_rand);
     for (int it = 0; it < n; it++) {
         const double r = s->rad[it];
         double v[3];
         for (int a = 0; a < 3; a++) {
-            double fsum = s->f_pi[3 * it + a] + 0.0 + s->f_pp[3 * it + a];
+            const double fext = P->kt > 0.0 ? s->f_rand[3 * it + a] : 0.0;
+            double fsum = s->f_pi[3 * it + a] + fext + s->f_pp[3 * it + a];
             v[a] = s->uf[3 * it + a] + fsum / 6.0 / P->pi / P->mu / r;
         }
         if (stage == 1) {
@@ -1059,6 +1155,12 @@ void oracle_get_particles(const oracle_sim *s, double *p, double *p_old,
     if (f_pp) memcpy(f_pp, s->f_pp, b);
     if (f_pi) memcpy(f_pi, s->f_pi, b);
     if (c_loc) memcpy(c_loc, s->c_loc, b / 3);
+}
+
+void oracle_get_particle_fluid(const oracle_sim *s, double *uf, double *f_rand)
+{
+    if (uf) memcpy(uf, s->uf, sizeof(double) * 3 * (size_t)s->n);
+    if (f_rand) memcpy(f_rand, s->f_rand, sizeof(double) * 3 * (size_t)s->n);
 }
 
 double oracle_get_tau(const oracle_sim *s) { return s->tau; }

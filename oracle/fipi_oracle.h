@@ -23,11 +23,20 @@
  *   cell list / pairs     brute-force O(N^2) enumeration
  *   contact forces        two-body closed form, Newton's third law
  *   f_pi                  planar tanh profile gives d = x
+ *   Heun stages + wrap    two-body closed form with boundary crossings
+ *   step order            convection from u^n (single mode, uniform u,
+ *                         closed form); free energy decreases in pure CH
+ *   calibration model     the wetting source and fixed particles in
+ *                         closed form
+ *   oracle_interp2        closed form on affine fields (the +1 node offset
+ *                         of the reference), constants reproduced
+ *   oracle_brownian       the exact distribution of g (all 10^6 pairs
+ *                         r1, r2 enumerated), uniform draws
+ *   oracle_drag, uf       uniform u: uf = U and f_drag = -(f_pi + f_pp)
  *   full step             independent NumPy restatement (tests/golden)
- * The composite step ordering is pinned only by the line-by-line
- * restatement and the independent NumPy restatement: the reference itself
- * needs FFTW, which this image lacks, so it cannot be run ("parity
- * unpinned" for FFTW's own round-off, which no check can reach).
+ * The reference itself needs FFTW, which this image lacks, so it cannot be
+ * run ("parity unpinned" for FFTW's own round-off, which no check can
+ * reach).
  */
 #ifndef FIPI_ORACLE_H
 #define FIPI_ORACLE_H
@@ -75,6 +84,20 @@ typedef struct oracle_params {
      *   forces at the full ramp are computed once and spread every step. */
     double wetting;
     int    fix_particles;
+    /* The physics the reference carries but main() never calls (0, 0 give
+     * the reference step):
+     *   interp_u: the local fluid velocity uf of element::advect
+     *   (element.cpp:162,168) -- 0: zero (element.cpp:40, the reference),
+     *   1: field::interpolate of u (the commented code, element.cpp:43-49),
+     *   2: field::interpolate2 of u (field.cpp:251-281, the "interpolation
+     *   with wider support" that comment names);
+     *   kt > 0: the Brownian force of element::get_f_rand
+     *   (element.cpp:109-144) in the f_ext slot of the advection's force sum
+     *   (f_pi + f_rand + f_pp), drawn per step, Heun stage, particle and
+     *   component from noise_seed (oracle_brownian); it is not spread. */
+    int    interp_u;
+    double kt;
+    unsigned long long noise_seed;
 } oracle_params;
 
 /* ---- FFT (replaces FFTW's fftw_execute_dft, field.cpp:438-455) -------- */
@@ -114,6 +137,27 @@ int oracle_contact_forces(const oracle_params *p, int n, const double *pos,
 /* field::interpolate, field.cpp:223-248, of one real grid field. */
 void oracle_interp(const oracle_params *p, const double *grid, int n,
                    const double *pos, double *out);
+/* field::interpolate2, field.cpp:251-281 (kernel size 3), of a real grid
+ * to n particles.  As the reference writes it, the two nodes of each axis
+ * are floor(x/dx)+1 and floor(x/dx)+2 ((1-size)/2 = -1 is subtracted) with
+ * weights 1-m', m', m' = (m+1)/3 for the fractional position m. */
+void oracle_interp2(const oracle_params *p, const double *grid, int n,
+                    const double *pos, double *out);
+/* element::get_f_rand (element.cpp:109-144) of n particles at step t
+ * (0-based) and Heun stage 1 or 2, out[3n]: sqrt(12 pi r_p mu kt/dt) g with
+ * g = sqrt(-2 log r1) cos(2 pi r2), r1, r2 from {0, .., 999}/1000, drawn
+ * again while |g| > 5.  rand() % 1000 is replaced by a counter-based draw:
+ * attempt k (0-based, at most 64; g = 0 after that) of component d takes
+ *   key = h(h(h(seed) ^ t) ^ ((3 (stage-1) + d) << 32 | it)),
+ *   u = h(key + (k+1) * 0xD1B54A32D192ED03),
+ *   r1 = (u >> 32) % 1000, r2 = (u mod 2^32) % 1000,
+ * h the splitmix64 finaliser (x + 0x9E3779B97F4A7C15, then the two
+ * xor-shift-multiply rounds and a final xor-shift). */
+void oracle_brownian(const oracle_params *p, int t, int stage, int n, double *out);
+/* element::get_f_drag (element.cpp:147-151): -6 pi r mu (u - uf) for n
+ * particles, vectors [3n]; radius NULL: r_p. */
+void oracle_drag(const oracle_params *p, int n, const double *u, const double *uf,
+                 const double *radius, double *out);
 
 /* cluster::compute_f_r, cluster.cpp:344-410: spread (f_pp+f_ext)/dx^3
  * of n particles to a 3-component real grid out[3*G] (component-major). */
@@ -152,6 +196,9 @@ void   oracle_get_grad_c(const oracle_sim *s, double *out);
 void   oracle_get_particles(const oracle_sim *s, double *p, double *p_old,
                             double *u, double *u_old, double *f_pp,
                             double *f_pi, double *c_loc);
+/* uf and f_rand (3n each, either may be NULL) of the last Heun stage:
+ * zero unless interp_u / kt are set. */
+void oracle_get_particle_fluid(const oracle_sim *s, double *uf, double *f_rand);
 double oracle_get_tau(const oracle_sim *s);
 /* Replace u (3G real) -- lets tests start a step from an arbitrary flow. */
 void   oracle_set_u(oracle_sim *s, const double *u);

@@ -27,8 +27,9 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_sizes():
-    # 4 int32 + 21 doubles + 4 int32 (see include/fipi.h)
-    assert C.sizeof(fipi_amd.CParams) == 4 * 4 + 21 * 8 + 4 * 4
+    # 4 int32 + 21 doubles + 4 int32, interp_u and its padding, kt, noise_seed
+    # (include/fipi.h; fipi.hip asserts the same size)
+    assert C.sizeof(fipi_amd.CParams) == 4 * 4 + 21 * 8 + 4 * 4 + 8 + 8 + 8
     assert C.sizeof(fipi_amd.CDiag) == 8 + 4 * 8
     assert C.sizeof(fipi_amd.CKernelStat) == 40 + 4 + 4 + 8 + 8
 
@@ -44,6 +45,10 @@ def test_param_checks_return_codes():
     p = fipi_amd.config_params(32, 10, 2.0, 10).replace(r_c=12.0)  # 2 cells per axis
     assert L.fipi_create(C.byref(fipi_amd._to_cparams(p)), C.byref(h)) == -1
     assert L.fipi_create(None, C.byref(h)) == -1
+    p = fipi_amd.config_params(32, 10, 2.0, 10).replace(interp_u=3)
+    assert L.fipi_create(C.byref(fipi_amd._to_cparams(p)), C.byref(h)) == -1
+    p = fipi_amd.config_params(32, 10, 2.0, 10).replace(kt=-1.0)
+    assert L.fipi_create(C.byref(fipi_amd._to_cparams(p)), C.byref(h)) == -1
     assert L.fipi_step(None, 1) == -1
     assert L.fipi_destroy(None) == 0
 

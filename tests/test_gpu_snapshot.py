@@ -1,12 +1,15 @@
 """Snapshot / restart (fipi_snapshot_*): a run continued from a snapshot
 in a fresh context matches the uninterrupted run bit for bit, in both
-precisions, with particles."""
+precisions, with particles; and an FP64 run restored mid-way (inside the
+SBDF2 history and the contact-force ramp) matches the oracle run straight
+through."""
 import numpy as np
 import pytest
 
 import fipi_amd
+import oracle
 from fipi_amd import FP32_MIXED, FP64
-from _cases import make_case
+from _cases import make_case, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -30,6 +33,27 @@ def test_restart_is_exact(gpu, prec):
     assert b.t == ref_t
     assert np.array_equal(b.phi(), ref_phi)
     assert np.array_equal(b.particles()["p"], ref_p)
+    b.close()
+
+
+def test_restart_against_oracle(gpu):
+    print("snapshot restart vs oracle", flush=True)
+    p, phi0, pos = make_case(32, 200, steps=40, seed=19, gap=-0.5)
+    cth = np.cos(np.deg2rad(fipi_amd.init_uniform(200, 60.0, 120.0, 20)))
+    a = fipi_amd.Simulation(p)
+    a.set_state(phi0, pos, None, cth)
+    a.step(5)
+    snap = a.snapshot()
+    a.close()
+    b = fipi_amd.Simulation(p)
+    b.restore(snap)
+    b.step(11)
+    assert b.t == 16
+    ref = oracle.Sim(p, phi0, pos, None, cth)
+    ref.step(16)
+    assert rel_l2(b.phi(), ref.phi()) < 1e-10
+    assert rel_l2(b.velocity(), ref.u()) < 1e-10
+    assert rel_l2(b.particles()["p"], ref.particles()["p"]) < 1e-10
     b.close()
 
 
