@@ -18,6 +18,7 @@ from .params import FP32_MIXED, FP64, Params, config_params, reference_params
 __all__ = [
     "FP64", "FP32_MIXED", "Params", "config_params", "reference_params",
     "Simulation", "Ensemble", "Adjoint", "FipiError", "lib", "LIB_PATH", "init_phi", "init_rsa", "init_uniform",
+    "init_phi_device", "init_rsa_device", "init_uniform_device",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -49,6 +50,8 @@ ABI_SYMBOLS = [
     "fipi_adjoint_create", "fipi_adjoint_destroy", "fipi_adjoint_last_error", "fipi_adjoint_set_target",
     "fipi_adjoint_forward", "fipi_adjoint_gradient",
     "fipi_init_phi", "fipi_init_rsa", "fipi_init_uniform",
+    "fipi_init_phi_device", "fipi_init_uniform_device", "fipi_init_rsa_device", "fipi_init_last_error",
+    "fipi_set_state_device",
 ]
 
 
@@ -166,6 +169,14 @@ def lib():
                                 C.c_double, C.c_uint64, i64, dp]
     L.fipi_init_rsa.restype = i64
     L.fipi_init_uniform.argtypes = [i64, C.c_double, C.c_double, C.c_uint64, dp]
+    L.fipi_init_phi_device.argtypes = [i64, C.c_double, C.c_double, C.c_uint64, vp, vp]
+    L.fipi_init_uniform_device.argtypes = [i64, C.c_double, C.c_double, C.c_uint64, vp, vp]
+    L.fipi_init_rsa_device.argtypes = [i64, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_uint64, i64, vp, vp]
+    L.fipi_init_rsa_device.restype = i64
+    L.fipi_init_last_error.argtypes = []
+    L.fipi_init_last_error.restype = C.c_char_p
+    L.fipi_set_state_device.argtypes = [vp, vp, vp, vp, vp]
     L.fipi_snapshot_size.argtypes = [vp]
     L.fipi_snapshot_size.restype = i64
     L.fipi_snapshot_save.argtypes = [vp, vp, i64]
@@ -234,6 +245,40 @@ def init_uniform(n: int, lo: float, hi: float, seed: int) -> np.ndarray:
     return out
 
 
+def _dev_init_check(rc):
+    if rc < 0:
+        raise FipiError(f"{ERRORS.get(int(rc), rc)}: {lib().fipi_init_last_error().decode()}")
+    return rc
+
+
+def init_phi_device(shape, mean: float, amp: float, seed: int):
+    """init_phi's values generated on the GPU (a float64 CUDA tensor)."""
+    import torch
+    out = torch.empty(int(np.prod(shape)), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    _dev_init_check(lib().fipi_init_phi_device(out.numel(), mean, amp, seed, C.c_void_p(out.data_ptr()), None))
+    return out.reshape(tuple(shape))
+
+
+def init_uniform_device(n: int, lo: float, hi: float, seed: int):
+    import torch
+    out = torch.empty(max(n, 1), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    _dev_init_check(lib().fipi_init_uniform_device(n, lo, hi, seed, C.c_void_p(out.data_ptr()), None))
+    return out[:n]
+
+
+def init_rsa_device(n: int, r: float, gap: float, box, seed: int, max_attempts: int = 100000):
+    """init_rsa's positions generated on the GPU: an (placed, 3) float64
+    CUDA tensor."""
+    import torch
+    out = torch.zeros((max(n, 1), 3), dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    placed = _dev_init_check(lib().fipi_init_rsa_device(n, r, gap, box[0], box[1], box[2], seed, max_attempts,
+                                                        C.c_void_p(out.data_ptr()), None))
+    return out[:placed]
+
+
 class Simulation:
     """One periodic box on the current GPU (a fipi_ctx)."""
 
@@ -272,6 +317,21 @@ class Simulation:
         r = None if radius is None else np.ascontiguousarray(radius, dtype=np.float64)
         ct = None if cos_theta is None else np.ascontiguousarray(cos_theta, dtype=np.float64)
         self._c(lib().fipi_set_state(self._h, _dp(phi), _dp(pos) if self.n else None, _dp(r), _dp(ct)))
+
+    def set_state_device(self, phi, pos=None, radius=None, cos_theta=None):
+        """set_state from float64 CUDA tensors (no host copy of the state)."""
+        import torch
+        p = self.params
+        ptr = lambda t: None if t is None else C.c_void_p(t.data_ptr())
+        phi = phi.contiguous()
+        assert phi.dtype == torch.float64 and phi.numel() == p.G
+        if self.n:
+            pos = pos.contiguous()
+            assert pos.dtype == torch.float64 and tuple(pos.shape) == (self.n, 3)
+        r = None if radius is None else radius.contiguous()
+        ct = None if cos_theta is None else cos_theta.contiguous()
+        torch.cuda.synchronize()
+        self._c(lib().fipi_set_state_device(self._h, ptr(phi), ptr(pos) if self.n else None, ptr(r), ptr(ct)))
 
     def set_velocity(self, u):
         u = np.ascontiguousarray(u, dtype=np.float64)

@@ -138,7 +138,7 @@ struct fipi_ctx {
     int *bsum = nullptr;       // scan block totals
     double *ymax = nullptr;    // argmax-y block partials
     int *yidx = nullptr;
-    int *d_flags = nullptr;  // [0] error bits, [1] contact-order overflow
+    int *d_flags = nullptr;  // [0] error bits, [1] contact-order overflow, [3] scratch flag of set_state
     int *h_flags = nullptr;  // pinned
     double *d_red = nullptr; // reduction scratch
     // FP32-mixed: FP64 k = 0 mode [c(0) ring of 2, history(0)], conv(0)
@@ -1340,12 +1340,115 @@ static int set_state_slab(fipi_ctx *ctx, const double *phi, const double *pos, c
     return FIPI_OK;
 }
 
+// set_state_slab for one box with device inputs (phi: G doubles; pos: 3n
+// AoS; radius / cos_theta: n doubles or NULL).  The state equals what
+// set_state_slab makes from the same values on the host.
+__global__ void k_fill_double(double *a, int n, double v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_any_nonzero(const double *a, int n, int *flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && a[i] != 0.0) atomicOr(flag, 1);
+}
+
+static int copy_in_real_dev(fipi_ctx *ctx, void *dst, const double *d_src, size_t n)
+{
+    if (ctx->fp64) {
+        HIPCHK(ctx, hipMemcpyAsync(dst, d_src, n * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        return FIPI_OK;
+    }
+    hipLaunchKernelGGL((k_from_double<float>), grid1(n, 256), dim3(256), 0, ctx->stream, d_src, (float *)dst, n);
+    HIPCHK(ctx, hipGetLastError());
+    return FIPI_OK;
+}
+
+static int set_state_dev(fipi_ctx *ctx, const double *d_phi, const double *d_pos, const double *d_radius,
+                         const double *d_cth)
+{
+    const size_t G = ctx->G;
+    int rc;
+    if ((rc = copy_in_real_dev(ctx, ctx->c, d_phi, G))) return rc;
+    for (int a = 0; a < 3; a++) HIPCHK(ctx, hipMemsetAsync(ctx->u[a], 0, G * ctx->esz, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->hist, 0, ctx->GH * 2 * ctx->esz, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->d_flags, 0, sizeof(int) * 4, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(ctx->t_dev, 0, sizeof(int), ctx->stream));
+    ctx->t = 0;
+    ctx->cur = 0;
+    ctx->conv_ready = false;
+    ctx->spread_ready = false;
+    ctx->tables_ready = false;
+    {
+        // the FP64 k = 0 mode (read by the FP32-mixed path): the same long
+        // double host sum as set_state_slab, from one copy of phi
+        std::vector<double> k0h((size_t)K0_STRIDE, 0.0);
+        std::vector<double> h(G);
+        HIPCHK(ctx, hipMemcpyAsync(h.data(), d_phi, G * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        long double sum = 0.0L;
+        for (size_t i = 0; i < G; i++) sum += (long double)h[i];
+        k0h[0] = (double)sum;
+        HIPCHK(ctx, hipMemcpyAsync(ctx->k0, k0h.data(), sizeof(double) * k0h.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        HIPCHK(ctx, hipMemsetAsync(ctx->conv0, 0, sizeof(double) * 2, ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    if (ctx->fp64) Impl<double>::init_fields(ctx);
+    else Impl<float>::init_fields(ctx);
+    if ((rc = copy_in_real_dev(ctx, ctx->c, d_phi, G))) return rc;
+    const int n = ctx->n;
+    if (n > 0) {
+        PartState &ps = ctx->ps;
+        hipStream_t st = ctx->stream;
+        hipLaunchKernelGGL(k_aos_to_soa, grid1(n, 256), dim3(256), 0, st, d_pos, ps.p[0], ps.p[1], ps.p[2], n);
+        hipLaunchKernelGGL(k_aos_to_soa, grid1(n, 256), dim3(256), 0, st, d_pos, ps.po[0], ps.po[1], ps.po[2], n);
+        for (int a = 0; a < 3; a++)
+            for (double *z : {ps.u[a], ps.uo[a], ps.fpp[a], ps.fpi[a], ps.dcdx[a], ps.uf[a], ps.frand[a]})
+                HIPCHK(ctx, hipMemsetAsync(z, 0, sizeof(double) * n, st));
+        HIPCHK(ctx, hipMemsetAsync(ps.tau_i, 0, sizeof(double) * n, st));
+        HIPCHK(ctx, hipMemsetAsync(ps.mark, 0xff, sizeof(int), st));
+        if (d_radius) HIPCHK(ctx, hipMemcpyAsync(ps.rad, d_radius, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        else hipLaunchKernelGGL(k_fill_double, grid1(n, 256), dim3(256), 0, st, ps.rad, n, ctx->p.r_p);
+        hipLaunchKernelGGL(k_fill_double, grid1(n, 256), dim3(256), 0, st, ps.cloc, n, 1.0);  // element.cpp:34
+        ctx->pp.has_ctheta = 0;
+        if (d_cth) {
+            HIPCHK(ctx, hipMemcpyAsync(ps.cth, d_cth, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+            HIPCHK(ctx, hipMemsetAsync(ctx->d_flags + 3, 0, sizeof(int), st));
+            hipLaunchKernelGGL(k_any_nonzero, grid1(n, 256), dim3(256), 0, st, (const double *)ps.cth, n,
+                               ctx->d_flags + 3);
+            int any = 0;
+            HIPCHK(ctx, hipMemcpyAsync(&any, ctx->d_flags + 3, sizeof(int), hipMemcpyDeviceToHost, st));
+            HIPCHK(ctx, hipStreamSynchronize(st));
+            HIPCHK(ctx, hipMemsetAsync(ctx->d_flags + 3, 0, sizeof(int), st));
+            ctx->pp.has_ctheta = any ? 1 : 0;
+        } else {
+            hipLaunchKernelGGL(k_fill_double, grid1(n, 256), dim3(256), 0, st, ps.cth, n, 0.0);
+        }
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipGetLastError());
+    ctx->has_state = true;
+    return FIPI_OK;
+}
+
 extern "C" {
 
 int fipi_set_state(fipi_ctx *ctx, const double *phi, const double *pos, const double *radius,
                    const double *cos_theta)
 {
     return set_state_slab(ctx, phi, pos, radius, cos_theta, nullptr);
+}
+
+int fipi_set_state_device(fipi_ctx *ctx, const double *d_phi, const double *d_pos, const double *d_radius,
+                          const double *d_cos_theta)
+{
+    if (!ctx || !d_phi) return set_err(ctx, FIPI_ERR_ARG, "ctx and phi are required");
+    if (ctx->nbox != 1) return set_err(ctx, FIPI_ERR_UNSUPPORTED, "single-box contexts only");
+    if (ctx->n > 0 && !d_pos) return set_err(ctx, FIPI_ERR_ARG, "pos is required when particles are on");
+    return set_state_dev(ctx, d_phi, d_pos, d_radius, d_cos_theta);
 }
 
 int fipi_set_velocity(fipi_ctx *ctx, const double *u)
@@ -1798,6 +1901,7 @@ int tuning_in(fipi_ctx *ctx, const fipi_tuning_t *in)
 #include "ensemble.hpp"
 #include "adjoint.hpp"
 #include "facade_ops.hpp"
+#include "init_device.hpp"
 
 // ---------------------------------------------------------------------------
 // Snapshot / restart: the whole state of a context (fields, SBDF2 history

@@ -396,6 +396,26 @@ int64_t fipi_init_rsa(int64_t n, double r, double gap, double lx, double ly, dou
 /* out[i] = U(lo, hi) from a counter-based generator (e.g. contact angles). */
 int fipi_init_uniform(int64_t n, double lo, double hi, uint64_t seed, double *out);
 
+/* ---- the same seeded inputs generated on the GPU ------------------------ */
+/* Device versions of the three generators above: the same values, written
+ * to device memory on `stream` (a hipStream_t, NULL = the null stream);
+ * they return when the values are written.  They replace the reference's
+ * unseeded initialisation (main.cpp:24, field.cpp:85-99, cluster.cpp:53-113)
+ * for states too large to build on the host.  fipi_init_rsa_device places
+ * the spheres by chunks of candidates checked in parallel and returns the
+ * same positions as fipi_init_rsa (or a negative status; the message is
+ * fipi_init_last_error()). */
+int fipi_init_phi_device(int64_t g, double mean, double amp, uint64_t seed, double *d_phi, void *stream);
+int fipi_init_uniform_device(int64_t n, double lo, double hi, uint64_t seed, double *d_out, void *stream);
+int64_t fipi_init_rsa_device(int64_t n, double r, double gap, double lx, double ly, double lz, uint64_t seed,
+                             int64_t max_attempts, double *d_pos, void *stream);
+const char *fipi_init_last_error(void);
+/* fipi_set_state from device arrays (phi: G doubles, pos: 3n, radius and
+ * cos_theta: n doubles or NULL); the state equals what fipi_set_state makes
+ * from the same values.  Single-box contexts. */
+int fipi_set_state_device(fipi_ctx *ctx, const double *d_phi, const double *d_pos, const double *d_radius,
+                          const double *d_cos_theta);
+
 #ifdef __cplusplus
 }
 #endif
