@@ -547,13 +547,44 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_cgrad_force(
 // y and z passes: a tile of TL neighbouring kx columns; lane = column,
 // t = position in the line (distribution D along y or z).
 // ---------------------------------------------------------------------------
+// Column tile and row (y for the z passes, z for the y passes) of a
+// workgroup.  Tiles narrower than 128 bytes share each 128-byte line of a
+// row with their neighbour; the GPU deals workgroups to its 8 XCDs (each
+// with its own L2) round-robin in launch order, so with such tiles the grid
+// has an even number of tiles per row and workgroups 16q + r and 16q + 8 + r
+// -- the same XCD, one after the other there -- take the two tiles of one
+// line: the line is fetched once.  Needs gridDim.x * gridDim.y to be a
+// multiple of 16 (every supported edge length is a multiple of 16).
+template <bool PAIR>
+__device__ __forceinline__ void col_tile(int &tile, int &row)
+{
+    if constexpr (!PAIR) {
+        tile = blockIdx.x;
+        row = blockIdx.y;
+    } else {
+        const int np = (int)gridDim.x >> 1;
+        const int l = (int)(blockIdx.x + gridDim.x * blockIdx.y);
+        const int pr = 8 * (l >> 4) + (l & 7);
+        tile = 2 * (pr % np) + ((l >> 3) & 1);
+        row = pr / np;
+    }
+}
+
+// tiles of TL columns per row for nxh columns (even when they pair)
+__host__ __device__ constexpr int col_tiles(int nxh, int TL, int bytes)
+{
+    return bytes < 128 ? ((nxh + TL - 1) / TL + 1) / 2 * 2 : (nxh + TL - 1) / TL;
+}
+
 #define FIPI_TILE_SETUP_E(T, N, TLV, EV)                             \
     constexpr int E = Line<T, N, EV>::E, P = Line<T, N, EV>::P, LL = Line<T, N, EV>::LL; \
     constexpr int TL = TLV;                                           \
     using C = cplx<T>;                                                \
     __shared__ C lds[TL * LL];                                        \
     const int lane = threadIdx.x % TL, t = threadIdx.x / TL;          \
-    const int kx = blockIdx.x * TL + lane;                            \
+    int tile_, row_;                                                  \
+    col_tile<(TL * (int)sizeof(C) < 128)>(tile_, row_);               \
+    const int kx = tile_ * TL + lane;                                 \
     const bool ok = kx < d.nxh;                                       \
     const size_t zb = blockIdx.z;                                     \
     C *L = lds + lane;                                                \
@@ -628,7 +659,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N, EY>::P)) void k_y(cpl
     a1 = boxp(a1, zb, d.hs);
     a2 = boxp(a2, zb, d.hs);
     extra = boxp(extra, zb, d.hs);
-    const int z = blockIdx.y;
+    const int z = row_;
     const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, 0, z);
     const uint32_t st = d.nxp;
     C v[NF][E];
@@ -712,7 +743,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_ch(
     c_next = boxp(c_next, zb, d.hs);
     f_now = boxp(f_now, zb, d.hs);
     hist = boxp(hist, zb, d.hs);
-    const int y = blockIdx.y;
+    const int y = row_;
     const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
     const uint32_t st = (uint32_t)d.nxp * d.ny;
     C v[E], cn[E], fn[E], hh[E];
@@ -830,7 +861,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_init(
     f_io = boxp(f_io, zb, d.hs);
     c_k = boxp(c_k, zb, d.hs);
     f_k = boxp(f_k, zb, d.hs);
-    const int y = blockIdx.y;
+    const int y = row_;
     const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
     const uint32_t st = (uint32_t)d.nxp * d.ny;
     C fv[E], v[E], w[E];
@@ -879,7 +910,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_stoke
     a1 = boxp(a1, zb, d.hs);
     a2 = boxp(a2, zb, d.hs);
     fz = boxp(fz, zb, d.hs);
-    const int y = blockIdx.y;
+    const int y = row_;
     const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
     const uint32_t st = (uint32_t)d.nxp * d.ny;
     C v[3][E];
@@ -955,7 +986,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_z(cplx<T>
 {
     FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
     a = boxp(a, zb, d.hs);
-    const int y = blockIdx.y;
+    const int y = row_;
     const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
     const uint32_t st = (uint32_t)d.nxp * d.ny;
     C v[E];

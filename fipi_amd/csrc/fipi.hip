@@ -347,7 +347,7 @@ struct Impl {
                 auto launch = [&](auto EC) {
                     constexpr int EY = decltype(EC)::value;
                     hipLaunchKernelGGL((k_y<T, N, SIGN, NF, GRAD, TB, EY>),
-                                       dim3((unsigned)((d.nxh + TL - 1) / TL), d.nz, ctx->nbox),
+                                       dim3((unsigned)col_tiles(d.nxh, TL, TB), d.nz, ctx->nbox),
                                        dim3((TL * Line<T, N, EY>::P)), 0, ctx->stream, a0, a1, a2, extra, d,
                                        ctx->ky, (const C *)ctx->twy);
                 };
@@ -378,7 +378,7 @@ struct Impl {
     static dim3 zgrid(const fipi_ctx *ctx)
     {
         constexpr int TL = TileW<T, TB>::L;
-        return dim3((unsigned)((ctx->d.nxh + TL - 1) / TL), ctx->d.ny, ctx->nbox);
+        return dim3((unsigned)col_tiles(ctx->d.nxh, TL, TB), ctx->d.ny, ctx->nbox);
     }
     template <int TB, int N>
     static dim3 zblock()

@@ -169,3 +169,35 @@ def test_mixed_vs_oracle_headline_density_128(gpu):
     assert e_phi < 1e-5
     assert e_area < 1e-5
     assert np.abs(dp).max() < 1e-3
+
+
+@pytest.mark.parametrize("prec", [fipi_amd.FP64, fipi_amd.FP32_MIXED])
+def test_tile_widths_leave_results_unchanged(gpu, prec):
+    """Every combination of y-pass, CH-hub and Stokes-hub column tiles (64
+    or 128 bytes; 64-byte tiles run in XCD-paired order) gives bitwise the
+    same step, and FP64 matches the oracle; a grid with 48 rows exercises
+    an odd tile count per row."""
+    print(f"tile widths prec={prec}", flush=True)
+    for n in (64, 48):
+        p, phi0, pos = make_case(n, 300 if n == 64 else 150, steps=40, seed=71, precision=prec)
+        outs = []
+        for y in (64, 128):
+            for h in (64, 128):
+                for s in (64, 128):
+                    sim = fipi_amd.Simulation(p)
+                    sim.set_state(phi0, pos)
+                    t = sim.tuning()
+                    t.update(y_tile_bytes=y, hub_tile_bytes=h, stokes_tile_bytes=s)
+                    sim.set_tuning(t)
+                    sim.step(6)
+                    outs.append((sim.phi(), sim.velocity(), sim.particles()["p"]))
+                    sim.close()
+        for o in outs[1:]:
+            for a, b in zip(o, outs[0]):
+                assert np.array_equal(a, b)
+        if prec == fipi_amd.FP64:
+            ref = oracle.Sim(p, phi0, pos)
+            ref.step(6)
+            assert rel_l2(outs[0][0], ref.phi()) < TOL
+            assert rel_l2(outs[0][1], ref.u()) < TOL
+            assert rel_l2(outs[0][2], ref.particles()["p"]) < TOL

@@ -91,9 +91,10 @@ def test_step_odd_radix(gpu, n, npart):
 
 
 def test_step_config1_64(gpu):
-    """64^3 benchmark units with 2.5 dx spheres (config 1 physics), 10 steps."""
+    """64^3 benchmark units with 2.5 dx spheres (config 1: 2,000 requested,
+    capped at the ~1,060 random sequential addition places), 10 steps."""
     print("config1 64^3", flush=True)
-    p, phi0, pos = make_case(64, 800, r_p=2.5, steps=200, seed=7)
+    p, phi0, pos = make_case(64, 2000, r_p=2.5, steps=200, seed=7, cap=True)
     (t, g_phi, r_phi, g_u, r_u, gp, rp), = _run_both(p, phi0, pos, 10)
     assert rel_l2(g_phi, r_phi) < TOL
     assert rel_l2(g_u, r_u) < TOL
@@ -102,10 +103,11 @@ def test_step_config1_64(gpu):
 
 def test_1000_steps_64(gpu):
     """The FP64 bar (1e-10 relative L2 after 1000 steps) at 64^3
-    with 800 spheres of radius 2.5 (config-1 physics), checked every 250
-    steps; the oracle takes about half a minute for this on the host."""
+    with config 1's spheres of radius 2.5 (2,000 requested, capped at what
+    random sequential addition places), checked every 250 steps; the oracle
+    takes about half a minute for this on the host."""
     print("1000 steps 64^3", flush=True)
-    p, phi0, pos = make_case(64, 800, r_p=2.5, steps=1000, seed=7)
+    p, phi0, pos = make_case(64, 2000, r_p=2.5, steps=1000, seed=7, cap=True)
     for t, g_phi, r_phi, g_u, r_u, gp, rp in _run_both(p, phi0, pos, 1000, chunks=4):
         print(t, rel_l2(g_phi, r_phi), rel_l2(g_u, r_u), rel_l2(gp["p"], rp["p"]), flush=True)
         assert rel_l2(g_phi, r_phi) < TOL, (t, rel_l2(g_phi, r_phi))
