@@ -143,13 +143,13 @@ __device__ __forceinline__ void store_r2c(const cplx<T> *v, cplx<T> *L, cplx<T> 
 #define FIPI_XPASS_SETUP(T, N) FIPI_XPASS_SETUP_E(T, N, (PlanE_for<T, N>::E))
 
 // The twiddle table of the N-point lines into LDS, read there by every
-// transform of the kernel instead of from global memory (a kernel with
-// several transforms otherwise reloads them after each LDS barrier; the
-// force pass went from 0.442 to 0.407 ms FP64, 0.269 to 0.228 ms FP32).
-// The first barrier of the first transform orders these writes before any
-// read.  Kernels with one transform keep the global table (COND false), and
-// so does the start-up pass k_x_cgrad_f, which the extra LDS pointer costs
-// a wave per SIMD.
+// transform of the kernel instead of from global memory (each transform
+// otherwise loads its twiddles after an LDS barrier, inside its dependency
+// chain; the force pass went from 0.442 to 0.407 ms FP64, 0.269 to 0.228 ms
+// FP32, the one-field y passes 4-5 % faster).  The first barrier of the
+// first transform orders these writes before any read.  The start-up pass
+// k_x_cgrad_f keeps the global table: there the change costs a wave per
+// SIMD.
 #define FIPI_TW_TO_LDS_IF(T, N, tw, COND)                                         \
     __shared__ cplx<T> twl_[(COND) ? (N) : 1];                                    \
     if constexpr (COND) {                                                         \
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_forward(Prod prod, cplx<T> *_
                                                   Dims d, const cplx<T> *__restrict__ tw, double *__restrict__ part)
 {
     FIPI_XPASS_SETUP(T, N)
-    FIPI_TW_TO_LDS_IF(T, N, tw, NF >= 2)
+    FIPI_TW_TO_LDS(T, N, tw)
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         C v[E];
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inverse(const cplx<T> *__rest
                                                   const cplx<T> *__restrict__ tw)
 {
     FIPI_XPASS_SETUP(T, N)
-    FIPI_TW_TO_LDS_IF(T, N, tw, NF >= 2)
+    FIPI_TW_TO_LDS(T, N, tw)
     const size_t rA = zb * d.hs + (size_t)rl * d.nxp, rB = rA + d.nxp;
     const size_t xA = zb * d.rs + (size_t)rl * d.nx, xB = xA + d.nx;
     C v[NF][E];
@@ -675,7 +675,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N, EY>::P)) void k_y(cpl
                                                               const cplx<T> *__restrict__ tw)
 {
     FIPI_TILE_SETUP_E(T, N, (TileW<T, TB>::L), EY)
-    FIPI_TW_TO_LDS_IF(T, N, tw, NF + (GRAD ? 1 : 0) >= 2)
+    FIPI_TW_TO_LDS(T, N, tw)
     a0 = boxp(a0, zb, d.hs);
     a1 = boxp(a1, zb, d.hs);
     a2 = boxp(a2, zb, d.hs);
@@ -1009,6 +1009,7 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_z(cplx<T>
                                                               const cplx<T> *__restrict__ tw)
 {
     FIPI_TILE_SETUP(T, N, (TileW<T, TB>::L))
+    FIPI_TW_TO_LDS(T, N, tw)
     a = boxp(a, zb, d.hs);
     const int y = row_;
     const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
