@@ -723,6 +723,8 @@ int fipi_adjoint_create(const fipi_params *params, const double *phi0, const dou
     }
     fipi_ctx *ctx = a->ctx;
     ctx->fix_particles = true;
+    // the checkpoints copy the real velocity: keep it written every step
+    ctx->u_lazy = false;
     const size_t G = ctx->G;
     a->phi0.assign(phi0, phi0 + G);
     if (n > 0) a->pos0.assign(pos, pos + 3 * (size_t)n);

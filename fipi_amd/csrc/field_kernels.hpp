@@ -275,23 +275,24 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_forward(Prod prod, cplx<T> *_
 }
 
 // Plain inverse x transform (C2R) of NF fields into real arrays.
-template <typename T, int N, int NF>
+// EV: the plan (values per thread), as the kernel whose output it repeats.
+template <typename T, int N, int NF, int EV = PlanE_for<T, N>::E>
 __global__ __launch_bounds__(XT_of<T>::v) void k_x_inverse(const cplx<T> *__restrict__ in0, const cplx<T> *__restrict__ in1,
                                                   const cplx<T> *__restrict__ in2, T *__restrict__ o0,
                                                   T *__restrict__ o1, T *__restrict__ o2, Dims d,
                                                   const cplx<T> *__restrict__ tw)
 {
-    FIPI_XPASS_SETUP(T, N)
+    FIPI_XPASS_SETUP_E(T, N, EV)
     FIPI_TW_TO_LDS(T, N, tw)
     const size_t rA = zb * d.hs + (size_t)rl * d.nxp, rB = rA + d.nxp;
     const size_t xA = zb * d.rs + (size_t)rl * d.nx, xB = xA + d.nx;
     C v[NF][E];
 #pragma unroll
-    for (int f = 0; f < NF; f++) load_c2r<T, N, false>(v[f], f == 0 ? in0 : (f == 1 ? in1 : in2), rA, rB, t, nullptr);
+    for (int f = 0; f < NF; f++) load_c2r<T, N, false, EV>(v[f], f == 0 ? in0 : (f == 1 ? in1 : in2), rA, rB, t, nullptr);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
         T *o = f == 0 ? o0 : (f == 1 ? o1 : o2);
-        fft_dist<T, N, +1>(v[f], L, t, tw);
+        fft_dist<T, N, +1, 1, EV>(v[f], L, t, tw);
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int x = t + q * P;
@@ -303,12 +304,14 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inverse(const cplx<T> *__rest
 
 // P13 + next step's P1: inverse x of u_hat (X) -> u (real), then
 // conv = u . grad c (src/main.cpp:91, field.cpp:392-400, same sum order as
-// ProdConv) and its forward x transform into h0 (X), in place of u_hat_x.
-// g holds grad c^{n+1}, the gradient the next step's convection uses.
-// part (optional): FP64 block sums of conv, as k_x_forward.
+// ProdConv) and its forward x transform into hout (X): h0, in place of
+// u_hat_x, or another array, which leaves u_hat intact.  u0 == nullptr: no
+// real u is stored (it lives in registers only).  g holds grad c^{n+1},
+// the gradient the next step's convection uses.  part (optional): FP64
+// block sums of conv, as k_x_forward.
 template <typename T, int N, int EV = XE_conv<T, N>::v>
-__global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const cplx<T> *h1, const cplx<T> *h2,
-                                                   T *__restrict__ u0,
+__global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(const cplx<T> *h0, const cplx<T> *h1, const cplx<T> *h2,
+                                                   cplx<T> *hout, T *__restrict__ u0,
                                                    T *__restrict__ u1, T *__restrict__ u2,
                                                    const T *__restrict__ g0, const T *__restrict__ g1,
                                                    const T *__restrict__ g2, Dims d, const cplx<T> *__restrict__ tw,
@@ -345,8 +348,10 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const c
 #pragma unroll
         for (int q = 0; q < E; q++) {
             const int x = t + q * P;
-            u[xA + x] = w[q].x;
-            u[xB + x] = w[q].y;
+            if (u != nullptr) {
+                u[xA + x] = w[q].x;
+                u[xB + x] = w[q].y;
+            }
             cv[q] = C{cv[q].x + w[q].x * ga[q], cv[q].y + w[q].y * gb[q]};
         }
     }
@@ -359,7 +364,7 @@ __global__ __launch_bounds__(XT_of<T>::v) void k_x_inv_conv(cplx<T> *h0, const c
         __syncthreads();
     }
     fft_dist<T, N, -1, 1, EV>(cv, L, t, tw);
-    store_r2c<T, N, EV>(cv, L, h0, rA, rB, t);
+    store_r2c<T, N, EV>(cv, L, hout, rA, rB, t);
 }
 
 // P5: inverse x of (c_hat, dy c_hat, dz c_hat) -> c, grad c (real), then

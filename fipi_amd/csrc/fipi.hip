@@ -99,6 +99,15 @@ struct fipi_ctx {
     // h[0] already holds the next step's conv (X) and conv_part its FP64
     // block sums: the previous step's last pass formed u.grad c
     bool conv_ready = false;
+    // Velocity kept in spectral form between steps (u_lazy): the last pass
+    // of a step forms u only in registers (for u.grad c) and writes no real
+    // u, which saves three real fields of HBM writes a step; it leaves
+    // u_hat (after the y transforms) in h[3], h[1], h[2] and the next step's
+    // conv in h[0].  u_stale: the real u arrays are behind; materialize_u
+    // forms them before anything outside the step reads them (getters,
+    // snapshots, the tuner).  Off where the particle stage interpolates u
+    // and in the ensemble and adjoint contexts.
+    bool u_lazy = false, u_stale = false;
     // calibration extension: wetting source W (real, added to f; null in the
     // reference physics) and particles held at their positions (no Heun
     // stages; their contact forces are spread every step unchanged)
@@ -334,6 +343,23 @@ struct Impl {
             hipLaunchKernelGGL((k_x_inverse<T, N, NF>), (xgrid<T, N>(d, ctx->nbox)), (xblock<T, N>(d)), 0, ctx->stream, i0, i1, i2, o0, o1, o2, d,
                                (const C *)ctx->twx);
         });
+    }
+
+    // real u from the u_hat the last step left in h[3], h[1], h[2]; the same
+    // inverse x transform (plan and arithmetic) as the step's last pass
+    static void materialize_u(fipi_ctx *ctx)
+    {
+        if (!ctx->u_stale) return;
+        const Dims d = ctx->d;
+        with_n(d.nx, [&](auto NC) {
+            constexpr int N = decltype(NC)::value;
+            constexpr int XE = XE_conv<T, N>::v;
+            hipLaunchKernelGGL((k_x_inverse<T, N, 3, XE>), (xgrid<T, N, XE>(d, ctx->nbox)), (xblock<T, N, XE>(d)), 0,
+                               ctx->stream, (const C *)H(ctx->h[3]), (const C *)H(ctx->h[1]),
+                               (const C *)H(ctx->h[2]), R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]), d,
+                               (const C *)ctx->twx);
+        });
+        ctx->u_stale = false;
     }
 
     template <int SIGN, int NF, bool GRAD>
@@ -606,6 +632,7 @@ struct Impl {
         const bool fork = beside && parts && !ctx->fix_particles;
         if (ns) {
             if (!ctx->conv_ready) {
+                materialize_u(ctx);
                 ProdConv<T> pr{R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]), R(ctx->g[0]), R(ctx->g[1]), R(ctx->g[2])};
                 ProfScope ps_(ctx, "x_fwd_conv", 6 * Rb + Hb);
                 x_forward<1>(ctx, pr, h0, nullptr, mixed ? ctx->conv_part : nullptr);
@@ -712,17 +739,26 @@ struct Impl {
                 y_pass<+1, 3, false>(ctx, h0, h1, h2, nullptr);
             }
             {
-                // u^{n+1} and, for the next step, conv = u^{n+1}.grad c^{n+1} (X) in h[0]
-                ProfScope ps_(ctx, "x_inv_u_conv", 4 * Hb + 6 * Rb);
+                // u^{n+1} and, for the next step, conv = u^{n+1}.grad c^{n+1}
+                // (X) in h[0]; with u_lazy, conv goes to h[3] (free after the
+                // force pass), no real u is written, and h[0] / h[3] swap
+                // roles, so that u_hat stays in h[3], h[1], h[2]
+                const bool lazy = ctx->u_lazy;
+                ProfScope ps_(ctx, "x_inv_u_conv", 4 * Hb + (lazy ? 3 : 6) * Rb);
                 with_n(d.nx, [&](auto NC) {
                     constexpr int N = decltype(NC)::value;
                     constexpr int XE = XE_conv<T, N>::v;
                     hipLaunchKernelGGL((k_x_inv_conv<T, N, XE>), (xgrid<T, N, XE>(d, ctx->nbox)), (xblock<T, N, XE>(d)), 0, st, h0, (const C *)h1,
-                                       (const C *)h2, R(ctx->u[0]), R(ctx->u[1]), R(ctx->u[2]),
+                                       (const C *)h2, lazy ? h3 : h0, lazy ? nullptr : R(ctx->u[0]),
+                                       lazy ? nullptr : R(ctx->u[1]), lazy ? nullptr : R(ctx->u[2]),
                                        (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]), d,
                                        (const C *)ctx->twx, mixed ? ctx->conv_part : nullptr);
                 });
                 ctx->conv_ready = true;
+                if (lazy) {
+                    std::swap(ctx->h[0], ctx->h[3]);
+                    ctx->u_stale = true;
+                }
             }
         }
         if (fork) {
@@ -1173,6 +1209,13 @@ extern "C" {
 
 }  // extern "C"
 
+// the real velocity arrays up to date (they lag while u_stale)
+static void ensure_real_u(fipi_ctx *ctx)
+{
+    if (ctx->fp64) Impl<double>::materialize_u(ctx);
+    else Impl<float>::materialize_u(ctx);
+}
+
 // A context of nbox equal boxes (the slab) with n_total particles in all.
 static int create_ctx(const fipi_params *params, int nbox, int n_total, fipi_ctx **out)
 {
@@ -1191,8 +1234,11 @@ static int create_ctx(const fipi_params *params, int nbox, int n_total, fipi_ctx
     ctx->fp64 = params->precision == FIPI_FP64;
     {
         const int nxh = params->nx / 2 + 1;
+        // row pitch: whole 128-byte lines, plus one spare line (measured at
+        // 256^3: the z hubs 4-6 % faster with the spare line, one or two
+        // per cent for the step; two or more spare lines were slower)
         const int q = params->precision == FIPI_FP64 ? 8 : 16;  // complex values per 128 bytes
-        ctx->d = Dims{params->nx, params->ny, params->nz, nxh, (nxh + q - 1) / q * q};
+        ctx->d = Dims{params->nx, params->ny, params->nz, nxh, (nxh + q - 1) / q * q + q};
     }
     ctx->G = (size_t)params->nx * params->ny * params->nz;
     ctx->GH = (size_t)ctx->d.nxp * params->ny * params->nz;
@@ -1222,7 +1268,10 @@ extern "C" {
 
 int fipi_create(const fipi_params *params, fipi_ctx **out)
 {
-    return create_ctx(params, 1, params ? params->n_particles : 0, out);
+    const int rc = create_ctx(params, 1, params ? params->n_particles : 0, out);
+    // u stays spectral between steps unless the particle stage reads it
+    if (rc == FIPI_OK) (*out)->u_lazy = params->interp_u == 0;
+    return rc;
 }
 
 int fipi_destroy(fipi_ctx *ctx)
@@ -1268,6 +1317,7 @@ static int set_state_slab(fipi_ctx *ctx, const double *phi, const double *pos, c
     ctx->t = 0;
     ctx->cur = 0;
     ctx->conv_ready = false;
+    ctx->u_stale = false;
     ctx->spread_ready = false;
     ctx->tables_ready = false;
     {
@@ -1379,6 +1429,7 @@ static int set_state_dev(fipi_ctx *ctx, const double *d_phi, const double *d_pos
     ctx->t = 0;
     ctx->cur = 0;
     ctx->conv_ready = false;
+    ctx->u_stale = false;
     ctx->spread_ready = false;
     ctx->tables_ready = false;
     {
@@ -1455,6 +1506,7 @@ int fipi_set_velocity(fipi_ctx *ctx, const double *u)
 {
     if (!ctx || !u) return set_err(ctx, FIPI_ERR_ARG, "ctx and u are required");
     ctx->conv_ready = false;
+    ctx->u_stale = false;
     for (int a = 0; a < 3; a++) {
         int rc = copy_in_real(ctx, ctx->u[a], u + (size_t)a * ctx->G, ctx->G);
         if (rc) return rc;
@@ -1472,6 +1524,7 @@ int fipi_get_phi(fipi_ctx *ctx, double *phi)
 int fipi_get_velocity(fipi_ctx *ctx, double *u)
 {
     if (!ctx || !u) return set_err(ctx, FIPI_ERR_ARG, "ctx and u are required");
+    ensure_real_u(ctx);
     for (int a = 0; a < 3; a++) {
         int rc = copy_out_real(ctx, u + (size_t)a * ctx->G, ctx->u[a], ctx->G);
         if (rc) return rc;
@@ -1975,6 +2028,7 @@ int fipi_snapshot_save(fipi_ctx *ctx, void *buf, int64_t size)
 {
     if (!ctx || !buf) return set_err(ctx, FIPI_ERR_ARG, "ctx and buf are required");
     if (!ctx->has_state) return set_err(ctx, FIPI_ERR_STATE, "no state to save");
+    ensure_real_u(ctx);
     const int64_t need = fipi_snapshot_size(ctx);
     if (size < need) return set_err(ctx, FIPI_ERR_OVERFLOW, "snapshot buffer too small");
     SnapHeader h{};
@@ -2030,6 +2084,7 @@ int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size)
     ctx->t = h.t;
     ctx->cur = h.cur;
     ctx->conv_ready = h.conv_ready != 0;
+    ctx->u_stale = false;
     ctx->pp.has_ctheta = h.has_ctheta;
     ctx->tune.y_e = h.y_e;
     ctx->spread_ready = false;
@@ -2053,6 +2108,7 @@ int fipi_tune(fipi_ctx *ctx, int32_t steps)
             if (b) (void)hipEventDestroy(b);
         }
     } keep;
+    ensure_real_u(ctx);
     size_t total = 0;
     for (auto &x : snapshot_buffers(ctx)) total += x.second;
     HIPCHK(ctx, hipMalloc(&keep.p, total));
@@ -2111,6 +2167,7 @@ int fipi_tune(fipi_ctx *ctx, int32_t steps)
     ctx->t = t0;
     ctx->cur = cur0;
     ctx->conv_ready = conv0;
+    ctx->u_stale = false;
     ctx->spread_ready = false;
     ctx->tables_ready = false;
     return rc;
