@@ -135,11 +135,9 @@ struct fipi_ctx {
     int nlines = 0;
     int *line_cnt = nullptr, *line_start = nullptr, *line_cursor = nullptr, *node_cnt = nullptr;
     void *spr_tmp = nullptr;  // spread node pool (8n nodes)
-    // the step's particle stages: cell tables of fixed capacity (tab[0]:
-    // the positions at the start of a step, tab[1]: after stage 1) and the
-    // spread's line buckets
-    CellTab tab[2] = {};
-    int tab_parts[2] = {0, 0};  // argmax partials each table holds
+    // the step's particle stages: three cell tables of fixed capacity in
+    // rotation (TabSet) and the spread's line buckets
+    TabSet tabs{};
     LineTab lines{};
     bool tables_ready = false;
     // scratch of the facade's complex transform (allocated on first use)
@@ -558,54 +556,67 @@ struct Impl {
         return 1.0;
     }
 
-    // tab[0] from the current positions (after the state was set or restored)
+    // The table the next stage reads (stage 1 of step t: (2t) % 3) from the
+    // current positions, after the state was set or restored, with one
+    // argmax partial per stage workgroup; the table that stage appends to is
+    // emptied too (no stage before it did).
     static void tables_init(fipi_ctx *ctx)
     {
         const PartParams &pp = ctx->pp;
         const PartState &ps = ctx->ps;
-        const CellTab &T0 = ctx->tab[0];
-        hipLaunchKernelGGL(k_tab_clear, grid1(ctx->ncell, 256), dim3(256), 0, ctx->stream, T0.cnt, ctx->ncell,
-                           T0.ovf_cnt, (int *)nullptr, (int *)nullptr);
-        hipLaunchKernelGGL(k_cells_append, grid1(pp.n, 256), dim3(256), 0, ctx->stream, pp, (const double *)ps.p[0],
-                           (const double *)ps.p[1], (const double *)ps.p[2], ps.box, T0, ctx->d_flags);
-        ctx->tab_parts[0] = (int)grid1(pp.n, 256).x;
+        const CellTab &T0 = ctx->tabs.t[(2 * ctx->t) % 3], &T1 = ctx->tabs.t[(2 * ctx->t + 1) % 3];
+        for (const CellTab *tb : {&T0, &T1})
+            hipLaunchKernelGGL(k_tab_clear, grid1(ctx->ncell, 256), dim3(256), 0, ctx->stream, tb->cnt, ctx->ncell,
+                               tb->ovf_cnt, (int *)nullptr, (int *)nullptr);
+        const dim3 tg = contact_tile_grid(pp);
+        hipLaunchKernelGGL(k_cells_append, dim3(tg.x * tg.y * tg.z), dim3(256), 0, ctx->stream, pp,
+                           (const double *)ps.p[0], (const double *)ps.p[1], (const double *)ps.p[2], ps.box, T0,
+                           ctx->d_flags);
         ctx->tables_ready = true;
     }
 
-    // One Heun stage (main.cpp:198-217): interpolation and f_pi (A), then
-    // contacts, advection and the next sort (B) over tab[stage-1] into
-    // tab[2-stage]; stage 2 also spreads f_pp at the new positions for the
-    // next step's force pass (when the Stokes solve is on).
+    // One Heun stage (main.cpp:198-217) in one launch: interpolation, f_pi,
+    // contacts, advection and the next sort, tile by tile over the stage's
+    // cell table (TabSet); stage 2 also appends the spread lines of f_pp at
+    // the new positions, whose node sums follow for the next step's force
+    // pass (when the Stokes solve is on).
     static void particle_stage(fipi_ctx *ctx, int stage)
     {
         hipStream_t st = ctx->stream;
         const PartParams &pp = ctx->pp;
         const PartState &ps = ctx->ps;
         if (!ctx->tables_ready) tables_init(ctx);
-        const int cur = stage - 1, nxt = 2 - stage;
-        const bool spread = stage == 2 && ctx->p.switch_ns != 0;
+        const bool ns = ctx->p.switch_ns != 0;
+        const bool spread = stage == 2 && ns;
         const double n = pp.n;
-        {
-            ProfScope ps_(ctx, "stage_a", n * (24.0 + 32.0 * sizeof(T) + 8.0 + 24.0 + 24.0 + 24.0 + 16.0));
-            hipLaunchKernelGGL((k_stage_a<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, ps, (const T *)R(ctx->c),
-                               (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]), (const T *)R(ctx->g[2]),
-                               (const T *)R(ctx->u[0]), (const T *)R(ctx->u[1]), (const T *)R(ctx->u[2]),
-                               (const int *)ctx->t_dev, stage, ctx->tab[cur], ctx->tab_parts[cur], ctx->tab[nxt],
-                               ctx->ncell, ctx->lines, spread ? 1 : 0);
+        const StageFields<T> F{(const T *)R(ctx->c), (const T *)R(ctx->g[0]), (const T *)R(ctx->g[1]),
+                               (const T *)R(ctx->g[2]), (const T *)R(ctx->u[0]), (const T *)R(ctx->u[1]),
+                               (const T *)R(ctx->u[2])};
+        const dim3 grid = contact_tile_grid(pp);
+        if (stage == 2) {
+            // it_mark of both sorts and the tau gate of stage 2
+            ProfScope ps_(ctx, "mark", 64.0 * grid.x * grid.y * grid.z);
+            hipLaunchKernelGGL((k_mark<T>), dim3(1), dim3(256), 0, st, pp, ps, F, ctx->tabs,
+                               (int)(grid.x * grid.y * grid.z), (const int *)ctx->t_dev);
         }
-        const double rmp = ramp((double)(ctx->t + 1), ctx->p.ramp_t);
-        LineTab lt = ctx->lines;
-        if (!spread) lt.cnt = nullptr;
+        if (pp.interp_u || pp.brown_amp > 0.0)
+            hipLaunchKernelGGL((k_fluid_noise<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, ps, F,
+                               (const int *)ctx->t_dev, stage);
         {
-            ProfScope ps_(ctx, "contact_tile", n * (4.0 + 32.0 * 2.3 + 24.0 + 8.0 + 24.0 + 5 * 24.0 + 32.0 + (spread ? 16.0 : 0.0)));
-            const dim3 grid = contact_tile_grid(pp);
+            // per particle: its record, 32 gathered grid values, radius and
+            // contact angle, the appended record; stage 1 writes p_old, u_old
+            // and the new position, stage 2 reads p_old, u_old and writes the
+            // step's state (p, u, f_pp, tau, c, grad c, f_pi) and 4 line slots
+            const double per = 32.0 + 32.0 * sizeof(T) + 16.0 + 32.0 +
+                               (stage == 1 ? 72.0 : 48.0 + 17 * 8.0 + (spread ? 16.0 : 0.0));
+            ProfScope ps_(ctx, stage == 1 ? "stage_1" : "stage_2", n * per);
             if (stage == 1)
-                hipLaunchKernelGGL(k_contact_tile<1>, grid, dim3(256), 0, st, pp, ps, ctx->tab[cur], ctx->tab[nxt], lt,
-                                   (const int *)ctx->t_dev, ctx->p.ramp_t, rmp, ctx->d_flags + 1);
+                hipLaunchKernelGGL((k_stage<1, T>), grid, dim3(256), 0, st, pp, ps, F, ctx->tabs, ctx->ncell,
+                                   ctx->lines, ns ? 1 : 0, (const int *)ctx->t_dev, ctx->p.ramp_t, ctx->d_flags + 1);
             else
-                hipLaunchKernelGGL(k_contact_tile<2>, grid, dim3(256), 0, st, pp, ps, ctx->tab[cur], ctx->tab[nxt], lt,
-                                   (const int *)ctx->t_dev, ctx->p.ramp_t, rmp, ctx->d_flags + 1);
-            ctx->tab_parts[nxt] = (int)(grid.x * grid.y * grid.z);
+                hipLaunchKernelGGL((k_stage<2, T>), grid, dim3(256), 0, st, pp, ps, F, ctx->tabs, ctx->ncell,
+                                   ctx->lines, spread ? 1 : 0, (const int *)ctx->t_dev, ctx->p.ramp_t,
+                                   ctx->d_flags + 1);
         }
         if (spread) {
             spread_nodes(ctx, pp, ps.p[0], ps.p[1], ps.p[2], ps.fpp[0], ps.fpp[1], ps.fpp[2]);
@@ -1038,18 +1049,16 @@ int setup_buffers(fipi_ctx *ctx)
     ALLOC(ctx->cell_ids, sizeof(int) * n);
     ALLOC(ctx->spr_tmp, sizeof(SpreadNode<T>) * spread_pool_nodes(ctx->nlines, (int)n));
     // the step's cell tables and spread line buckets
-    for (int k = 0; k < 2; k++) {
-        CellTab &tb = ctx->tab[k];
+    for (int k = 0; k < 3; k++) {
+        CellTab &tb = ctx->tabs.t[k];
         ALLOC(tb.cnt, sizeof(int) * (ctx->ncell + 1));
         ALLOC(tb.rec, sizeof(CellRec) * CELL_CAP * (size_t)ctx->ncell);
         ALLOC(tb.ovf_cnt, sizeof(int));
         ALLOC(tb.ovf, sizeof(CellRec) * n);
         {
-            // argmax partials: one per contact tile or per 256 particles
+            // argmax partials: one per stage workgroup
             const dim3 tg = contact_tile_grid(pp);
-            const size_t parts = std::max<size_t>((size_t)tg.x * tg.y * tg.z, n / 256 + 1) + 2;
-            ALLOC(tb.ymax, sizeof(double) * parts);
-            ALLOC(tb.yidx, sizeof(int) * parts);
+            ALLOC(tb.mp, sizeof(MarkPart) * ((size_t)tg.x * tg.y * tg.z));
         }
         HIPCHK(ctx, hipMemset(tb.cnt, 0, sizeof(int) * (ctx->ncell + 1)));
         HIPCHK(ctx, hipMemset(tb.ovf_cnt, 0, sizeof(int)));
@@ -1076,6 +1085,7 @@ int setup_buffers(fipi_ctx *ctx)
     PartState &ps = ctx->ps;
     for (int a = 0; a < 3; a++) {
         ALLOC(ps.p[a], sizeof(double) * n);
+        ALLOC(ps.p1[a], sizeof(double) * n);
         ALLOC(ps.po[a], sizeof(double) * n);
         ALLOC(ps.u[a], sizeof(double) * n);
         ALLOC(ps.uo[a], sizeof(double) * n);
@@ -1090,6 +1100,8 @@ int setup_buffers(fipi_ctx *ctx)
     ALLOC(ps.cth, sizeof(double) * n);
     ALLOC(ps.cloc, sizeof(double) * n);
     ALLOC(ps.tau_i, sizeof(double) * n);
+    ALLOC(ps.nodes, sizeof(T) * 32 * n);
+    ALLOC(ps.gnode, sizeof(uint32_t) * n);
     ALLOC(ps.cell, sizeof(int) * n);
     if (ctx->nbox > 1) {
         int *bx;
@@ -1098,6 +1110,8 @@ int setup_buffers(fipi_ctx *ctx)
     }
     ps.err = ctx->d_flags;
     ALLOC(ps.mark, sizeof(int));
+    ALLOC(ps.mark_on, sizeof(int));
+    HIPCHK(ctx, hipMemset(ps.mark_on, 0, sizeof(int)));
 #undef ALLOC
     return FIPI_OK;
 }

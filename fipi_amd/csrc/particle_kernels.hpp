@@ -45,14 +45,18 @@ struct PartParams {
 
 struct PartState {
     double *p[3], *po[3], *u[3], *uo[3], *fpp[3], *fpi[3];
+    double *p1[3];               // positions after Heun stage 1 of the step
     double *rad, *cth;
     double *cloc, *dcdx[3];
     double *uf[3], *frand[3];    // local fluid velocity and Brownian force of the last stage
     double *tau_i;               // per-particle packing-energy contributions
+    void *nodes;                 // [n][4][8] grid values (c, grad c at the 8 corners) stage 1 gathered, as T
+    uint32_t *gnode;             // [n] their corner 0 (byte offset in the box's field)
     double *sp[3];               // positions in cell-sorted order (sp[d][s] = p[d][ids[s]])
     int *cell;                   // contact cell of each particle
     int *err;                    // device error flags (bit 0: sorting range)
     int *mark;                   // it_mark (argmax y), persists across sorts
+    int *mark_on;                // the stage-2 tau gate: f_pi of it_mark non-zero
     const int *box;              // box of each particle (null: all in box 0)
 };
 
@@ -710,6 +714,75 @@ __device__ __forceinline__ double interp1(const PartParams &P, const T *__restri
     r += (1.0 - m) * n * o * (double)v[i + j1 * sy + k1 * sz];
     r += m * (1.0 - n) * o * (double)v[i1 + j * sy + k1 * sz];
     r += m * n * o * (double)v[i1 + j1 * sy + k1 * sz];
+    return r;
+}
+
+// interp1 split in two: the eight weights and corner offsets of a position
+// (computed once), then the weighted sum of one field, in interp1's order
+// and association; several fields at one position share the first part,
+// and the corners are 32-bit byte offsets from the field's base (a box's
+// field is below 4 GiB), so the gathers need no 64-bit vector addresses.
+struct Tri {
+    double w[8];
+    uint32_t off[8];
+};
+
+template <typename T>
+__device__ __forceinline__ Tri tri_of(const PartParams &P, double px, double py, double pz)
+{
+    const int Nx = P.nx, Ny = P.ny, Nz = P.nz;
+    const int i = (int)floor(px / P.dx), j = (int)floor(py / P.dy), k = (int)floor(pz / P.dz);
+    const double m = px / P.dx - floor(px / P.dx);
+    const double n = py / P.dy - floor(py / P.dy);
+    const double o = pz / P.dz - floor(pz / P.dz);
+    const int i1 = (i + 1) % Nx, j1 = (j + 1) % Ny, k1 = (k + 1) % Nz;
+    const uint32_t sy = (uint32_t)Nx, sz = (uint32_t)Nx * (uint32_t)Ny, es = (uint32_t)sizeof(T);
+    Tri t;
+    t.w[0] = (1.0 - m) * (1.0 - n) * (1.0 - o);
+    t.w[1] = (1.0 - m) * n * (1.0 - o);
+    t.w[2] = m * (1.0 - n) * (1.0 - o);
+    t.w[3] = m * n * (1.0 - o);
+    t.w[4] = (1.0 - m) * (1.0 - n) * o;
+    t.w[5] = (1.0 - m) * n * o;
+    t.w[6] = m * (1.0 - n) * o;
+    t.w[7] = m * n * o;
+    t.off[0] = (i + j * sy + k * sz) * es;
+    t.off[1] = (i + j1 * sy + k * sz) * es;
+    t.off[2] = (i1 + j * sy + k * sz) * es;
+    t.off[3] = (i1 + j1 * sy + k * sz) * es;
+    t.off[4] = (i + j * sy + k1 * sz) * es;
+    t.off[5] = (i + j1 * sy + k1 * sz) * es;
+    t.off[6] = (i1 + j * sy + k1 * sz) * es;
+    t.off[7] = (i1 + j1 * sy + k1 * sz) * es;
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ double tri_sum(const Tri &t, const T *__restrict__ v)
+{
+    const char *b = reinterpret_cast<const char *>(v);
+    double r = t.w[0] * (double)*reinterpret_cast<const T *>(b + t.off[0]);
+#pragma unroll
+    for (int q = 1; q < 8; q++) r += t.w[q] * (double)*reinterpret_cast<const T *>(b + t.off[q]);
+    return r;
+}
+
+// the eight corner values of one field, and their weighted sum (tri_sum's
+// arithmetic on values already gathered)
+template <typename T>
+__device__ __forceinline__ void tri_gather(const Tri &t, const T *__restrict__ v, T *out)
+{
+    const char *b = reinterpret_cast<const char *>(v);
+#pragma unroll
+    for (int q = 0; q < 8; q++) out[q] = *reinterpret_cast<const T *>(b + t.off[q]);
+}
+
+template <typename T>
+__device__ __forceinline__ double tri_dot(const Tri &t, const T *v)
+{
+    double r = t.w[0] * (double)v[0];
+#pragma unroll
+    for (int q = 1; q < 8; q++) r += t.w[q] * (double)v[q];
     return r;
 }
 

@@ -1,18 +1,19 @@
-// One Heun particle stage of the time step in two launches, and the
-// spread of the particle forces in one more:
+// One Heun particle stage of the time step in one launch, and the spread
+// of the particle forces in one more:
 //
-//   k_stage_a   per particle: trilinear interpolation of c and grad c
-//               (element.cpp:55-71), f_pi (element.cpp:75-97); block 0
-//               reduces it_mark (cluster.cpp:128-138) from the argmax
-//               partials of the positions the current cell table holds;
-//               the next table's counters are cleared.
-//   k_contact_tile  per tile of contact cells, staged in LDS: contact forces
-//               over the 27 neighbour cells (cluster.cpp:174-275), summed in
-//               the reference's order; the Heun stage and the periodic wrap
+//   k_stage     per tile of contact cells, staged in LDS: it_mark
+//               (cluster.cpp:128-138) from the argmax partials of the
+//               positions the current cell table holds; per particle the
+//               trilinear interpolation of c and grad c (element.cpp:55-71)
+//               and f_pi (element.cpp:75-97), the contact forces over the 27
+//               neighbour cells (cluster.cpp:174-275), summed in the
+//               reference's order, the Heun stage and the periodic wrap
 //               (element.cpp:156-186); the new position is appended to the
 //               next table (the sort of the following stage,
 //               cluster.cpp:116-169); in stage 2 the particle id is also
 //               appended to the four grid lines its spread touches.
+//               Particles are visited tile by tile, so neighbouring threads
+//               gather from neighbouring grid nodes.
 //   k_spread_nodes  per grid line: the two spread entries of every particle
 //               on the line, sorted by (x, particle id) and summed per node
 //               in ascending particle order (cluster.cpp:344-410); the nodes
@@ -41,14 +42,62 @@ struct __align__(16) CellRec {
     int id, cell;
 };
 
+// Argmax-y partial of one workgroup's particles (cluster.cpp:128-138: the
+// strictly largest y > 0, first index on ties; idx < 0: none) with that
+// particle's position, from which the next stage takes it_mark and the
+// mark's attachment force.
+struct __align__(16) MarkPart {
+    double x, y, z;
+    int idx, pad;
+};
+
 struct CellTab {
     int *cnt;       // [ncell] records appended (may exceed CELL_CAP)
     CellRec *rec;   // [ncell * CELL_CAP]
     int *ovf_cnt;   // records in the overflow list
     CellRec *ovf;   // [n] records beyond a cell's capacity
-    double *ymax;   // argmax-y partials of the appended positions, per block
-    int *yidx;
+    MarkPart *mp;   // argmax-y partials of the appended positions, one per
+                    // workgroup of the appending launch
 };
+
+// The three cell tables of the particle stages.  Stage s of a run (s = 2t +
+// stage - 1, t the device step counter) reads table s % 3, appends to
+// (s + 1) % 3 and empties (s + 2) % 3, the table the stage before it read.
+// The choice is made on the device, so a captured step replays unchanged.
+struct TabSet {
+    CellTab t[3];
+};
+
+// argmax-y candidate: (y descending, index ascending) is a total order
+// with "none" (i < 0) below every candidate, so combines in any order give
+// the same result
+struct MarkC {
+    double x, y, z;
+    int i;
+};
+
+__device__ __forceinline__ void mark_take(MarkC &m, const MarkC &o)
+{
+    if (o.i >= 0 && (m.i < 0 || o.y > m.y || (o.y == m.y && o.i < m.i))) m = o;
+}
+
+// combine one candidate per thread over a 256-thread workgroup; every
+// thread gets the result (sm: 4 LDS slots, free again on return)
+__device__ __forceinline__ MarkC block_mark(MarkC m, MarkC *sm)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const MarkC q{__shfl_xor(m.x, o), __shfl_xor(m.y, o), __shfl_xor(m.z, o), __shfl_xor(m.i, o)};
+        mark_take(m, q);
+    }
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    MarkC r = sm[0];
+#pragma unroll
+    for (int w = 1; w < 4; w++) mark_take(r, sm[w]);
+    __syncthreads();
+    return r;
+}
 
 struct LineTab {
     int *cnt;        // [nlines] particles appended (may exceed LINE_CAP)
@@ -107,23 +156,28 @@ __device__ __forceinline__ void line_append(LineTab T, int line, int id)
 
 // ---------------------------------------------------------------------------
 // Table from positions (after the state is set or restored) and its argmax
-// partials; the table's counters must be zero.
+// partials, one per workgroup (launched with as many workgroups as the
+// stage kernel); the table's counters must be zero.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_cells_append(PartParams P, const double *__restrict__ px,
                                                       const double *__restrict__ py, const double *__restrict__ pz,
                                                       const int *__restrict__ box, CellTab T, int *__restrict__ err)
 {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    block_argmax_y(a < P.n ? py[a] : -1.0, a, T.ymax, T.yidx);
-    if (a >= P.n) return;
-    int i, j, k;
-    const double x = px[a], y = py[a], z = pz[a];
-    const int cc = cell_of(P, x, y, z, &i, &j, &k);
-    if (cc < 0) {
-        atomicOr(err, 1);
-        return;
+    __shared__ MarkC sm[4];
+    MarkC m{0.0, 0.0, 0.0, -1};
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < P.n; a += gridDim.x * blockDim.x) {
+        const double x = px[a], y = py[a], z = pz[a];
+        if (y > 0.0) mark_take(m, MarkC{x, y, z, a});
+        int i, j, k;
+        const int cc = cell_of(P, x, y, z, &i, &j, &k);
+        if (cc < 0) {
+            atomicOr(err, 1);
+            continue;
+        }
+        cell_append(P, T, cc + part_box(box, a) * P.ncell_box, x, y, z, a);
     }
-    cell_append(P, T, cc + part_box(box, a) * P.ncell_box, x, y, z, a);
+    const MarkC r = block_mark(m, sm);
+    if (threadIdx.x == 0) T.mp[blockIdx.x] = MarkPart{r.x, r.y, r.z, r.i, 0};
 }
 
 // counters of a table (and optionally of the spread lines) to zero; a
@@ -190,94 +244,7 @@ __device__ __forceinline__ double heun_velocity(const PartParams &P, double fpi,
 }
 
 // ---------------------------------------------------------------------------
-// Stage kernel A
-// ---------------------------------------------------------------------------
-// it_mark from nparts argmax partials: the strictly largest y > 0, first
-// index on ties; the previous value is kept if no y > 0 (cluster.cpp:128-138)
-__device__ void reduce_mark(const double *__restrict__ ymax, const int *__restrict__ yidx, int nparts, int *mark)
-{
-    __shared__ double sy[256];
-    __shared__ int si[256];
-    double by = 0.0;
-    int bi = -1;
-    for (int k = threadIdx.x; k < nparts; k += blockDim.x) argmax_take(by, bi, ymax[k], yidx[k]);
-    sy[threadIdx.x] = by;
-    si[threadIdx.x] = bi;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            double y = sy[threadIdx.x];
-            int i = si[threadIdx.x];
-            argmax_take(y, i, sy[threadIdx.x + o], si[threadIdx.x + o]);
-            sy[threadIdx.x] = y;
-            si[threadIdx.x] = i;
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0 && si[0] >= 0) *mark = si[0];
-}
-
-// clear_lines: also reset the spread's overflow list and node pool (the
-// stage whose B appends spread lines)
-// With P.interp_u also uf from u (element.cpp:43-49 / field.cpp:251-281),
-// with P.brown_amp the Brownian force of this step and stage.
-template <typename T>
-__global__ __launch_bounds__(256) void k_stage_a(PartParams P, PartState S, const T *__restrict__ c,
-                                                 const T *__restrict__ g0, const T *__restrict__ g1,
-                                                 const T *__restrict__ g2, const T *__restrict__ u0,
-                                                 const T *__restrict__ u1, const T *__restrict__ u2,
-                                                 const int *__restrict__ t_dev, int stage, CellTab cur, int nparts,
-                                                 CellTab nxt, int ncell, LineTab lt, int clear_lines)
-{
-    if (blockIdx.x == 0) reduce_mark(cur.ymax, cur.yidx, nparts, S.mark);
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int q = a; q < ncell; q += gridDim.x * blockDim.x) nxt.cnt[q] = 0;
-    if (a == 0) {
-        *nxt.ovf_cnt = 0;
-        if (clear_lines) {
-            *lt.ovf_cnt = 0;
-            *lt.pool_top = 0;
-        }
-    }
-    if (a >= P.n) return;
-    const double px = S.p[0][a], py = S.p[1][a], pz = S.p[2][a];
-    if (!grid_cell_ok(P, px, py, pz)) {
-        atomicOr(S.err, 2);
-        return;
-    }
-    const int bx = part_box(S.box, a);
-    c += bx * P.rs;
-    g0 += bx * P.rs;
-    g1 += bx * P.rs;
-    g2 += bx * P.rs;
-    const double phi = interp1(P, c, px, py, pz);
-    const double gx = interp1(P, g0, px, py, pz);
-    const double gy = interp1(P, g1, px, py, pz);
-    const double gz = interp1(P, g2, px, py, pz);
-    S.cloc[a] = phi;
-    S.dcdx[0][a] = gx;
-    S.dcdx[1][a] = gy;
-    S.dcdx[2][a] = gz;
-    double f[3];
-    capillary_force(P, phi, gx, gy, gz, S.rad[a], P.has_ctheta ? S.cth[a] : 0.0, f);
-    S.fpi[0][a] = f[0];
-    S.fpi[1][a] = f[1];
-    S.fpi[2][a] = f[2];
-    if (P.interp_u) {
-        const T *us[3] = {u0 + bx * P.rs, u1 + bx * P.rs, u2 + bx * P.rs};
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-            S.uf[d][a] = P.interp_u == 1 ? interp1(P, us[d], px, py, pz) : interp2(P, us[d], px, py, pz);
-    }
-    if (P.brown_amp > 0.0) {
-        const int t = *t_dev;
-#pragma unroll
-        for (int d = 0; d < 3; d++) S.frand[d][a] = P.brown_amp * brownian_g(P.noise_seed, t, stage, a, d, P.pi);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Stage kernel B
+// Stage kernel: cell records and contact sums
 // ---------------------------------------------------------------------------
 // Visit the records of global cell co in slot order (the table part, then
 // the overflow records of that cell): f(rec, ref) with ref >= 0 a table
@@ -387,18 +354,19 @@ __device__ void contact_slow_tab(const PartParams &P, const CellTab &T, int a, d
 }
 
 // ---------------------------------------------------------------------------
-// Stage kernel B: contacts over LDS tiles of contact cells
+// Stage kernel: a tile of contact cells per workgroup
 // ---------------------------------------------------------------------------
 // A workgroup owns a tile of TILE_X x TILE_Y x TILE_Z contact cells (one
 // thread per cell) and stages the records of the tile and its one-cell halo
 // (periodic) in LDS, each cell's records sorted by id.  Every thread then
-// handles the particles of its cell from LDS: a bit per (neighbour cell,
-// slot) marks the partners in contact, the partners b < a are summed in
-// ascending id and the partners b > a in traversal order (neighbour cells
-// k, j, i; ids ascending in a cell), as cluster.cpp:174-275 accumulates;
-// then the Heun stage, the wrap, the next sort and (stage 2) the spread
-// lines.  A tile whose halo holds an overflowing cell or more records than
-// the LDS takes the exact global-memory path (contact_slow_tab).
+// handles the particles of its cell from LDS: the interpolation and f_pi at
+// the particle, a bit per (neighbour cell, slot) marking the partners in
+// contact, the partners b < a summed in ascending id and the partners b > a
+// in traversal order (neighbour cells k, j, i; ids ascending in a cell), as
+// cluster.cpp:174-275 accumulates; then the Heun stage, the wrap, the next
+// sort and (stage 2) the spread lines.  A tile whose halo holds an
+// overflowing cell or more records than the LDS takes the exact
+// global-memory path (contact_slow_tab).
 constexpr int TILE_X = 8, TILE_Y = 8, TILE_Z = 2;
 constexpr int TILE_CELLS = TILE_X * TILE_Y * TILE_Z;   // 128 cells, ~170 spheres at 20 %
 constexpr int HALO_X = TILE_X + 2, HALO_Y = TILE_Y + 2, HALO_Z = TILE_Z + 2;
@@ -451,26 +419,137 @@ __device__ __forceinline__ int halo_nb(int hc, int n)
     return hc + dx + HALO_X * (dy + HALO_Y * dz);
 }
 
-template <int STAGE>
-__global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S, CellTab cur, CellTab nxt, LineTab lt,
-                                                      const int *__restrict__ t_dev, double ramp_t, double rmp_host,
-                                                      int *__restrict__ overflow)
+// Grid values of c, grad c (and u) at a particle: field.cpp:223-248 as
+// element.cpp:55-71 calls it, box offset applied by the caller.
+template <typename T>
+struct StageFields {
+    const T *c, *g0, *g1, *g2, *u0, *u1, *u2;
+};
+
+// Between the two stages of a step: it_mark of the sorts of stage 1 and
+// stage 2 (cluster.cpp:128-138: the strictly largest y > 0, first index on
+// ties; kept when no y > 0), from the argmax partials of the table stage 1
+// read (positions at the step's start) and of the table it filled
+// (positions after stage 1, also in p1), and whether the attachment force of
+// the stage-2 mark is non-zero at its position (the tau gate of stage 2,
+// cluster.cpp:256-259).  One workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void k_mark(PartParams P, PartState S, StageFields<T> F, TabSet tabs, int nparts,
+                                              const int *__restrict__ t_dev)
+{
+    __shared__ MarkC smk[4];
+    const int k3 = (2 * *t_dev) % 3;
+    const MarkPart *m0 = tabs.t[k3].mp, *m1 = tabs.t[(k3 + 1) % 3].mp;
+    MarkC a{0.0, 0.0, 0.0, -1}, b{0.0, 0.0, 0.0, -1};
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+        const MarkPart q0 = m0[k], q1 = m1[k];
+        mark_take(a, MarkC{q0.x, q0.y, q0.z, q0.idx});
+        mark_take(b, MarkC{q1.x, q1.y, q1.z, q1.idx});
+    }
+    a = block_mark(a, smk);
+    b = block_mark(b, smk);
+    if (threadIdx.x != 0) return;
+    int mk = b.i;
+    double mp[3] = {b.x, b.y, b.z};
+    if (mk < 0) {
+        mk = a.i >= 0 ? a.i : *S.mark;
+        if (mk >= 0) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) mp[d] = S.p1[d][mk];
+        }
+    }
+    bool on = false;
+    if (mk >= 0) {
+        *S.mark = mk;
+        double phi = 0.0, gr[3] = {0.0, 0.0, 0.0}, f[3];
+        if (grid_cell_ok(P, mp[0], mp[1], mp[2])) {
+            const size_t bo = (size_t)part_box(S.box, mk) * P.rs;
+            const Tri tr = tri_of<T>(P, mp[0], mp[1], mp[2]);
+            phi = tri_sum(tr, F.c + bo);
+            gr[0] = tri_sum(tr, F.g0 + bo);
+            gr[1] = tri_sum(tr, F.g1 + bo);
+            gr[2] = tri_sum(tr, F.g2 + bo);
+        }
+        capillary_force(P, phi, gr[0], gr[1], gr[2], S.rad[mk], P.has_ctheta ? S.cth[mk] : 0.0, f);
+        on = vnorm(f[0], f[1], f[2]) != 0.0;
+    }
+    *S.mark_on = on ? 1 : 0;
+}
+
+// The reference's unused physics of one stage, per particle at its
+// stage-start position (p for stage 1, p1 for stage 2), before k_stage:
+// uf from u (element.cpp:43-49 / field.cpp:251-281) and the Brownian force
+// of this step and stage (element.cpp:109-144).
+template <typename T>
+__global__ __launch_bounds__(256) void k_fluid_noise(PartParams P, PartState S, StageFields<T> F,
+                                                     const int *__restrict__ t_dev, int stage)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= P.n) return;
+    const double *const *pp = stage == 1 ? S.p : S.p1;
+    const double px = pp[0][a], py = pp[1][a], pz = pp[2][a];
+    if (P.interp_u && grid_cell_ok(P, px, py, pz)) {
+        const size_t bo = (size_t)part_box(S.box, a) * P.rs;
+        const T *us[3] = {F.u0 + bo, F.u1 + bo, F.u2 + bo};
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            S.uf[d][a] = P.interp_u == 1 ? interp1(P, us[d], px, py, pz) : interp2(P, us[d], px, py, pz);
+    }
+    if (P.brown_amp > 0.0) {
+        const int t = *t_dev;
+#pragma unroll
+        for (int d = 0; d < 3; d++) S.frand[d][a] = P.brown_amp * brownian_g(P.noise_seed, t, stage, a, d, P.pi);
+    }
+}
+
+// lines_on: stage 1 resets the spread's overflow list and node pool for
+// stage 2; stage 2 appends each particle to its spread lines.  Three waves
+// per SIMD (168 VGPRs): the tiles of a 256^3 box then fit the GPU at once.
+template <int STAGE, typename T>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_stage(PartParams P, PartState S, StageFields<T> F, TabSet tabs, int ncell,
+                                               LineTab lt, int lines_on, const int *__restrict__ t_dev,
+                                               double ramp_t, int *__restrict__ overflow)
 {
     __shared__ int hoff[HALO_N + 1];
-    __shared__ CellRec rec[TILE_RECS];
+    // the halo's records: exact positions, and an FP32 copy with the id (w)
+    // for the reject test of the contact search
+    __shared__ double rxs[TILE_RECS], rys[TILE_RECS], rzs[TILE_RECS];
+    __shared__ float4 rq[TILE_RECS];
     __shared__ int plist[TILE_RECS];   // the tile's particles: LDS record
     __shared__ short pcell[TILE_RECS];  // and halo cell
     __shared__ int wsum[4];
+    __shared__ MarkC smk[4];
+    auto rec_at = [&](int e) { return CellRec{rxs[e], rys[e], rzs[e], __float_as_int(rq[e].w), 0}; };
     const int tid = threadIdx.x;
     const int ntx = tiles_of(P.ncx, TILE_X), nty = tiles_of(P.ncy, TILE_Y), ntz = tiles_of(P.ncz, TILE_Z);
     const int box = blockIdx.z / ntz;
     const int x0 = blockIdx.x * TILE_X, y0 = blockIdx.y * TILE_Y, z0 = (blockIdx.z % ntz) * TILE_Z;
     const int cb = box * P.ncell_box;
-    const double rmp = t_dev ? ramp_dev((double)(*t_dev + 1), ramp_t) : rmp_host;
+    const int tnow = *t_dev;
+    const double rmp = ramp_dev((double)(tnow + 1), ramp_t);
     // slightly above r_eq^2: below it the exact root decides
     const double r_eq2_hi = P.r_eq * P.r_eq * (1.0 + 1e-12);
     (void)nty;
     (void)ntx;
+    const int nwg = (int)(gridDim.x * gridDim.y * gridDim.z);
+    const int wg = (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
+    const int k3 = (2 * tnow + STAGE - 1) % 3;
+    const CellTab cur = tabs.t[k3], nxt = tabs.t[(k3 + 1) % 3], clr = tabs.t[(k3 + 2) % 3];
+    // 0. the table the next stage appends to, emptied (the stage before this
+    // one read it); stage 1 also resets the spread lines' shared lists
+    for (int q = wg * 256 + tid; q < ncell; q += nwg * 256) clr.cnt[q] = 0;
+    if (wg == 0 && tid == 0) {
+        *clr.ovf_cnt = 0;
+        if (STAGE == 1 && lines_on) {
+            *lt.ovf_cnt = 0;
+            *lt.pool_top = 0;
+        }
+    }
+    const size_t bo = (size_t)box * P.rs;
+    // stage 2: whether it_mark's attachment force is non-zero (the tau
+    // gate, cluster.cpp:256-259), from k_mark; stage 1 keeps no tau (stage 2
+    // overwrites it), so it needs no gate
+    const int gate = STAGE == 2 ? *S.mark_on : 0;
     // 1. record counts of the halo cells (HALO_PER consecutive cells a thread)
     int cntv[HALO_PER], cov[HALO_PER];
     int over = 0;
@@ -516,15 +595,21 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
             const int h = HALO_PER * tid + q;
             if (h >= HALO_N) continue;
             const int b0 = hoff[h], m = cntv[q];
-            for (int e = 0; e < m; e++) rec[b0 + e] = cur.rec[(size_t)cov[q] * CELL_CAP + e];
-            for (int i = b0 + 1; i < b0 + m; i++) {
-                const CellRec v = rec[i];
-                int j = i - 1;
-                while (j >= b0 && rec[j].id > v.id) {
-                    rec[j + 1] = rec[j];
+            // insertion by id while loading (a cell holds a few records)
+            for (int e = 0; e < m; e++) {
+                const CellRec v = cur.rec[(size_t)cov[q] * CELL_CAP + e];
+                int j = b0 + e - 1;
+                while (j >= b0 && __float_as_int(rq[j].w) > v.id) {
+                    rxs[j + 1] = rxs[j];
+                    rys[j + 1] = rys[j];
+                    rzs[j + 1] = rzs[j];
+                    rq[j + 1] = rq[j];
                     j--;
                 }
-                rec[j + 1] = v;
+                rxs[j + 1] = v.x;
+                rys[j + 1] = v.y;
+                rzs[j + 1] = v.z;
+                rq[j + 1] = make_float4((float)v.x, (float)v.y, (float)v.z, __int_as_float(v.id));
             }
         }
     }
@@ -536,32 +621,82 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
     const bool cell_on = tid < TILE_CELLS && gx < P.ncx && gy < P.ncy && gz < P.ncz;
     const int hc_t = (lx + 1) + HALO_X * ((ly + 1) + HALO_Y * (lz + 1));
     const int co = cb + gx + P.ncx * (gy + P.ncy * gz);
-    double by = 0.0;  // argmax-y of this thread's particles after the stage
-    int bi = -1;
-    // per-particle inputs of the stage's end, loaded before the contact
-    // search so that their latency hides behind it
+    MarkC mine{0.0, 0.0, 0.0, -1};  // argmax-y of this thread's particles after the stage
+    // Per particle before the contact search: the interpolation of c and
+    // grad c at its stage-start position and f_pi (interpolate_local_variables
+    // and compute_f_pi, main.cpp:198-200), and the inputs of the stage's end.
+    // Stage 2 stores the particle's local values (the step's last ones).  uf
+    // and the Brownian force (off in the reference step) come from
+    // k_fluid_noise, launched before this kernel when they are on.
     struct Pre {
         double fpi[3], r, po[3], uo[3];
     };
-    // uf and the Brownian force (off in the reference step) are read at the
-    // stage's end only, so that they hold no registers across the search
     const bool with_uf = P.interp_u != 0, with_noise = P.brown_amp > 0.0;
-    auto prefetch = [&](int a) {
+    auto prefetch = [&](int a, double ax, double ay, double az) {
         Pre q;
+        q.r = S.rad[a];
+        const double ct = P.has_ctheta ? S.cth[a] : 0.0;
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            q.fpi[d] = S.fpi[d][a];
             q.po[d] = STAGE == 2 ? S.po[d][a] : 0.0;
             q.uo[d] = STAGE == 2 ? S.uo[d][a] : 0.0;
         }
-        q.r = S.rad[a];
+        // c and grad c at the particle.  Both stages read the same fields
+        // (c^{n+1}): stage 1 keeps the 32 corner values it gathers, and stage
+        // 2 takes them when the particle is still in the same grid cell (the
+        // same corners, so the same values and the same sums), two
+        // contiguous lines instead of sixteen scattered ones.
+        double phi = 0.0, gr[3] = {0.0, 0.0, 0.0};
+        if (!grid_cell_ok(P, ax, ay, az)) {
+            atomicOr(S.err, 2);
+        } else {
+            const Tri tr = tri_of<T>(P, ax, ay, az);
+            T *kept = reinterpret_cast<T *>(S.nodes) + (size_t)a * 32;
+            const T *fld[4] = {F.c + bo, F.g0 + bo, F.g1 + bo, F.g2 + bo};
+            double val[4];
+            if (STAGE == 2 && S.gnode[a] == tr.off[0]) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) val[f] = tri_dot(tr, kept + 8 * f);
+            } else if (STAGE == 1) {
+#pragma unroll
+                for (int f = 0; f < 4; f++) {
+                    T v[8];
+                    tri_gather(tr, fld[f], v);
+#pragma unroll
+                    for (int c8 = 0; c8 < 8; c8++) kept[8 * f + c8] = v[c8];
+                    val[f] = tri_dot(tr, v);
+                }
+                S.gnode[a] = tr.off[0];
+            } else {
+#pragma unroll
+                for (int f = 0; f < 4; f++) val[f] = tri_sum(tr, fld[f]);
+            }
+            phi = val[0];
+            gr[0] = val[1];
+            gr[1] = val[2];
+            gr[2] = val[3];
+        }
+        capillary_force(P, phi, gr[0], gr[1], gr[2], q.r, ct, q.fpi);
+        if (STAGE == 2) {
+            S.cloc[a] = phi;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                S.dcdx[d][a] = gr[d];
+                S.fpi[d][a] = q.fpi[d];
+            }
+        }
         return q;
     };
+    // Stage 1 keeps its results in the next table's records and writes by
+    // id only what stage 2 and the caller read (p_old, u_old, and the new
+    // position to p1); stage 2 writes the step's particle state.
     auto finish = [&](int a, double ax, double ay, double az, const double *fo, double tau, const Pre &q) {
-        S.fpp[0][a] = fo[0];
-        S.fpp[1][a] = fo[1];
-        S.fpp[2][a] = fo[2];
-        S.tau_i[a] = tau;
+        if (STAGE == 2) {
+            S.fpp[0][a] = fo[0];
+            S.fpp[1][a] = fo[1];
+            S.fpp[2][a] = fo[2];
+            S.tau_i[a] = tau;
+        }
         // Heun stage (element.cpp:156-173), periodic wrap
         // (element.cpp:177-186)
         const double r = q.r;
@@ -583,15 +718,16 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
                 x = q.po[d] + (q.uo[d] + v) * (0.5 * P.dt);
             }
             np[d] = fmod(x + L[d], L[d]);
-            S.p[d][a] = np[d];
+            if (STAGE == 1) S.p1[d][a] = np[d];
+            else S.p[d][a] = np[d];
         }
         // the sort of the next stage (cluster.cpp:116-169)
         int ci, cj, ck;
         const int cc = cell_of(P, np[0], np[1], np[2], &ci, &cj, &ck);
         if (cc < 0) atomicOr(S.err, 1);
         else cell_append(P, nxt, cc + cb, np[0], np[1], np[2], a);
-        argmax_take(by, bi, np[1], np[1] > 0.0 ? a : -1);
-        if (STAGE == 2 && lt.cnt) {
+        if (np[1] > 0.0) mark_take(mine, MarkC{np[0], np[1], np[2], a});
+        if (STAGE == 2 && lines_on) {
             // the spread of f_pp at the new position for the next step's force
             int l4[4];
             if (!spread_lines(P, np[0], np[1], np[2], box, l4)) {
@@ -604,9 +740,7 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
     };
     // tau counts when f_pi of the particle and of it_mark are non-zero
     // (cluster.cpp:256-259)
-    const int mk = *S.mark;
-    const bool mark_on = mk >= 0 && vnorm(S.fpi[0][mk], S.fpi[1][mk], S.fpi[2][mk]) != 0.0;
-    auto tau_gate = [&](const Pre &q) { return mark_on && vnorm(q.fpi[0], q.fpi[1], q.fpi[2]) != 0.0; };
+    auto tau_gate = [&](const Pre &q) { return gate != 0 && vnorm(q.fpi[0], q.fpi[1], q.fpi[2]) != 0.0; };
     if (fb) {
         // exact path from global memory (overflowing cells or a crowded
         // halo), one tile cell a thread
@@ -615,7 +749,7 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
             int o_, pp_, q_;
             cell_visit(cur, co, [&](const CellRec &ra, int) {
                 const int a = ra.id;
-                const Pre q = prefetch(a);
+                const Pre q = prefetch(a, ra.x, ra.y, ra.z);
                 cell_of(P, ra.x, ra.y, ra.z, &o_, &pp_, &q_);
                 double fo[3], tau = 0.0;
                 contact_slow_tab(P, cur, a, ra.x, ra.y, ra.z, o_, pp_, q_, cb, rmp, tau_gate(q), fo, &tau);
@@ -635,12 +769,21 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
             pcell[pb + k] = (short)hc_t;
         }
         __syncthreads();
+        // FP32 reject test: a pair within r_eq has |d| < r_eq + 1e-4 in FP32
+        // (positions below 2^24 ulps of the box), so only pairs that pass
+        // (d^2 <= (r_eq + 1e-3)^2) take the exact FP64 test; where the
+        // minimum-image threshold comes within 1e-2 of r_eq every pair takes it
+        const float Lf[3] = {(float)P.lx, (float)P.ly, (float)P.lz};
+        const float thf[3] = {(float)(P.dcx * (P.ncx - 2)), (float)(P.dcy * (P.ncy - 2)), (float)(P.dcz * (P.ncz - 2))};
+        const double thmin = fmin(P.dcx * (P.ncx - 2), fmin(P.dcy * (P.ncy - 2), P.dcz * (P.ncz - 2)));
+        const float lim2 = thmin - P.r_eq > 1e-2 ? (float)((P.r_eq + 1e-3) * (P.r_eq + 1e-3)) : 3.0e38f;
         for (int pi = tid; pi < ptot; pi += 256) {
-            const CellRec ra = rec[plist[pi]];
+            const CellRec ra = rec_at(plist[pi]);
             const int hc = pcell[pi];
             const int a = ra.id;
-            const Pre q = prefetch(a);
             const double ax = ra.x, ay = ra.y, az = ra.z;
+            const Pre q = prefetch(a, ax, ay, az);
+            const float axf = (float)ax, ayf = (float)ay, azf = (float)az;
             uint64_t hit[HIT_WORDS];
 #pragma unroll
             for (int w = 0; w < HIT_WORDS; w++) hit[w] = 0;
@@ -650,7 +793,14 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
                 const int hm = hc + HALO_X * (row % 3 - 1 + HALO_Y * (row / 3 - 1));  // the dx = 0 cell
                 const int r0 = hoff[hm - 1], r1 = hoff[hm], r2 = hoff[hm + 1], r3 = hoff[hm + 2];
                 for (int e = r0; e < r3; e++) {
-                    const CellRec rb = rec[e];
+                    const float4 bq = rq[e];
+                    float dv[3] = {axf - bq.x, ayf - bq.y, azf - bq.z};
+#pragma unroll
+                    for (int c3 = 0; c3 < 3; c3++)
+                        dv[c3] = dv[c3] > thf[c3] ? dv[c3] - Lf[c3] : (dv[c3] < -thf[c3] ? dv[c3] + Lf[c3] : dv[c3]);
+                    const float d2f = dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2];
+                    if (d2f > lim2) continue;
+                    const CellRec rb = rec_at(e);
                     if (rb.id == a) continue;
                     if (in_contact(P, ax, ay, az, a, rb, r_eq2_hi)) {
                         // neighbour n = 3 row + j (j: which of the three cells)
@@ -667,7 +817,7 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
             }
             auto rec_of = [&](int bit) {
                 const int n = bit / CELL_CAP;
-                return rec[hoff[halo_nb(hc, n)] + bit % CELL_CAP];
+                return rec_at(hoff[halo_nb(hc, n)] + bit % CELL_CAP);
             };
             double fx = 0.0, fy = 0.0, fz = 0.0, tau = 0.0;
             // partners b < a in ascending id
@@ -717,13 +867,12 @@ __global__ __launch_bounds__(256) void k_contact_tile(PartParams P, PartState S,
             finish(a, ax, ay, az, fo, tau, q);
         }
     }
-    // argmax-y partial of this workgroup's particles (block_argmax_y takes
-    // one candidate per thread)
-    block_argmax_y(bi >= 0 ? by : -1.0, bi, nxt.ymax, nxt.yidx,
-                   (int)(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)));
+    // argmax-y partial of this workgroup's particles for the next stage
+    const MarkC r = block_mark(mine, smk);
+    if (tid == 0) nxt.mp[wg] = MarkPart{r.x, r.y, r.z, r.i, 0};
 }
 
-// workgroups of k_contact_tile for a slab
+// workgroups of k_stage (and of k_cells_append) for a slab
 inline dim3 contact_tile_grid(const PartParams &P)
 {
     return dim3((unsigned)tiles_of(P.ncx, TILE_X), (unsigned)tiles_of(P.ncy, TILE_Y),
