@@ -688,12 +688,13 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N, EY>::P)) void k_y(cpl
     const int z = row_;
     const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, 0, z);
     const uint32_t st = d.nxp;
+    auto fld = [&](int f) { return f == 0 ? a0 : (f == 1 ? a1 : a2); };
     C v[NF][E];
 #pragma unroll
-    for (int f = 0; f < NF; f++) tile_load<T, N, EY>(v[f], f == 0 ? a0 : (f == 1 ? a1 : a2), base, st, t, ok);
+    for (int f = 0; f < NF; f++) tile_load<T, N, EY>(v[f], fld(f), base, st, t, ok);
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-        C *a = f == 0 ? a0 : (f == 1 ? a1 : a2);
+        C *a = fld(f);
         if (GRAD && f == 0) {
             C w[E];
 #pragma unroll
@@ -980,7 +981,9 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_stoke
         // gives zero).  The branch also keeps one point's FP64 temporaries
         // live at a time.
         if (k2 != 0.0) {
-            const double ik2 = 1.0 / k2;
+            // the reciprocal within one ulp (recip_nr), not an IEEE division:
+            // a third of the FP32 hub's VALU time went to the division
+            const double ik2 = recip_nr(k2);
             const double cu = ik2 * kp.inv_mu_g;
 #pragma unroll
             for (int e = 0; e < 3; e++) {

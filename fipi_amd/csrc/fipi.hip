@@ -467,11 +467,11 @@ struct Impl {
         LineTab lt = ctx->lines;
         ctx->spread_ready = false;  // the pool now holds these nodes
         {
-            ProfScope ps_(ctx, "spread_lines", 24.0 * pp.n + 16.0 * pp.n);
+            ProfScope ps_(ctx, "spread_lines", 48.0 * pp.n + 4.0 * sizeof(LineEnt<T>) * pp.n);
             hipLaunchKernelGGL(k_tab_clear, dim3(1), dim3(64), 0, st, lt.cnt, 0, lt.ovf_cnt, (int *)nullptr,
                                lt.pool_top);
-            hipLaunchKernelGGL(k_spread_lines, grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, box, lt,
-                               ctx->d_flags);
+            hipLaunchKernelGGL((k_spread_lines<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, f0, f1, f2,
+                               box, lt, ctx->d_flags);
         }
         spread_nodes(ctx, pp, px, py, pz, f0, f1, f2);
     }
@@ -480,7 +480,8 @@ struct Impl {
                              const double *pz, const double *f0, const double *f1, const double *f2)
     {
         const int nl = ctx->nlines;
-        ProfScope ps_(ctx, "spread_nodes", 16.0 * pp.n + 4 * 48.0 * pp.n + 8.0 * pp.n * sizeof(SpreadNode<T>) + 12.0 * nl);
+        // the four line entries of each particle in, its node sums out
+        ProfScope ps_(ctx, "spread_nodes", 4.0 * sizeof(LineEnt<T>) * pp.n + 8.0 * pp.n * sizeof(SpreadNode<T>) + 12.0 * nl);
         hipLaunchKernelGGL((k_spread_nodes<T>), grid1(nl, NODE_LPB), dim3(256), 0, ctx->stream, pp, ctx->lines, px, py,
                            pz, f0, f1, f2, (SpreadNode<T> *)ctx->spr_tmp, nl);
     }
@@ -593,12 +594,6 @@ struct Impl {
                                (const T *)R(ctx->g[2]), (const T *)R(ctx->u[0]), (const T *)R(ctx->u[1]),
                                (const T *)R(ctx->u[2])};
         const dim3 grid = contact_tile_grid(pp);
-        if (stage == 2) {
-            // it_mark of both sorts and the tau gate of stage 2
-            ProfScope ps_(ctx, "mark", 64.0 * grid.x * grid.y * grid.z);
-            hipLaunchKernelGGL((k_mark<T>), dim3(1), dim3(256), 0, st, pp, ps, F, ctx->tabs,
-                               (int)(grid.x * grid.y * grid.z), (const int *)ctx->t_dev);
-        }
         if (pp.interp_u || pp.brown_amp > 0.0)
             hipLaunchKernelGGL((k_fluid_noise<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, ps, F,
                                (const int *)ctx->t_dev, stage);
@@ -1066,7 +1061,7 @@ int setup_buffers(fipi_ctx *ctx)
     {
         LineTab &lt = ctx->lines;
         lt.cnt = ctx->line_cnt;
-        ALLOC(lt.pid, sizeof(int) * LINE_CAP * (size_t)ctx->nlines);
+        ALLOC(lt.ent, sizeof(LineEnt<T>) * LINE_CAP * (size_t)ctx->nlines);
         ALLOC(lt.ovf_cnt, sizeof(int));
         ALLOC(lt.ovf, sizeof(int2) * 4 * n);
         lt.node_base = ctx->line_start;
@@ -1627,7 +1622,11 @@ static int diag_box(fipi_ctx *ctx, int b, int p0, int p1, fipi_diag *out)
     }
     out->free_energy = h[1] * cell;
     out->interface_area = h[2] * cell;
-    out->tau = h[3];
+    // the tau terms count only when it_mark's attachment force is non-zero
+    // (cluster.cpp:256-259); the step's stage 2 records that gate
+    int mark_on = 0;
+    if (ctx->n > 0) HIPCHK(ctx, hipMemcpy(&mark_on, ctx->ps.mark_on, sizeof(int), hipMemcpyDeviceToHost));
+    out->tau = mark_on ? h[3] : 0.0;
     return FIPI_OK;
 }
 
@@ -1980,7 +1979,7 @@ int tuning_in(fipi_ctx *ctx, const fipi_tuning_t *in)
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr int32_t SNAP_VERSION = 1;
+constexpr int32_t SNAP_VERSION = 2;
 
 struct SnapHeader {
     char magic[8];
@@ -2021,6 +2020,7 @@ std::vector<std::pair<void *, size_t>> snapshot_buffers(fipi_ctx *ctx)
         v.push_back({ps.cloc, D});
         v.push_back({ps.tau_i, D});
         v.push_back({ps.mark, sizeof(int)});
+        v.push_back({ps.mark_on, sizeof(int)});
         if (ps.box) v.push_back({(void *)ps.box, sizeof(int) * n});
     }
     return v;

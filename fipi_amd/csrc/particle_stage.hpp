@@ -101,7 +101,7 @@ __device__ __forceinline__ MarkC block_mark(MarkC m, MarkC *sm)
 
 struct LineTab {
     int *cnt;        // [nlines] particles appended (may exceed LINE_CAP)
-    int *pid;        // [nlines * LINE_CAP]
+    void *ent;       // [nlines * LINE_CAP] LineEnt<T>
     int *ovf_cnt;    // entries in the overflow list
     int2 *ovf;       // [4n] (line, particle) beyond a line's capacity
     int *node_base;  // [nlines] first node of each line in the pool
@@ -143,14 +143,76 @@ __device__ __forceinline__ bool spread_lines(const PartParams &P, double px, dou
     return true;
 }
 
-__device__ __forceinline__ void line_append(LineTab T, int line, int id)
+// The spread entries of particle a on line (yl, zl): the corners of
+// cluster.cpp:368-406 whose (y, z) is the line's, at x and x+1, with the
+// weights w = f * (1-o) * (1-p) * (1-q) etc. evaluated in that order;
+// f = (f_pp + f_ext)/dx/dy/dz, f_ext = 0.
+template <typename T>
+__device__ __forceinline__ void line_entries(const PartParams &P, double px, double py, double pz, double f0,
+                                             double f1, double f2, int yl, int zl, int *x0, int *x1, T *w0, T *w1)
 {
-    const int slot = atomicAdd(&T.cnt[line], 1);
-    if (slot < LINE_CAP) {
-        T.pid[(size_t)line * LINE_CAP + slot] = id;
-    } else {
-        const int o = atomicAdd(T.ovf_cnt, 1);
-        T.ovf[o] = make_int2(line, id);
+    int x, y, z;
+    double o, p, q;
+    spread_corner(P, px, py, pz, &x, &y, &z, &o, &p, &q);
+    const double f[3] = {(f0 + 0.0) / P.dx / P.dy / P.dz, (f1 + 0.0) / P.dx / P.dy / P.dz,
+                         (f2 + 0.0) / P.dx / P.dy / P.dz};
+    const bool ylo = y == yl, zlo = z == zl;
+    *x0 = x;
+    *x1 = (x + 1) % P.nx;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        double a, b;  // at x, at x1
+        if (ylo && zlo) {        // corners 0 (x) and 1 (x1)
+            a = f[d] * (1 - o) * (1 - p) * (1 - q);
+            b = f[d] * o * (1 - p) * (1 - q);
+        } else if (!ylo && zlo) {  // corners 6 (x) and 2 (x1)
+            a = f[d] * (1 - o) * p * (1 - q);
+            b = f[d] * o * p * (1 - q);
+        } else if (ylo) {          // corners 5 (x) and 3 (x1)
+            a = f[d] * (1 - o) * (1 - p) * q;
+            b = f[d] * o * (1 - p) * q;
+        } else {                   // corners 7 (x) and 4 (x1)
+            a = f[d] * (1 - o) * p * q;
+            b = f[d] * o * p * q;
+        }
+        w0[d] = (T)a;
+        w1[d] = (T)b;
+    }
+}
+
+// A particle's two spread entries on one grid line, formed when it is
+// appended, so that the node kernel reads them in place: x of the first
+// node (the second is x + 1, periodic), the particle id, and the weighted
+// forces at the two nodes (line_entries).
+template <typename T>
+struct LineEnt {
+    int x0, id;
+    T w0[3], w1[3];
+};
+
+// Appends particle id at (px, py, pz) with force (f0, f1, f2) to the four
+// lines l4 of its spread (overflow: (line, id), the exact path recomputes
+// the entries from the particle arrays).
+template <typename T>
+__device__ __forceinline__ void line_append4(const PartParams &P, LineTab Lt, const int *l4, double px, double py,
+                                             double pz, double f0, double f1, double f2, int id)
+{
+    LineEnt<T> *ent = reinterpret_cast<LineEnt<T> *>(Lt.ent);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int line = l4[k];
+        const int slot = atomicAdd(&Lt.cnt[line], 1);
+        if (slot < LINE_CAP) {
+            const int lb = line % P.nlines_box;
+            LineEnt<T> e;
+            int x1;
+            line_entries<T>(P, px, py, pz, f0, f1, f2, lb % P.ny, lb / P.ny, &e.x0, &x1, e.w0, e.w1);
+            e.id = id;
+            ent[(size_t)line * LINE_CAP + slot] = e;
+        } else {
+            const int o = atomicAdd(Lt.ovf_cnt, 1);
+            Lt.ovf[o] = make_int2(line, id);
+        }
     }
 }
 
@@ -193,20 +255,24 @@ __global__ void k_tab_clear(int *cnt, int n, int *ovf_cnt, int *line_ovf, int *p
     }
 }
 
-// spread lines of n particles (the spread outside the step's stage 2)
+// spread lines of n particles with forces f (the spread outside the step's
+// stage 2)
+template <typename T>
 __global__ __launch_bounds__(256) void k_spread_lines(PartParams P, const double *__restrict__ px,
                                                       const double *__restrict__ py, const double *__restrict__ pz,
-                                                      const int *__restrict__ box, LineTab T, int *__restrict__ err)
+                                                      const double *__restrict__ f0, const double *__restrict__ f1,
+                                                      const double *__restrict__ f2, const int *__restrict__ box,
+                                                      LineTab Lt, int *__restrict__ err)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= P.n) return;
     int l4[4];
-    if (!spread_lines(P, px[a], py[a], pz[a], part_box(box, a), l4)) {
+    const double x = px[a], y = py[a], z = pz[a];
+    if (!spread_lines(P, x, y, z, part_box(box, a), l4)) {
         atomicOr(err, 2);
         return;
     }
-#pragma unroll
-    for (int q = 0; q < 4; q++) line_append(T, l4[q], a);
+    line_append4<T>(P, Lt, l4, x, y, z, f0[a], f1[a], f2[a], a);
 }
 
 // ---------------------------------------------------------------------------
@@ -426,19 +492,20 @@ struct StageFields {
     const T *c, *g0, *g1, *g2, *u0, *u1, *u2;
 };
 
-// Between the two stages of a step: it_mark of the sorts of stage 1 and
-// stage 2 (cluster.cpp:128-138: the strictly largest y > 0, first index on
-// ties; kept when no y > 0), from the argmax partials of the table stage 1
-// read (positions at the step's start) and of the table it filled
-// (positions after stage 1, also in p1), and whether the attachment force of
-// the stage-2 mark is non-zero at its position (the tau gate of stage 2,
-// cluster.cpp:256-259).  One workgroup.
+// it_mark of the sorts of stage 1 and stage 2 (cluster.cpp:128-138: the
+// strictly largest y > 0, first index on ties; kept when no y > 0), from the
+// argmax partials of the table stage 1 read (positions at the step's start)
+// and of the table it filled (positions after stage 1, also in p1), and
+// whether the attachment force of the stage-2 mark is non-zero at its
+// position (the tau gate, cluster.cpp:256-259; diagnostics apply it to the
+// sum of the particles' tau terms).  One workgroup of stage 2 runs it beside
+// its tile (nothing it reads is written during stage 2); every thread of
+// that workgroup calls it.
 template <typename T>
-__global__ __launch_bounds__(256) void k_mark(PartParams P, PartState S, StageFields<T> F, TabSet tabs, int nparts,
-                                              const int *__restrict__ t_dev)
+__device__ void stage_mark(const PartParams &P, const PartState &S, const StageFields<T> &F, const TabSet &tabs,
+                           int nparts, int tnow, MarkC *smk)
 {
-    __shared__ MarkC smk[4];
-    const int k3 = (2 * *t_dev) % 3;
+    const int k3 = (2 * tnow) % 3;
     const MarkPart *m0 = tabs.t[k3].mp, *m1 = tabs.t[(k3 + 1) % 3].mp;
     MarkC a{0.0, 0.0, 0.0, -1}, b{0.0, 0.0, 0.0, -1};
     for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
@@ -546,10 +613,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         }
     }
     const size_t bo = (size_t)box * P.rs;
-    // stage 2: whether it_mark's attachment force is non-zero (the tau
-    // gate, cluster.cpp:256-259), from k_mark; stage 1 keeps no tau (stage 2
-    // overwrites it), so it needs no gate
-    const int gate = STAGE == 2 ? *S.mark_on : 0;
+    // stage 2: it_mark and the tau gate, in the first workgroup
+    if (STAGE == 2 && wg == 0) stage_mark<T>(P, S, F, tabs, nwg, tnow, smk);
     // 1. record counts of the halo cells (HALO_PER consecutive cells a thread)
     int cntv[HALO_PER], cov[HALO_PER];
     int over = 0;
@@ -730,17 +795,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
         if (STAGE == 2 && lines_on) {
             // the spread of f_pp at the new position for the next step's force
             int l4[4];
-            if (!spread_lines(P, np[0], np[1], np[2], box, l4)) {
-                atomicOr(S.err, 2);
-            } else {
-#pragma unroll
-                for (int k = 0; k < 4; k++) line_append(lt, l4[k], a);
-            }
+            if (!spread_lines(P, np[0], np[1], np[2], box, l4)) atomicOr(S.err, 2);
+            else line_append4<T>(P, lt, l4, np[0], np[1], np[2], fo[0], fo[1], fo[2], a);
         }
     };
     // tau counts when f_pi of the particle and of it_mark are non-zero
-    // (cluster.cpp:256-259)
-    auto tau_gate = [&](const Pre &q) { return gate != 0 && vnorm(q.fpi[0], q.fpi[1], q.fpi[2]) != 0.0; };
+    // (cluster.cpp:256-259): the particle's part here, it_mark's part
+    // (mark_on) where the terms are summed; stage 1 keeps no tau (stage 2
+    // overwrites it)
+    auto tau_gate = [&](const Pre &q) { return STAGE == 2 && vnorm(q.fpi[0], q.fpi[1], q.fpi[2]) != 0.0; };
     if (fb) {
         // exact path from global memory (overflowing cells or a crowded
         // halo), one tile cell a thread
@@ -882,43 +945,6 @@ inline dim3 contact_tile_grid(const PartParams &P)
 // ---------------------------------------------------------------------------
 // Spread nodes (cluster.cpp:344-410) per grid line
 // ---------------------------------------------------------------------------
-// The spread entries of particle a on line (yl, zl): the corners of
-// cluster.cpp:368-406 whose (y, z) is the line's, at x and x+1, with the
-// weights w = f * (1-o) * (1-p) * (1-q) etc. evaluated in that order;
-// f = (f_pp + f_ext)/dx/dy/dz, f_ext = 0.
-template <typename T>
-__device__ __forceinline__ void line_entries(const PartParams &P, double px, double py, double pz, double f0,
-                                             double f1, double f2, int yl, int zl, int *x0, int *x1, T *w0, T *w1)
-{
-    int x, y, z;
-    double o, p, q;
-    spread_corner(P, px, py, pz, &x, &y, &z, &o, &p, &q);
-    const double f[3] = {(f0 + 0.0) / P.dx / P.dy / P.dz, (f1 + 0.0) / P.dx / P.dy / P.dz,
-                         (f2 + 0.0) / P.dx / P.dy / P.dz};
-    const bool ylo = y == yl, zlo = z == zl;
-    *x0 = x;
-    *x1 = (x + 1) % P.nx;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        double a, b;  // at x, at x1
-        if (ylo && zlo) {        // corners 0 (x) and 1 (x1)
-            a = f[d] * (1 - o) * (1 - p) * (1 - q);
-            b = f[d] * o * (1 - p) * (1 - q);
-        } else if (!ylo && zlo) {  // corners 6 (x) and 2 (x1)
-            a = f[d] * (1 - o) * p * (1 - q);
-            b = f[d] * o * p * (1 - q);
-        } else if (ylo) {          // corners 5 (x) and 3 (x1)
-            a = f[d] * (1 - o) * (1 - p) * q;
-            b = f[d] * o * (1 - p) * q;
-        } else {                   // corners 7 (x) and 4 (x1)
-            a = f[d] * (1 - o) * p * q;
-            b = f[d] * o * p * q;
-        }
-        w0[d] = (T)a;
-        w1[d] = (T)b;
-    }
-}
-
 constexpr int NODE_LANES = 16;                 // lanes per grid line
 constexpr int NODE_LPB = 256 / NODE_LANES;     // lines per workgroup
 constexpr int NODE_MAXE = 2 * LINE_CAP;        // entries per line on the fast path
@@ -966,10 +992,9 @@ __global__ __launch_bounds__(256) void k_spread_nodes(PartParams P, LineTab lt, 
     for (int t = 0; t < NODE_PER / 2; t++) {
         const int k = lane + t * W;
         if (k < m) {
-            const int a = lt.pid[(size_t)line * LINE_CAP + k];
-            int x0, x1;
-            T w0[3], w1[3];
-            line_entries<T>(P, px[a], py[a], pz[a], f0[a], f1[a], f2[a], yl, zl, &x0, &x1, w0, w1);
+            const LineEnt<T> e = reinterpret_cast<const LineEnt<T> *>(lt.ent)[(size_t)line * LINE_CAP + k];
+            const int a = e.id, x0 = e.x0, x1 = (x0 + 1) % P.nx;
+            const T *w0 = e.w0, *w1 = e.w1;
             mk[2 * t] = ((uint64_t)(uint32_t)x0 << 32) | (uint32_t)a;
             mk[2 * t + 1] = ((uint64_t)(uint32_t)x1 << 32) | (uint32_t)a;
             key[sub][2 * k] = mk[2 * t];
@@ -1056,7 +1081,7 @@ __global__ __launch_bounds__(256) void k_spread_nodes(PartParams P, LineTab lt, 
     const int tail = ((nlines + NODE_LPB - 1) / NODE_LPB) * NODE_REGION;
     const int no = *lt.ovf_cnt;
     auto visit = [&](auto &&f) {
-        for (int k = 0; k < LINE_CAP; k++) f(lt.pid[(size_t)line * LINE_CAP + k]);
+        for (int k = 0; k < LINE_CAP; k++) f(reinterpret_cast<const LineEnt<T> *>(lt.ent)[(size_t)line * LINE_CAP + k].id);
         for (int k = 0; k < no; k++)
             if (lt.ovf[k].x == line) f(lt.ovf[k].y);
     };
