@@ -197,3 +197,34 @@ def test_headline_density_128(gpu):
         assert e[0] < TOL and e[1] < TOL and e[2] < TOL, (sim.t, e)
         assert e[3] < 1e-8, (sim.t, e)
     sim.close()
+
+
+def test_particles_leaving_their_grid_cell_32(gpu):
+    """Stiff contacts between overlapping spheres (k_n = 20, dt = 0.05):
+    spheres cross grid cells within Heun stage 1, so stage 2 cannot reuse
+    the corner values stage 1 gathered and takes them from the fields.  The
+    oracle run shows that such crossings happen; FP64 against the oracle
+    every step, tau included."""
+    print("grid-cell crossings between the stages", flush=True)
+    p, phi0, _ = make_case(32, 150, steps=20, seed=7)
+    p = p.replace(k_n=20.0, dt=0.05)
+    pos = fipi_amd.init_rsa(150, 2.0, -1.0, (p.lx, p.ly, p.lz), 7)
+    sim = fipi_amd.Simulation(p)
+    sim.set_state(phi0, pos)
+    ref = oracle.Sim(p, phi0, pos)
+    L = np.array([p.lx, p.ly, p.lz])
+    crossings = 0
+    for t in range(10):
+        sim.step(1)
+        ref.step(1)
+        rp, gp = ref.particles(), sim.particles()
+        p1 = np.fmod(rp["p_old"] + p.dt * rp["u_old"] + L, L)
+        crossings += int(np.any(np.floor(rp["p_old"]) != np.floor(p1), axis=1).sum())
+        assert rel_l2(sim.phi(), ref.phi()) < TOL, t
+        assert rel_l2(gp["p"], rp["p"]) < TOL, (t, rel_l2(gp["p"], rp["p"]))
+        assert rel_l2(gp["f_pi"], rp["f_pi"]) < 1e-8, t
+        assert rel_l2(gp["f_pp"], rp["f_pp"]) < 1e-8, t
+        tau_g, tau_r = sim.diagnostics()["tau"], ref.tau()
+        assert abs(tau_g - tau_r) <= 1e-10 * max(1.0, abs(tau_r)), (t, tau_g, tau_r)
+    sim.close()
+    assert crossings > 0
