@@ -108,6 +108,10 @@ struct fipi_ctx {
     // snapshots, the tuner).  Off where the particle stage interpolates u
     // and in the ensemble and adjoint contexts.
     bool u_lazy = false, u_stale = false;
+    // the step's particle stages wrote the per-particle blocks (Carry,
+    // StepOut) but not the id-indexed arrays the API reads; unpack_particles
+    // forms them before anything outside the step reads them
+    bool parts_stale = false;
     // calibration extension: wetting source W (real, added to f; null in the
     // reference physics) and particles held at their positions (no Heun
     // stages; their contact forces are spread every step unchanged)
@@ -360,6 +364,16 @@ struct Impl {
         ctx->u_stale = false;
     }
 
+    // the id-indexed particle arrays from the last step's blocks, after the
+    // particle stream has ended
+    static void unpack_particles(fipi_ctx *ctx)
+    {
+        if (!ctx->parts_stale || ctx->n == 0) return;
+        if (ctx->p_pending) (void)hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+        hipLaunchKernelGGL((k_unpack<T>), grid1(ctx->n, 256), dim3(256), 0, ctx->stream, ctx->ps, ctx->n);
+        ctx->parts_stale = false;
+    }
+
     template <int SIGN, int NF, bool GRAD>
     static void y_pass(fipi_ctx *ctx, C *a0, C *a1, C *a2, C *extra)
     {
@@ -473,17 +487,19 @@ struct Impl {
             hipLaunchKernelGGL((k_spread_lines<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, f0, f1, f2,
                                box, lt, ctx->d_flags);
         }
-        spread_nodes(ctx, pp, px, py, pz, f0, f1, f2);
+        spread_nodes(ctx, pp, px, py, pz, f0, f1, f2, nullptr);
     }
 
+    // so: forces of the overflow path from the StepOut blocks (after stage 2)
     static void spread_nodes(fipi_ctx *ctx, const PartParams &pp, const double *px, const double *py,
-                             const double *pz, const double *f0, const double *f1, const double *f2)
+                             const double *pz, const double *f0, const double *f1, const double *f2,
+                             const StepOut *so)
     {
         const int nl = ctx->nlines;
         // the four line entries of each particle in, its node sums out
         ProfScope ps_(ctx, "spread_nodes", 4.0 * sizeof(LineEnt<T>) * pp.n + 8.0 * pp.n * sizeof(SpreadNode<T>) + 12.0 * nl);
         hipLaunchKernelGGL((k_spread_nodes<T>), grid1(nl, NODE_LPB), dim3(256), 0, ctx->stream, pp, ctx->lines, px, py,
-                           pz, f0, f1, f2, (SpreadNode<T> *)ctx->spr_tmp, nl);
+                           pz, f0, f1, f2, so, (SpreadNode<T> *)ctx->spr_tmp, nl);
     }
 
     // CSR cell list with ids ascending per cell (unit entries, adjoint)
@@ -614,9 +630,11 @@ struct Impl {
                                    ctx->d_flags + 1);
         }
         if (spread) {
-            spread_nodes(ctx, pp, ps.p[0], ps.p[1], ps.p[2], ps.fpp[0], ps.fpp[1], ps.fpp[2]);
+            spread_nodes(ctx, pp, ps.p[0], ps.p[1], ps.p[2], ps.fpp[0], ps.fpp[1], ps.fpp[2],
+                         (const StepOut *)ps.out);
             ctx->spread_ready = true;
         }
+        if (stage == 2) ctx->parts_stale = true;
     }
 
     // beside: this step's particle stage runs on pstream after the step's
@@ -1080,7 +1098,6 @@ int setup_buffers(fipi_ctx *ctx)
     PartState &ps = ctx->ps;
     for (int a = 0; a < 3; a++) {
         ALLOC(ps.p[a], sizeof(double) * n);
-        ALLOC(ps.p1[a], sizeof(double) * n);
         ALLOC(ps.po[a], sizeof(double) * n);
         ALLOC(ps.u[a], sizeof(double) * n);
         ALLOC(ps.uo[a], sizeof(double) * n);
@@ -1095,8 +1112,8 @@ int setup_buffers(fipi_ctx *ctx)
     ALLOC(ps.cth, sizeof(double) * n);
     ALLOC(ps.cloc, sizeof(double) * n);
     ALLOC(ps.tau_i, sizeof(double) * n);
-    ALLOC(ps.nodes, sizeof(T) * 32 * n);
-    ALLOC(ps.gnode, sizeof(uint32_t) * n);
+    ALLOC(ps.carry, sizeof(Carry<T>) * n);
+    ALLOC(ps.out, sizeof(StepOut) * n);
     ALLOC(ps.cell, sizeof(int) * n);
     if (ctx->nbox > 1) {
         int *bx;
@@ -1225,6 +1242,13 @@ static void ensure_real_u(fipi_ctx *ctx)
     else Impl<float>::materialize_u(ctx);
 }
 
+// the id-indexed particle arrays up to date (they lag while parts_stale)
+static void ensure_particles(fipi_ctx *ctx)
+{
+    if (ctx->fp64) Impl<double>::unpack_particles(ctx);
+    else Impl<float>::unpack_particles(ctx);
+}
+
 // A context of nbox equal boxes (the slab) with n_total particles in all.
 static int create_ctx(const fipi_params *params, int nbox, int n_total, fipi_ctx **out)
 {
@@ -1327,6 +1351,7 @@ static int set_state_slab(fipi_ctx *ctx, const double *phi, const double *pos, c
     ctx->cur = 0;
     ctx->conv_ready = false;
     ctx->u_stale = false;
+    ctx->parts_stale = false;
     ctx->spread_ready = false;
     ctx->tables_ready = false;
     {
@@ -1439,6 +1464,7 @@ static int set_state_dev(fipi_ctx *ctx, const double *d_phi, const double *d_pos
     ctx->cur = 0;
     ctx->conv_ready = false;
     ctx->u_stale = false;
+    ctx->parts_stale = false;
     ctx->spread_ready = false;
     ctx->tables_ready = false;
     {
@@ -1557,6 +1583,7 @@ int fipi_get_particles(fipi_ctx *ctx, double *pos, double *pos_old, double *u, d
     if (!ctx) return set_err(ctx, FIPI_ERR_ARG, "ctx is NULL");
     const int n = ctx->n;
     if (n == 0) return FIPI_OK;
+    ensure_particles(ctx);
     double *tmp;
     HIPCHK(ctx, hipMalloc((void **)&tmp, sizeof(double) * 3 * n));
     double *const *src[6] = {ctx->ps.p, ctx->ps.po, ctx->ps.u, ctx->ps.uo, ctx->ps.fpp, ctx->ps.fpi};
@@ -1586,6 +1613,7 @@ static int diag_box(fipi_ctx *ctx, int b, int p0, int p1, fipi_diag *out)
 {
     if (!ctx || !out) return set_err(ctx, FIPI_ERR_ARG, "ctx and out are required");
     if (b < 0 || b >= ctx->nbox) return set_err(ctx, FIPI_ERR_ARG, "box out of range");
+    ensure_particles(ctx);
     hipStream_t st = ctx->stream;
     const double dx = ctx->p.lx / ctx->p.nx;
     const size_t off = (size_t)b * ctx->G * ctx->esz;
@@ -1902,6 +1930,7 @@ int fipi_get_particle_fluid(fipi_ctx *ctx, double *uf, double *f_rand, double *f
     const int n = ctx->n;
     if (n == 0) return FIPI_OK;
     if (ctx->p_pending) HIPCHK(ctx, hipStreamSynchronize(ctx->pstream));
+    ensure_particles(ctx);
     const PartState &ps = ctx->ps;
     double *tmp;
     HIPCHK(ctx, hipMalloc((void **)&tmp, sizeof(double) * 3 * n));
@@ -2043,6 +2072,7 @@ int fipi_snapshot_save(fipi_ctx *ctx, void *buf, int64_t size)
     if (!ctx || !buf) return set_err(ctx, FIPI_ERR_ARG, "ctx and buf are required");
     if (!ctx->has_state) return set_err(ctx, FIPI_ERR_STATE, "no state to save");
     ensure_real_u(ctx);
+    ensure_particles(ctx);
     const int64_t need = fipi_snapshot_size(ctx);
     if (size < need) return set_err(ctx, FIPI_ERR_OVERFLOW, "snapshot buffer too small");
     SnapHeader h{};
@@ -2099,6 +2129,7 @@ int fipi_snapshot_load(fipi_ctx *ctx, const void *buf, int64_t size)
     ctx->cur = h.cur;
     ctx->conv_ready = h.conv_ready != 0;
     ctx->u_stale = false;
+    ctx->parts_stale = false;
     ctx->pp.has_ctheta = h.has_ctheta;
     ctx->tune.y_e = h.y_e;
     ctx->spread_ready = false;
@@ -2123,6 +2154,7 @@ int fipi_tune(fipi_ctx *ctx, int32_t steps)
         }
     } keep;
     ensure_real_u(ctx);
+    ensure_particles(ctx);
     size_t total = 0;
     for (auto &x : snapshot_buffers(ctx)) total += x.second;
     HIPCHK(ctx, hipMalloc(&keep.p, total));
@@ -2182,6 +2214,7 @@ int fipi_tune(fipi_ctx *ctx, int32_t steps)
     ctx->cur = cur0;
     ctx->conv_ready = conv0;
     ctx->u_stale = false;
+    ctx->parts_stale = false;
     ctx->spread_ready = false;
     ctx->tables_ready = false;
     return rc;

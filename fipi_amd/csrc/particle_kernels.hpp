@@ -45,13 +45,15 @@ struct PartParams {
 
 struct PartState {
     double *p[3], *po[3], *u[3], *uo[3], *fpp[3], *fpi[3];
-    double *p1[3];               // positions after Heun stage 1 of the step
     double *rad, *cth;
     double *cloc, *dcdx[3];
     double *uf[3], *frand[3];    // local fluid velocity and Brownian force of the last stage
     double *tau_i;               // per-particle packing-energy contributions
-    void *nodes;                 // [n][4][8] grid values (c, grad c at the 8 corners) stage 1 gathered, as T
-    uint32_t *gnode;             // [n] their corner 0 (byte offset in the box's field)
+    // per-particle blocks of the step's particle stage (particle_stage.hpp):
+    // Carry<T> (stage 1 to stage 2) and StepOut (the step's results, from
+    // which u, u_old, p_old, f_pp, f_pi, grad c, c and tau above are formed
+    // on demand)
+    void *carry, *out;
     double *sp[3];               // positions in cell-sorted order (sp[d][s] = p[d][ids[s]])
     int *cell;                   // contact cell of each particle
     int *err;                    // device error flags (bit 0: sorting range)

@@ -68,6 +68,26 @@ struct TabSet {
     CellTab t[3];
 };
 
+// What Heun stage 1 hands to stage 2 for one particle, one contiguous block
+// indexed by id (written once by stage 1, read once by stage 2): the 32
+// grid values stage 1 gathered (c and grad c at the 8 corners, as T) and
+// their corner 0 (byte offset in the box's field), p_old and u_old, the
+// position after stage 1, the radius and cos theta.
+template <typename T>
+struct alignas(128) Carry {
+    T nodes[32];
+    double po[3], uo[3], p1[3], r, ct;
+    uint32_t gnode, pad;
+};
+
+// The step's results for one particle, written by stage 2 in one 128-byte
+// block: the velocity of stage 2, f_pp, f_pi, grad c and c at the
+// particle, and its tau term.  The id-indexed arrays of PartState are
+// formed from these blocks and the Carry blocks on demand (k_unpack).
+struct alignas(128) StepOut {
+    double u[3], fpp[3], fpi[3], dcdx[3], cloc, tau;
+};
+
 // argmax-y candidate: (y descending, index ascending) is a total order
 // with "none" (i < 0) below every candidate, so combines in any order give
 // the same result
@@ -495,7 +515,7 @@ struct StageFields {
 // it_mark of the sorts of stage 1 and stage 2 (cluster.cpp:128-138: the
 // strictly largest y > 0, first index on ties; kept when no y > 0), from the
 // argmax partials of the table stage 1 read (positions at the step's start)
-// and of the table it filled (positions after stage 1, also in p1), and
+// and of the table it filled (positions after stage 1, also in Carry), and
 // whether the attachment force of the stage-2 mark is non-zero at its
 // position (the tau gate, cluster.cpp:256-259; diagnostics apply it to the
 // sum of the particles' tau terms).  One workgroup of stage 2 runs it beside
@@ -522,7 +542,7 @@ __device__ void stage_mark(const PartParams &P, const PartState &S, const StageF
         mk = a.i >= 0 ? a.i : *S.mark;
         if (mk >= 0) {
 #pragma unroll
-            for (int d = 0; d < 3; d++) mp[d] = S.p1[d][mk];
+            for (int d = 0; d < 3; d++) mp[d] = reinterpret_cast<const Carry<T> *>(S.carry)[mk].p1[d];
         }
     }
     bool on = false;
@@ -544,7 +564,7 @@ __device__ void stage_mark(const PartParams &P, const PartState &S, const StageF
 }
 
 // The reference's unused physics of one stage, per particle at its
-// stage-start position (p for stage 1, p1 for stage 2), before k_stage:
+// stage-start position (p for stage 1, Carry::p1 for stage 2), before k_stage:
 // uf from u (element.cpp:43-49 / field.cpp:251-281) and the Brownian force
 // of this step and stage (element.cpp:109-144).
 template <typename T>
@@ -553,8 +573,9 @@ __global__ __launch_bounds__(256) void k_fluid_noise(PartParams P, PartState S, 
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= P.n) return;
-    const double *const *pp = stage == 1 ? S.p : S.p1;
-    const double px = pp[0][a], py = pp[1][a], pz = pp[2][a];
+    const Carry<T> &cy = reinterpret_cast<const Carry<T> *>(S.carry)[a];
+    const double px = stage == 1 ? S.p[0][a] : cy.p1[0], py = stage == 1 ? S.p[1][a] : cy.p1[1],
+                 pz = stage == 1 ? S.p[2][a] : cy.p1[2];
     if (P.interp_u && grid_cell_ok(P, px, py, pz)) {
         const size_t bo = (size_t)part_box(S.box, a) * P.rs;
         const T *us[3] = {F.u0 + bo, F.u1 + bo, F.u2 + bo};
@@ -694,17 +715,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     // and the Brownian force (off in the reference step) come from
     // k_fluid_noise, launched before this kernel when they are on.
     struct Pre {
-        double fpi[3], r, po[3], uo[3];
+        double fpi[3], r, ct, po[3], uo[3];
     };
     const bool with_uf = P.interp_u != 0, with_noise = P.brown_amp > 0.0;
+    Carry<T> *const carry = reinterpret_cast<Carry<T> *>(S.carry);
+    StepOut *const sout = reinterpret_cast<StepOut *>(S.out);
     auto prefetch = [&](int a, double ax, double ay, double az) {
         Pre q;
-        q.r = S.rad[a];
-        const double ct = P.has_ctheta ? S.cth[a] : 0.0;
+        const Carry<T> &cy = carry[a];
+        if (STAGE == 1) {
+            q.r = S.rad[a];
+            q.ct = P.has_ctheta ? S.cth[a] : 0.0;
+        } else {
+            q.r = cy.r;
+            q.ct = cy.ct;
+        }
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            q.po[d] = STAGE == 2 ? S.po[d][a] : 0.0;
-            q.uo[d] = STAGE == 2 ? S.uo[d][a] : 0.0;
+            q.po[d] = STAGE == 2 ? cy.po[d] : 0.0;
+            q.uo[d] = STAGE == 2 ? cy.uo[d] : 0.0;
         }
         // c and grad c at the particle.  Both stages read the same fields
         // (c^{n+1}): stage 1 keeps the 32 corner values it gathers, and stage
@@ -716,22 +745,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             atomicOr(S.err, 2);
         } else {
             const Tri tr = tri_of<T>(P, ax, ay, az);
-            T *kept = reinterpret_cast<T *>(S.nodes) + (size_t)a * 32;
             const T *fld[4] = {F.c + bo, F.g0 + bo, F.g1 + bo, F.g2 + bo};
             double val[4];
-            if (STAGE == 2 && S.gnode[a] == tr.off[0]) {
+            if (STAGE == 2 && cy.gnode == tr.off[0]) {
 #pragma unroll
-                for (int f = 0; f < 4; f++) val[f] = tri_dot(tr, kept + 8 * f);
+                for (int f = 0; f < 4; f++) val[f] = tri_dot(tr, cy.nodes + 8 * f);
             } else if (STAGE == 1) {
+                Carry<T> &cw = carry[a];
 #pragma unroll
                 for (int f = 0; f < 4; f++) {
                     T v[8];
                     tri_gather(tr, fld[f], v);
 #pragma unroll
-                    for (int c8 = 0; c8 < 8; c8++) kept[8 * f + c8] = v[c8];
+                    for (int c8 = 0; c8 < 8; c8++) cw.nodes[8 * f + c8] = v[c8];
                     val[f] = tri_dot(tr, v);
                 }
-                S.gnode[a] = tr.off[0];
+                cw.gnode = tr.off[0];
             } else {
 #pragma unroll
                 for (int f = 0; f < 4; f++) val[f] = tri_sum(tr, fld[f]);
@@ -741,26 +770,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             gr[1] = val[2];
             gr[2] = val[3];
         }
-        capillary_force(P, phi, gr[0], gr[1], gr[2], q.r, ct, q.fpi);
+        capillary_force(P, phi, gr[0], gr[1], gr[2], q.r, q.ct, q.fpi);
         if (STAGE == 2) {
-            S.cloc[a] = phi;
+            StepOut &o = sout[a];
+            o.cloc = phi;
 #pragma unroll
             for (int d = 0; d < 3; d++) {
-                S.dcdx[d][a] = gr[d];
-                S.fpi[d][a] = q.fpi[d];
+                o.dcdx[d] = gr[d];
+                o.fpi[d] = q.fpi[d];
             }
         }
         return q;
     };
-    // Stage 1 keeps its results in the next table's records and writes by
-    // id only what stage 2 and the caller read (p_old, u_old, and the new
-    // position to p1); stage 2 writes the step's particle state.
+    // Stage 1 hands its results to stage 2 in the particle's Carry block;
+    // stage 2 writes the step's results to its StepOut block and the new
+    // position to p.
     auto finish = [&](int a, double ax, double ay, double az, const double *fo, double tau, const Pre &q) {
         if (STAGE == 2) {
-            S.fpp[0][a] = fo[0];
-            S.fpp[1][a] = fo[1];
-            S.fpp[2][a] = fo[2];
-            S.tau_i[a] = tau;
+            StepOut &o = sout[a];
+            o.fpp[0] = fo[0];
+            o.fpp[1] = fo[1];
+            o.fpp[2] = fo[2];
+            o.tau = tau;
         }
         // Heun stage (element.cpp:156-173), periodic wrap
         // (element.cpp:177-186)
@@ -775,16 +806,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
             const double v = heun_velocity(P, q.fpi[d], fext, fo[d], r, uf);
             double x;
             if (STAGE == 1) {
-                S.uo[d][a] = v;
-                S.po[d][a] = p0[d];
+                carry[a].uo[d] = v;
+                carry[a].po[d] = p0[d];
                 x = p0[d] + v * P.dt;
             } else {
-                S.u[d][a] = v;
+                sout[a].u[d] = v;
                 x = q.po[d] + (q.uo[d] + v) * (0.5 * P.dt);
             }
             np[d] = fmod(x + L[d], L[d]);
-            if (STAGE == 1) S.p1[d][a] = np[d];
+            if (STAGE == 1) carry[a].p1[d] = np[d];
             else S.p[d][a] = np[d];
+        }
+        if (STAGE == 1) {
+            carry[a].r = r;
+            carry[a].ct = q.ct;
         }
         // the sort of the next stage (cluster.cpp:116-169)
         int ci, cj, ck;
@@ -963,12 +998,14 @@ __host__ __device__ constexpr size_t spread_pool_nodes(int nlines, int n)
 // in ascending id; node_base / node_cnt locate them in the pool and the
 // line's counter is cleared.  Lines beyond LINE_CAP particles take an exact
 // repeated-selection path over the overflow list.
+// so (optional): the forces of the overflow path come from the particles'
+// StepOut blocks (the step's stage 2) instead of f0, f1, f2.
 template <typename T>
 __global__ __launch_bounds__(256) void k_spread_nodes(PartParams P, LineTab lt, const double *__restrict__ px,
                                                       const double *__restrict__ py, const double *__restrict__ pz,
                                                       const double *__restrict__ f0, const double *__restrict__ f1,
-                                                      const double *__restrict__ f2, SpreadNode<T> *__restrict__ nodes,
-                                                      int nlines)
+                                                      const double *__restrict__ f2, const StepOut *__restrict__ so,
+                                                      SpreadNode<T> *__restrict__ nodes, int nlines)
 {
     constexpr int W = NODE_LANES;
     __shared__ uint64_t key[NODE_LPB][NODE_MAXE];
@@ -1097,7 +1134,7 @@ __global__ __launch_bounds__(256) void k_spread_nodes(PartParams P, LineTab lt, 
             visit([&](int a) {
                 int x0, x1;
                 T w0[3], w1[3];
-                line_entries<T>(P, px[a], py[a], pz[a], f0[a], f1[a], f2[a], yl, zl, &x0, &x1, w0, w1);
+                line_entries<T>(P, px[a], py[a], pz[a], so ? so[a].fpp[0] : f0[a], so ? so[a].fpp[1] : f1[a], so ? so[a].fpp[2] : f2[a], yl, zl, &x0, &x1, w0, w1);
                 if (x0 > last_x && x0 < nxn) nxn = x0;
                 if (x1 > last_x && x1 < nxn) nxn = x1;
             });
@@ -1112,7 +1149,7 @@ __global__ __launch_bounds__(256) void k_spread_nodes(PartParams P, LineTab lt, 
                         if (a <= last_id || a >= nid) return;
                         int x0, x1;
                         T w0[3], w1[3];
-                        line_entries<T>(P, px[a], py[a], pz[a], f0[a], f1[a], f2[a], yl, zl, &x0, &x1, w0, w1);
+                        line_entries<T>(P, px[a], py[a], pz[a], so ? so[a].fpp[0] : f0[a], so ? so[a].fpp[1] : f1[a], so ? so[a].fpp[2] : f2[a], yl, zl, &x0, &x1, w0, w1);
                         if (x0 == nxn) { nid = a; wn[0] = w0[0]; wn[1] = w0[1]; wn[2] = w0[2]; }
                         else if (x1 == nxn) { nid = a; wn[0] = w1[0]; wn[1] = w1[1]; wn[2] = w1[2]; }
                     });
@@ -1139,6 +1176,29 @@ __global__ __launch_bounds__(256) void k_spread_nodes(PartParams P, LineTab lt, 
         }
     }
     lt.cnt[line] = 0;
+}
+
+// The id-indexed particle arrays from the step's blocks: p_old and u_old
+// from Carry, u, f_pp, f_pi, grad c, c and tau from StepOut (for the API,
+// snapshots and diagnostics, after steps whose stages wrote only the blocks).
+template <typename T>
+__global__ __launch_bounds__(256) void k_unpack(PartState S, int n)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const Carry<T> &cy = reinterpret_cast<const Carry<T> *>(S.carry)[a];
+    const StepOut &o = reinterpret_cast<const StepOut *>(S.out)[a];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        S.po[d][a] = cy.po[d];
+        S.uo[d][a] = cy.uo[d];
+        S.u[d][a] = o.u[d];
+        S.fpp[d][a] = o.fpp[d];
+        S.fpi[d][a] = o.fpi[d];
+        S.dcdx[d][a] = o.dcdx[d];
+    }
+    S.cloc[a] = o.cloc;
+    S.tau_i[a] = o.tau;
 }
 
 }  // namespace fipi
