@@ -228,3 +228,20 @@ def test_particles_leaving_their_grid_cell_32(gpu):
         assert abs(tau_g - tau_r) <= 1e-10 * max(1.0, abs(tau_r)), (t, tau_g, tau_r)
     sim.close()
     assert crossings > 0
+
+
+@pytest.mark.parametrize("n,npart", [(32, 2000), (64, 12000)])
+def test_crowded_cells_32_64(gpu, n, npart):
+    """Small overlapping spheres (r = 0.5, contact distance 1) crowded into
+    6-dx contact cells: 16 per cell on average at 32^3, so cells exceed the
+    table's 16 slots and tiles take the exact path from global memory; at
+    64^3 fast and exact tiles mix.  FP64 against the oracle."""
+    print(f"crowded cells {n}^3 + {npart}", flush=True)
+    p, phi0, _ = make_case(n, npart, r_p=0.5, steps=20, seed=41)
+    pos = fipi_amd.init_rsa(npart, 0.5, -0.3, (p.lx, p.ly, p.lz), 41)
+    assert pos.shape[0] == npart
+    for t, g_phi, r_phi, g_u, r_u, gp, rp in _run_both(p, phi0, pos, 6, chunks=3):
+        assert rel_l2(g_phi, r_phi) < TOL, (t, rel_l2(g_phi, r_phi))
+        assert rel_l2(g_u, r_u) < TOL, (t, rel_l2(g_u, r_u))
+        assert rel_l2(gp["p"], rp["p"]) < TOL, (t, rel_l2(gp["p"], rp["p"]))
+        assert rel_l2(gp["f_pp"], rp["f_pp"]) < 1e-8, (t, rel_l2(gp["f_pp"], rp["f_pp"]))
