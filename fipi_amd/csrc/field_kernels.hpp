@@ -733,15 +733,39 @@ struct KParams {
 };
 
 // FP64 k = 0 mode (FP32-mixed path): k0[cur] = c_hat^n(0), k0[cur^1] gets
-// c_hat^{n+1}(0), k0[2] = history(0); conv0 = FP64 sum of conv (its k = 0
-// transform).  k0 == nullptr: the k = 0 mode is carried like every other.
-// Box b's values are at k0 + K0_STRIDE*b and conv0 + b.
+// c_hat^{n+1}(0), k0[2] = history(0); conv(0) = FP64 sum of conv (its k = 0
+// transform), formed by the hub's workgroup that holds k = 0 from the x
+// pass's partial sums and kept in conv0.  k0 == nullptr: the k = 0 mode is
+// carried like every other.  Box b's values are at k0 + K0_STRIDE*b,
+// conv0 + b and part + nb*b.
 constexpr int K0_STRIDE = 4;
 struct K0 {
     double *k0;
-    const double *conv0;
+    double *conv0;       // out: the FP64 sum of conv per box
+    const double *part;  // the x pass's per-workgroup FP64 sums of conv, nb per box
+    int nb;
     int cur;
 };
+
+// Fixed-order FP64 sum of part[0..nb): 256 strided partial sums, then a
+// tree (k_sum_strided's order for one box), by any number of threads; the
+// result is valid in every thread.  s: 256 doubles of LDS.
+__device__ __forceinline__ double fixed_sum256(const double *__restrict__ part, int nb, double *s)
+{
+    for (int v = threadIdx.x; v < 256; v += blockDim.x) {
+        double a = 0;
+        for (int i = v; i < nb; i += 256) a += part[i];
+        s[v] = a;
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        for (int v = threadIdx.x; v < o; v += blockDim.x) s[v] += s[v + o];
+        __syncthreads();
+    }
+    const double r = s[0];
+    __syncthreads();
+    return r;
+}
 
 // P3: CH update.  conv (XY) -> conv_hat; with c_hat^n, f_hat^n and the
 // history term hist = -c^{n-1} + 2dtM k^2 f^{n-1} + 2dt conv^{n-1}:
@@ -774,6 +798,12 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_ch(
     const int y = row_;
     const uint32_t base = (uint32_t)hidx(d, ok ? kx : 0, y, 0);
     const uint32_t st = (uint32_t)d.nxp * d.ny;
+    // the FP64 mass mode's conv(0), in the workgroup that holds k = 0
+    double cv0 = 0.0;
+    if (k0.k0 != nullptr && kp.use_conv && tile_ == 0 && y == 0) {
+        cv0 = fixed_sum256(k0.part + (size_t)zb * k0.nb, k0.nb, reinterpret_cast<double *>(lds));
+        if (threadIdx.x == 0) k0.conv0[zb] = cv0;
+    }
     C v[E], cn[E], fn[E], hh[E];
     if (kp.use_conv) tile_load<T, N>(v, conv_io, base, st, t, ok);
     tile_load<T, N>(cn, c_now, base, st, t, ok);
@@ -834,7 +864,6 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_ch(
             // mass mode in FP64: k^2 = 0, so only c, conv and history enter
             double *kb = k0.k0 + K0_STRIDE * zb;
             const double c0 = kb[k0.cur], h0 = kb[2];
-            const double cv0 = kp.use_conv ? k0.conv0[zb] : 0.0;
             double n0;
             if constexpr (FIRST) n0 = c0 - cv0 * kp.dt;
             else n0 = (c0 * 4.0 - cv0 * (4.0 * kp.dt) + h0) / 3.0;

@@ -662,13 +662,6 @@ struct Impl {
                 x_forward<1>(ctx, pr, h0, nullptr, mixed ? ctx->conv_part : nullptr);
             }
             ctx->conv_ready = false;
-            if (mixed) {
-                // FP64 conv(0) = sum of conv over the grid, fixed order
-                const int nb = xblocks<T>(d);
-                ProfScope ps_(ctx, "k0_sum", 8.0 * nb);
-                hipLaunchKernelGGL(k_sum_strided, dim3(ctx->nbox), dim3(256), 0, st, (const double *)ctx->conv_part,
-                                   nb, 1, 1, ctx->conv0);
-            }
             ProfScope ps_(ctx, "y_fwd_1", 2 * Hb);
             y_pass<-1, 1, false>(ctx, h0, nullptr, nullptr, nullptr);
         }
@@ -682,7 +675,8 @@ struct Impl {
                 constexpr int N = decltype(NC)::value;
                 with_tb(ctx->tune.hub_tb, [&](auto TBC) {
                     constexpr int TB = decltype(TBC)::value;
-                    const K0 k0{mixed ? ctx->k0 : nullptr, ctx->conv0, ctx->cur};
+                    // FP64 conv(0) = sum of conv over the grid, fixed order, in the hub
+                    const K0 k0{mixed ? ctx->k0 : nullptr, ctx->conv0, ctx->conv_part, xblocks<T>(d), ctx->cur};
                     if (kp.first)
                         hipLaunchKernelGGL((k_hub_ch<T, N, TB, true>), zgrid<TB>(ctx), (zblock<TB, N>()), 0, st, h0,
                                            h1, ns ? h2 : nullptr, (const C *)ctx->ck[ctx->cur], H(ctx->ck[nxt]),
