@@ -1010,8 +1010,8 @@ __global__ __launch_bounds__((TileW<T, TB>::L * Line<T, N>::P)) void k_hub_stoke
         // gives zero).  The branch also keeps one point's FP64 temporaries
         // live at a time.
         if (k2 != 0.0) {
-            // the reciprocal within one ulp (recip_nr), not an IEEE division:
-            // a third of the FP32 hub's VALU time went to the division
+            // the reciprocal within one ulp (recip_nr): an IEEE double
+            // division is a much longer instruction sequence
             const double ik2 = recip_nr(k2);
             const double cu = ik2 * kp.inv_mu_g;
 #pragma unroll
