@@ -245,3 +245,23 @@ def test_crowded_cells_32_64(gpu, n, npart):
         assert rel_l2(g_u, r_u) < TOL, (t, rel_l2(g_u, r_u))
         assert rel_l2(gp["p"], rp["p"]) < TOL, (t, rel_l2(gp["p"], rp["p"]))
         assert rel_l2(gp["f_pp"], rp["f_pp"]) < 1e-8, (t, rel_l2(gp["f_pp"], rp["f_pp"]))
+
+
+def test_crowded_spread_lines_32(gpu):
+    """8,000 small spheres (r = 0.5) in a 32^3 box: about 31 spread entries
+    per grid line on average, so that many lines exceed the line table's 32
+    slots and take the exact overflow path of the spread, and the spheres in
+    the overflow lists get no corner copies from the force pass (stage 1
+    gathers their corner values from the fields).  Contact cells of 2 dx
+    hold two spheres on average, so the contact search stays on its fast
+    path.  FP64 against the oracle."""
+    print("crowded spread lines 32^3 + 8000", flush=True)
+    n, npart = 32, 8000
+    p, phi0, pos = make_case(n, npart, r_p=0.5, steps=20, seed=43, gap=-0.3, cell=2.0)
+    for t, g_phi, r_phi, g_u, r_u, gp, rp in _run_both(p, phi0, pos, 6, chunks=3):
+        assert rel_l2(g_phi, r_phi) < TOL, (t, rel_l2(g_phi, r_phi))
+        assert rel_l2(g_u, r_u) < TOL, (t, rel_l2(g_u, r_u))
+        assert rel_l2(gp["p"], rp["p"]) < TOL, (t, rel_l2(gp["p"], rp["p"]))
+        assert rel_l2(gp["f_pi"], rp["f_pi"]) < 1e-8, (t, rel_l2(gp["f_pi"], rp["f_pi"]))
+        assert rel_l2(gp["f_pp"], rp["f_pp"]) < 1e-8, (t, rel_l2(gp["f_pp"], rp["f_pp"]))
+
