@@ -483,7 +483,7 @@ struct Impl {
         {
             ProfScope ps_(ctx, "spread_lines", 48.0 * pp.n + 4.0 * sizeof(LineEnt<T>) * pp.n);
             hipLaunchKernelGGL(k_tab_clear, dim3(1), dim3(64), 0, st, lt.cnt, 0, lt.ovf_cnt, (int *)nullptr,
-                               lt.pool_top);
+                               lt.pool_top, (int *)nullptr);
             hipLaunchKernelGGL((k_spread_lines<T>), grid1(pp.n, 256), dim3(256), 0, st, pp, px, py, pz, f0, f1, f2,
                                box, lt, ctx->d_flags);
         }
@@ -584,7 +584,7 @@ struct Impl {
         const CellTab &T0 = ctx->tabs.t[(2 * ctx->t) % 3], &T1 = ctx->tabs.t[(2 * ctx->t + 1) % 3];
         for (const CellTab *tb : {&T0, &T1})
             hipLaunchKernelGGL(k_tab_clear, grid1(ctx->ncell, 256), dim3(256), 0, ctx->stream, tb->cnt, ctx->ncell,
-                               tb->ovf_cnt, (int *)nullptr, (int *)nullptr);
+                               tb->ovf_cnt, (int *)nullptr, (int *)nullptr, tb->ovf_head);
         const dim3 tg = contact_tile_grid(pp);
         hipLaunchKernelGGL(k_cells_append, dim3(tg.x * tg.y * tg.z), dim3(256), 0, ctx->stream, pp,
                            (const double *)ps.p[0], (const double *)ps.p[1], (const double *)ps.p[2], ps.box, T0,
@@ -1062,6 +1062,8 @@ int setup_buffers(fipi_ctx *ctx)
         ALLOC(tb.rec, sizeof(CellRec) * CELL_CAP * (size_t)ctx->ncell);
         ALLOC(tb.ovf_cnt, sizeof(int));
         ALLOC(tb.ovf, sizeof(CellRec) * n);
+        ALLOC(tb.ovf_head, sizeof(int) * (ctx->ncell + 1));
+        HIPCHK(ctx, hipMemset(tb.ovf_head, 0xff, sizeof(int) * (ctx->ncell + 1)));  // -1: no chain
         {
             // argmax partials: one per stage workgroup
             const dim3 tg = contact_tile_grid(pp);

@@ -36,10 +36,11 @@ constexpr int CELL_CAP = 16;  // records per contact cell before overflow
 constexpr int LINE_CAP = 32;  // particles per grid line before overflow
 
 // A particle as the cell table holds it: its position at the time of the
-// append and its id (cell: the global cell, used by overflow records).
+// append and its id (next: in the overflow list, the next record of the
+// same cell, -1 after the last).
 struct __align__(16) CellRec {
     double x, y, z;
-    int id, cell;
+    int id, next;
 };
 
 // Argmax-y partial of one workgroup's particles (cluster.cpp:128-138: the
@@ -56,6 +57,8 @@ struct CellTab {
     CellRec *rec;   // [ncell * CELL_CAP]
     int *ovf_cnt;   // records in the overflow list
     CellRec *ovf;   // [n] records beyond a cell's capacity
+    int *ovf_head;  // [ncell] each cell's first overflow record (-1: none);
+                    // -1 wherever cnt is 0
     MarkPart *mp;   // argmax-y partials of the appended positions, one per
                     // workgroup of the appending launch
 };
@@ -138,11 +141,14 @@ __device__ __forceinline__ void cell_append(const PartParams &P, CellTab T, int 
     r.y = y;
     r.z = z;
     r.id = id;
-    r.cell = g;
+    r.next = -1;
     if (slot < CELL_CAP) {
         T.rec[(size_t)g * CELL_CAP + slot] = r;
     } else {
+        // pushed on the cell's chain (chain order is arbitrary: every
+        // reader orders records by id)
         const int o = atomicAdd(T.ovf_cnt, 1);
+        r.next = atomicExch(&T.ovf_head[g], o);
         T.ovf[o] = r;
     }
 }
@@ -262,12 +268,16 @@ __global__ __launch_bounds__(256) void k_cells_append(PartParams P, const double
     if (threadIdx.x == 0) T.mp[blockIdx.x] = MarkPart{r.x, r.y, r.z, r.i, 0};
 }
 
-// counters of a table (and optionally of the spread lines) to zero; a
-// kernel, not a memset, so that graph capture records no memset node
-__global__ void k_tab_clear(int *cnt, int n, int *ovf_cnt, int *line_ovf, int *pool_top)
+// counters of a table (and optionally of the spread lines) to zero, and
+// a cell table's overflow chains (head, optional) to empty; a kernel, not a
+// memset, so that graph capture records no memset node
+__global__ void k_tab_clear(int *cnt, int n, int *ovf_cnt, int *line_ovf, int *pool_top, int *head)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int q = i; q < n; q += gridDim.x * blockDim.x) cnt[q] = 0;
+    for (int q = i; q < n; q += gridDim.x * blockDim.x) {
+        cnt[q] = 0;
+        if (head) head[q] = -1;
+    }
     if (i == 0) {
         *ovf_cnt = 0;
         if (line_ovf) *line_ovf = 0;
@@ -332,9 +342,9 @@ __device__ __forceinline__ double heun_velocity(const PartParams &P, double fpi,
 // ---------------------------------------------------------------------------
 // Stage kernel: cell records and contact sums
 // ---------------------------------------------------------------------------
-// Visit the records of global cell co in slot order (the table part, then
-// the overflow records of that cell): f(rec, ref) with ref >= 0 a table
-// slot and ref < 0 overflow entry -1-ref.
+// Visit the records of global cell co: the table part in slot order, then
+// the cell's overflow chain: f(rec, ref) with ref >= 0 a table slot and
+// ref < 0 overflow entry -1-ref.
 template <class F>
 __device__ __forceinline__ void cell_visit(const CellTab &T, int co, F &&f)
 {
@@ -342,10 +352,10 @@ __device__ __forceinline__ void cell_visit(const CellTab &T, int co, F &&f)
     const int mt = m < CELL_CAP ? m : CELL_CAP;
     for (int e = 0; e < mt; e++) f(T.rec[(size_t)co * CELL_CAP + e], co * CELL_CAP + e);
     if (m > CELL_CAP) {
-        const int no = *T.ovf_cnt;
-        for (int k = 0; k < no; k++) {
+        for (int k = T.ovf_head[co]; k >= 0;) {
             const CellRec r = T.ovf[k];
-            if (r.cell == co) f(r, -1 - k);
+            f(r, -1 - k);
+            k = r.next;
         }
     }
 }
@@ -370,8 +380,10 @@ __device__ __forceinline__ void contact_rec(const PartParams &P, double ax, doub
     w[3] = 0.5 * ((f0 * d[0] + f1 * d[1]) + f2 * d[2]);
 }
 
-// Exact reference-order sum by repeated scans over the table (waves whose
-// partner lists overflow the fast path).
+// Exact reference-order sum over the table (tiles whose records overflow
+// the fast path): the partners b < a by repeated scans of the 27 cells for
+// the next id, the partners b > a cell by cell.
+constexpr int SLOW_NC = 8;  // partners in contact collected per cell in one pass
 __device__ void contact_slow_tab(const PartParams &P, const CellTab &T, int a, double ax, double ay, double az,
                                  int o, int pp, int q, int cb, double rmp, bool tau_on, double *fo, double *tau_out)
 {
@@ -408,7 +420,40 @@ __device__ void contact_slow_tab(const PartParams &P, const CellTab &T, int a, d
         for (int j = pp - 1; j <= pp + 1; j++)
             for (int i = o - 1; i <= o + 1; i++) {
                 const int co = cb + wrapc(i, P.ncx) + wrapc(j, P.ncy) * P.ncx + wrapc(k, P.ncz) * P.ncx * P.ncy;
-                // partners b > a of one cell in ascending id (list order)
+                // partners b > a of one cell in ascending id (list order):
+                // one pass collects those in contact, in id order, when
+                // there are at most SLOW_NC; repeated scans otherwise
+                int cid[SLOW_NC], cref[SLOW_NC];
+                int nc = 0;
+                bool more = false;
+                cell_visit(T, co, [&](const CellRec &r, int ref) {
+                    if (r.id <= a || more) return;
+                    double d[3];
+                    pair_disp_pos(P, ax, ay, az, r.x, r.y, r.z, a, r.id, d);
+                    if (!(vnorm(d[0], d[1], d[2]) < P.r_eq)) return;
+                    if (nc == SLOW_NC) {
+                        more = true;
+                        return;
+                    }
+                    int s = nc++;
+                    for (; s > 0 && cid[s - 1] > r.id; s--) {
+                        cid[s] = cid[s - 1];
+                        cref[s] = cref[s - 1];
+                    }
+                    cid[s] = r.id;
+                    cref[s] = ref;
+                });
+                if (!more) {
+                    for (int s = 0; s < nc; s++) {
+                        double w[4];
+                        contact_rec(P, ax, ay, az, a, cell_rec(T, cref[s]), rmp, w);
+                        fx = fx + w[0];
+                        fy = fy + w[1];
+                        fz = fz + w[2];
+                        if (tau_on) tau += w[3];
+                    }
+                    continue;
+                }
                 int lastc = a;
                 for (;;) {
                     int nb = 0x7fffffff;
@@ -625,7 +670,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const CellTab cur = tabs.t[k3], nxt = tabs.t[(k3 + 1) % 3], clr = tabs.t[(k3 + 2) % 3];
     // 0. the table the next stage appends to, emptied (the stage before this
     // one read it); stage 1 also resets the spread lines' shared lists
-    for (int q = wg * 256 + tid; q < ncell; q += nwg * 256) clr.cnt[q] = 0;
+    for (int q = wg * 256 + tid; q < ncell; q += nwg * 256) {
+        clr.cnt[q] = 0;
+        clr.ovf_head[q] = -1;
+    }
     if (wg == 0 && tid == 0) {
         *clr.ovf_cnt = 0;
         if (STAGE == 1 && lines_on) {

@@ -247,6 +247,23 @@ def test_crowded_cells_32_64(gpu, n, npart):
         assert rel_l2(gp["f_pp"], rp["f_pp"]) < 1e-8, (t, rel_l2(gp["f_pp"], rp["f_pp"]))
 
 
+def test_heavily_crowded_cells_32(gpu):
+    """8,000 spheres of radius 0.5 in 6-dx contact cells: 64 a cell on
+    average, four times the table's 16 slots, so every tile takes the exact
+    path and most records sit in the cells' overflow chains.  FP64 against
+    the oracle every two steps."""
+    print("heavily crowded cells 32^3 + 8000, 6-dx cells", flush=True)
+    n, npart = 32, 8000
+    p, phi0, pos = make_case(n, npart, r_p=0.5, steps=20, seed=47, gap=-0.3)
+    for t, g_phi, r_phi, g_u, r_u, gp, rp in _run_both(p, phi0, pos, 4, chunks=2):
+        print(f"t={t}", flush=True)
+        assert rel_l2(g_phi, r_phi) < TOL, (t, rel_l2(g_phi, r_phi))
+        assert rel_l2(g_u, r_u) < TOL, (t, rel_l2(g_u, r_u))
+        assert rel_l2(gp["p"], rp["p"]) < TOL, (t, rel_l2(gp["p"], rp["p"]))
+        assert rel_l2(gp["f_pi"], rp["f_pi"]) < 1e-8, (t, rel_l2(gp["f_pi"], rp["f_pi"]))
+        assert rel_l2(gp["f_pp"], rp["f_pp"]) < 1e-8, (t, rel_l2(gp["f_pp"], rp["f_pp"]))
+
+
 def test_crowded_spread_lines_32(gpu):
     """8,000 small spheres (r = 0.5) in a 32^3 box: about 31 spread entries
     per grid line on average, so that many lines exceed the line table's 32
