@@ -1078,6 +1078,8 @@ int setup_buffers(fipi_ctx *ctx)
         ALLOC(lt.ent, sizeof(LineEnt<T>) * LINE_CAP * (size_t)ctx->nlines);
         ALLOC(lt.ovf_cnt, sizeof(int));
         ALLOC(lt.ovf, sizeof(int2) * 4 * n);
+        ALLOC(lt.ovf_head, sizeof(int) * (ctx->nlines + 1));
+        HIPCHK(ctx, hipMemset(lt.ovf_head, 0xff, sizeof(int) * (ctx->nlines + 1)));  // -1: no chain
         lt.node_base = ctx->line_start;
         lt.node_cnt = ctx->node_cnt;
         ALLOC(lt.pool_top, sizeof(int));

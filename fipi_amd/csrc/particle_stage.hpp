@@ -126,7 +126,10 @@ struct LineTab {
     int *cnt;        // [nlines] particles appended (may exceed LINE_CAP)
     void *ent;       // [nlines * LINE_CAP] LineEnt<T>
     int *ovf_cnt;    // entries in the overflow list
-    int2 *ovf;       // [4n] (line, particle) beyond a line's capacity
+    int2 *ovf;       // [4n] (particle, next) beyond a line's capacity: each
+                     // line's chain, next = -1 after the last
+    int *ovf_head;   // [nlines] first overflow entry of each line (-1: none);
+                     // -1 wherever cnt is 0
     int *node_base;  // [nlines] first node of each line in the pool
     int *node_cnt;   // [nlines]
     int *pool_top;   // nodes allocated
@@ -237,7 +240,7 @@ __device__ __forceinline__ void line_append4(const PartParams &P, LineTab Lt, co
             ent[(size_t)line * LINE_CAP + slot] = e;
         } else {
             const int o = atomicAdd(Lt.ovf_cnt, 1);
-            Lt.ovf[o] = make_int2(line, id);
+            Lt.ovf[o] = make_int2(id, atomicExch(&Lt.ovf_head[line], o));
         }
     }
 }
@@ -1164,11 +1167,13 @@ __global__ __launch_bounds__(256) void k_spread_nodes(PartParams P, LineTab lt, 
     // node, the particles in ascending id (table part and overflow list)
     if (lane != 0) return;
     const int tail = ((nlines + NODE_LPB - 1) / NODE_LPB) * NODE_REGION;
-    const int no = *lt.ovf_cnt;
     auto visit = [&](auto &&f) {
         for (int k = 0; k < LINE_CAP; k++) f(reinterpret_cast<const LineEnt<T> *>(lt.ent)[(size_t)line * LINE_CAP + k].id);
-        for (int k = 0; k < no; k++)
-            if (lt.ovf[k].x == line) f(lt.ovf[k].y);
+        for (int k = lt.ovf_head[line]; k >= 0;) {
+            const int2 e = lt.ovf[k];
+            f(e.x);
+            k = e.y;
+        }
     };
     int nh = 0;
     for (int pass = 0; pass < 2; pass++) {
@@ -1224,6 +1229,7 @@ __global__ __launch_bounds__(256) void k_spread_nodes(PartParams P, LineTab lt, 
         }
     }
     lt.cnt[line] = 0;
+    lt.ovf_head[line] = -1;
 }
 
 // The id-indexed particle arrays from the step's blocks: p_old and u_old

@@ -282,3 +282,21 @@ def test_crowded_spread_lines_32(gpu):
         assert rel_l2(gp["f_pi"], rp["f_pi"]) < 1e-8, (t, rel_l2(gp["f_pi"], rp["f_pi"]))
         assert rel_l2(gp["f_pp"], rp["f_pp"]) < 1e-8, (t, rel_l2(gp["f_pp"], rp["f_pp"]))
 
+
+def test_heavily_crowded_spread_lines_32(gpu):
+    """16,000 spheres of radius 0.5 in 32^3 (2-dx contact cells): 62 spread
+    entries per grid line on average and at least 43 on every line, so every
+    line's spread takes the exact path over its overflow chain (31,000
+    entries in the overflow list).  FP64 against the oracle every two
+    steps."""
+    print("heavily crowded spread lines 32^3 + 16000", flush=True)
+    n, npart = 32, 16000
+    p, phi0, pos = make_case(n, npart, r_p=0.5, steps=20, seed=53, gap=-0.3, cell=2.0)
+    for t, g_phi, r_phi, g_u, r_u, gp, rp in _run_both(p, phi0, pos, 4, chunks=2):
+        print(f"t={t}", flush=True)
+        assert rel_l2(g_phi, r_phi) < TOL, (t, rel_l2(g_phi, r_phi))
+        assert rel_l2(g_u, r_u) < TOL, (t, rel_l2(g_u, r_u))
+        assert rel_l2(gp["p"], rp["p"]) < TOL, (t, rel_l2(gp["p"], rp["p"]))
+        assert rel_l2(gp["f_pi"], rp["f_pi"]) < 1e-8, (t, rel_l2(gp["f_pi"], rp["f_pi"]))
+        assert rel_l2(gp["f_pp"], rp["f_pp"]) < 1e-8, (t, rel_l2(gp["f_pp"], rp["f_pp"]))
+
