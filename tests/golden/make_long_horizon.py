@@ -102,9 +102,11 @@ def main():
 
     def save(t_done):
         # everything recorded through step t_done (rewritten at every
-        # position checkpoint, so a long run leaves a usable fixture)
+        # position checkpoint, so a long run leaves a usable fixture);
+        # run_steps: the run length the workload was made for (the contact
+        # ramp spans 0.2 of it)
         np.savez_compressed(
-            out, n=n, particles=npart, steps=t_done, sample_idx=idx, pos_idx=pidx.astype(np.int32),
+            out, n=n, particles=npart, steps=t_done, run_steps=steps, sample_idx=idx, pos_idx=pidx.astype(np.int32),
             ab_max_disp=ab_maxdp[:t_done], ab_max_sphere=ab_maxid[:t_done], ab_pos_rel=ab_pos_rel[:t_done],
             ck_steps=np.array(ck_steps, np.int32), ck_phi=np.array(ck_phi), ck_u=np.array(ck_u),
             ck_phi_norm=np.array(ck_phin), ck_u_norm=np.array(ck_un), ck_ab_phi=np.array(ck_ab_phi),

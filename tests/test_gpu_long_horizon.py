@@ -33,8 +33,11 @@ def _fixture(n):
 def test_fp64_1000_steps_against_oracle_fixture(gpu, n):
     fx = _fixture(n)
     npart, steps = int(fx["particles"]), int(fx["steps"])
-    print(f"long horizon {n}^3 + {npart}, {steps} steps", flush=True)
-    p = fipi_amd.config_params(n, npart, 2.0, steps)
+    # the workload of a run of run_steps (its contact ramp spans 0.2 of it),
+    # checked through the steps the fixture holds
+    run_steps = int(fx["run_steps"]) if "run_steps" in fx.files else steps
+    print(f"long horizon {n}^3 + {npart}, {steps} of {run_steps} steps", flush=True)
+    p = fipi_amd.config_params(n, npart, 2.0, run_steps)
     phi0 = fipi_amd.init_phi((n, n, n), 0.0, 0.05, 1)
     pos = fipi_amd.init_rsa(npart, 2.0, 0.5, (p.lx, p.ly, p.lz), 2)
     assert pos.shape[0] == npart
