@@ -90,7 +90,8 @@ def test_fp32_mixed_against_oracle_fixture(gpu, n):
     tolerance), u within 1e-3 at the sample points (the tolerance names no
     bar for u; FP32 transforms of the small flow field leave ~1e-4), and no
     sphere more than 1e-3 dx from the oracle's position at the position
-    checkpoints."""
+    checkpoints, plus 100 times the largest displacement between the two
+    oracle runs there (a sphere that two FP64 runs already part on)."""
     fx = _fixture(n)
     npart, steps = int(fx["particles"]), int(fx["steps"])
     run_steps = int(fx["run_steps"]) if "run_steps" in fx.files else steps
@@ -119,8 +120,9 @@ def test_fp32_mixed_against_oracle_fixture(gpu, n):
             d = sim.particles()["p"][pidx] - fx["pos"][pos_at[t]]
             d -= L * np.round(d / L)
             dmax = float(np.abs(d).max())
-            msg += f" max |dp| {dmax:.2e} dx"
-            assert dmax < 1e-3, msg
+            ab_max = float(fx["ab_max_disp"][t - 1])
+            msg += f" max |dp| {dmax:.2e} dx (two oracles {ab_max:.2e})"
+            assert dmax < 1e-3 + 100 * ab_max, msg
         print(msg, flush=True)
         assert e_phi < 1e-5 and n_phi < 1e-5 and e_u < 1e-3, msg
     sim.close()
