@@ -1,16 +1,19 @@
-"""FP64 and FP32-mixed over 1000 steps against oracle fixtures (tests/golden/long_*.npz,
-written by tests/golden/make_long_horizon.py in this repository).
+"""FP64 and FP32-mixed over 1000 steps against oracle fixtures
+(tests/golden/long_*.npz, written by tests/golden/make_long_horizon.py in
+this repository).
 
-The fixture holds, every 50 (128^3) or 100 (256^3) steps, phi and u at seeded grid points and
-their global L2 norms from the oracle, the positions of the spheres at a
-few checkpoints, and the divergence of two oracle runs that differ only in
-their FFT factor order (radix 4 + 2 against radix 2).  phi and u must agree
-at 1e-10 throughout.  Positions must agree at 1e-10 up to the step where
-the two oracle runs still agree to 1e-12; beyond it (a sphere on a
-vanishing grad c turns its capillary force with the round-off, and any two
-FP64 implementations part there) the GPU may differ from the oracle by no
-more than 100 times what the two oracle runs differ by.  The FP32-mixed
-path is held to the FP32-mixed tolerance against the same fixtures.
+The fixture holds, every 50 (128^3) or 100 (256^3) steps, phi and u at
+seeded grid points and their global L2 norms from the oracle, the
+positions of the spheres at a few checkpoints, and the divergence of two
+oracle runs that differ only in their FFT factor order (radix 4 + 2
+against radix 2).  Up to the step where the two oracle runs' positions
+still agree to 1e-12, phi and u must agree at 1e-10 and positions at
+1e-10.  Beyond it (a sphere on a vanishing grad c turns its capillary
+force with the round-off, and any two FP64 implementations part there;
+where such a sphere moves far enough, its force reaches phi and u) the GPU
+may differ from the oracle by no more than 100 times what the two oracle
+runs differ by (plus the bars above).  The FP32-mixed path is held to the
+FP32-mixed tolerance against the same fixtures.
 """
 import os
 
@@ -63,7 +66,10 @@ def test_fp64_1000_steps_against_oracle_fixture(gpu, n):
         e_u = np.linalg.norm(u[:, idx] - ru) / np.linalg.norm(ru)
         n_phi = abs(np.linalg.norm(phi) / fx["ck_phi_norm"][k] - 1)
         n_u = abs(np.linalg.norm(u) / fx["ck_u_norm"][k] - 1)
-        msg = f"t={t}: phi {e_phi:.2e} (norm {n_phi:.1e}) u {e_u:.2e} (norm {n_u:.1e})"
+        # the two oracle runs' own difference in phi and u at this checkpoint
+        ab_phi, ab_u = float(fx["ck_ab_phi"][k]), float(fx["ck_ab_u"][k])
+        msg = (f"t={t}: phi {e_phi:.2e} (norm {n_phi:.1e}) u {e_u:.2e} (norm {n_u:.1e}) "
+               f"[two oracles: phi {ab_phi:.1e} u {ab_u:.1e}]")
         if t in pos_at:
             gp = sim.particles()["p"][pidx]
             rp = fx["pos"][pos_at[t]]
@@ -76,8 +82,13 @@ def test_fp64_1000_steps_against_oracle_fixture(gpu, n):
             else:
                 assert e_p < 100 * ab[t - 1] + 1e-10, msg
         print(msg, flush=True)
-        assert e_phi < 1e-10 and e_u < 1e-10, msg
-        assert n_phi < 1e-12 and n_u < 1e-12, msg
+        if t <= horizon:
+            assert e_phi < 1e-10 and e_u < 1e-10, msg
+            assert n_phi < 1e-12 and n_u < 1e-12, msg
+        else:
+            # past the horizon a parted sphere's force reaches the flow
+            assert e_phi < 1e-10 + 100 * ab_phi and e_u < 1e-10 + 100 * ab_u, msg
+            assert n_phi < 1e-12 + 100 * ab_phi and n_u < 1e-12 + 100 * ab_u, msg
     sim.close()
     print(f"oracle orders agree to 1e-12 through step {horizon}", flush=True)
 
